@@ -1,0 +1,203 @@
+/*
+ * lsd.h -- C-ABI of the MI355X-native per-timestep path of the Leeds
+ * Spherical Dynamo code (Boussinesq, insulating inner core, Nic = 1).
+ *
+ * Every entry point takes plain pointers and sizes, returns 0 on success and
+ * a negative LSD_E* code on failure; lsd_last_error() then holds the message
+ * (the reference stops the program with the same text, e.g.
+ * modules/meshs.F90:454 'mes_lu_find', modules/timestep.F90:582 'tim_invX.1').
+ *
+ * Layouts (all FP64 on the host side, identical to the reference's types,
+ * modules/variables.F90:65-86):
+ *   coll  : re[h*N + n], im[h*N + n]        Fortran Re(N, 0:H-1)
+ *   spec  : re[n*H + h], im[n*H + h]        Fortran Re(0:H-1, N)
+ *   phys  : p[(n*Ph + k)*Th + j]            Fortran Re(Th, 0:Ph-1, N)
+ * with harmonics h in the reference's folded order
+ * (Scripts/parallel.h_template:65-80): column c = 0..M/2-1 holds m = c
+ * (l = m..L-1) followed by m' = M-1-c (l = m'..L-1).
+ * Theta points j ascend in x = cos(theta) from -1 (modules/legendre.F90:73).
+ *
+ * One context = one HIP device + one stream; all fields stay resident in
+ * HBM between calls.  Contexts are not shared between host threads.
+ */
+#ifndef LSD_H
+#define LSD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LSD_OK            0
+#define LSD_E_CONFIG     -1   /* parameter check failed (par_precompute)        */
+#define LSD_E_HIP        -2   /* a HIP runtime call failed                        */
+#define LSD_E_SINGULAR   -3   /* zero pivot in a banded LU (dgbtrf info /= 0)     */
+#define LSD_E_ARG        -4   /* bad argument (null pointer, unknown id)          */
+#define LSD_E_DIVERGED   -5   /* corrector loop gave up (tim_check_cgce)          */
+
+/* Precision of the step.  PLAIN keeps every operation in FP64 and is the
+ * drop-in; the mixed forms keep the state, the P_l^m recursion and the LU
+ * factors in FP64. */
+#define LSD_PLAIN_F64     0
+#define LSD_MIXED_F32     1   /* FP32 transforms + FP32 solves, FP64 refinement   */
+#define LSD_MIXED_BF16X3  2   /* as F32, Legendre on bf16x3 split MFMA            */
+
+/* Run parameters; names follow Scripts/parameters.F90_template:19-264. */
+typedef struct lsd_config {
+    int L;          /* i_L  : l in [0, L)                                       */
+    int M;          /* i_M  : m index in [0, M), must be even                  */
+    int Mp;         /* i_Mp : azimuthal periodicity                            */
+    int N;          /* i_N  : radial points                                     */
+    int Th;         /* theta points; 0 selects the reference rule 3L/2          */
+    int Ph;         /* phi points; 0 selects the reference rule 3M              */
+    int vel, cod, comp, mag;        /* b_*_tstep; vel must be 1                  */
+    int vel_bc, cod_bc, comp_bc;    /* i_vel_bc, i_cod_bc, i_comp_bc (1 only)    */
+    int precision;                  /* LSD_PLAIN_F64 / LSD_MIXED_*               */
+    double rratio;  /* d_rratio                                                 */
+    double Ra;      /* d_Ra                                                     */
+    double Ra_comp; /* d_Ra_comp                                                */
+    double Pr;      /* d_Pr                                                     */
+    double Sc;      /* d_Sc                                                     */
+    double Pm;      /* d_Pm                                                     */
+    double E;       /* d_E                                                      */
+    double dt;      /* d_timestep (> 0: fixed step)                             */
+    double implicit;/* d_implicit                                               */
+    double dterr;   /* d_dterr                                                  */
+    double source;  /* d_source                                                 */
+    double source_comp; /* d_source_comp                                        */
+} lsd_config;
+
+/* Coll-field identifiers. */
+#define LSD_VEL_TOR   0
+#define LSD_VEL_POL   1
+#define LSD_COD       2
+#define LSD_COMP      3
+#define LSD_MAG_TOR   4
+#define LSD_MAG_POL   5
+#define LSD_N_VEL_TOR 6    /* nonlinear terms of the last evaluation           */
+#define LSD_N_VEL_POL 7
+#define LSD_N_COD     8
+#define LSD_N_COMP    9
+#define LSD_N_MAG_TOR 10
+#define LSD_N_MAG_POL 11
+#define LSD_NUM_COLL  12
+
+/* Physical-field identifiers (main.F90:36-95 names). */
+#define LSD_P_VEL_R    0
+#define LSD_P_VEL_T    1
+#define LSD_P_VEL_P    2
+#define LSD_P_VCURL_R  3
+#define LSD_P_VCURL_T  4
+#define LSD_P_VCURL_P  5
+#define LSD_P_MAG_R    6
+#define LSD_P_MAG_T    7
+#define LSD_P_MAG_P    8
+#define LSD_P_MCURL_R  9
+#define LSD_P_MCURL_T  10
+#define LSD_P_MCURL_P  11
+#define LSD_P_COD_GR   12
+#define LSD_P_COD_GT   13
+#define LSD_P_COD_GP   14
+#define LSD_P_COMP_GR  15
+#define LSD_P_COMP_GT  16
+#define LSD_P_COMP_GP  17
+#define LSD_NUM_PHYS   18
+
+/* Legendre / quadrature tables (modules/legendre.F90:41-56). */
+#define LSD_T_GP         0   /* leg_gp(Th)                                      */
+#define LSD_T_GW         1   /* leg_gw(Th)                                      */
+#define LSD_T_P          2   /* leg_P(Th2, 0:H-1), j fastest                    */
+#define LSD_T_P_         3
+#define LSD_T_SIN1MP     4
+#define LSD_T_LLP_       5
+#define LSD_T_LLSIN1MP   6
+#define LSD_T_GWP        7
+#define LSD_T_GWLLP_     8
+#define LSD_T_GWLLSIN1MP 9
+
+/* Radial mesh arrays (modules/meshs.F90:74-103); band matrices are returned
+ * in the reference storage M(KL+1+n-j, j), i.e. out[(j-1)*7 + (3+n-j)]. */
+#define LSD_M_R        0     /* r(N)                                            */
+#define LSD_M_DR1      1     /* dr(1)%M (7, N)                                  */
+#define LSD_M_DR2      2     /* dr(2)%M                                         */
+#define LSD_M_RADLAP   3     /* radLap%M                                        */
+#define LSD_M_R1DR     4     /* r1dr%M                                          */
+#define LSD_M_INTR2DR  5     /* intr2dr(N)                                      */
+
+typedef struct lsd_step_stats {
+    int    steps;          /* completed timesteps                               */
+    int    evaluations;    /* nonlinear evaluations performed                   */
+    int    last_its;       /* corrector iterations of the last step             */
+    int    max_its;        /* max corrector iterations over the call            */
+    double last_dterr;     /* tim_dterr of the last accepted corrector          */
+    double t;              /* tim_t after the call                              */
+} lsd_step_stats;
+
+typedef struct lsd_ctx lsd_ctx;
+
+/* Last error message of the calling thread ("" if none). */
+const char* lsd_last_error(void);
+
+/* Harmonic count H = L*M - Mp*(M-1)*M/2 (Scripts/parameters.F90_template:255)
+ * and the resolved Th, Ph of a configuration; returns LSD_E_CONFIG on a
+ * failed par_precompute check (Scripts/parameters.F90_template:274-288). */
+int lsd_config_sizes(const lsd_config* cfg, int* H, int* Th, int* Ph);
+
+/* Replaces program/main.F90:initialise (main.F90:220-400) minus I/O:
+ * mes_precompute, leg_precompute (FP64 recursion on the GPU), var_precompute,
+ * tra_precompute, tim_precompute, *_precompute and *_matrices at cfg->dt.
+ * All fields start at zero. */
+int lsd_create(const lsd_config* cfg, int device, lsd_ctx** out);
+void lsd_destroy(lsd_ctx* ctx);
+
+/* Copy a coll field in (reference layout, host memory).  Replaces the field
+ * fill of io_load_state / io_load_coll (program/io.F90:349-483,1278-1355). */
+int lsd_set_coll(lsd_ctx* ctx, int field, const double* re, const double* im);
+int lsd_get_coll(lsd_ctx* ctx, int field, double* re, double* im);
+/* Physical field of the last lsd_transform (reference phys layout). */
+int lsd_get_phys(lsd_ctx* ctx, int field, double* p);
+
+/* Tables and mesh arrays as built on the device, for checking. */
+int lsd_get_table(lsd_ctx* ctx, int table, double* out);
+int lsd_get_mesh(lsd_ctx* ctx, int which, double* out);
+
+/* The four phases of one pass of the main loop (program/main.F90:101-157):
+ *  lsd_transform   vel_transform, cod_transform, comp_transform, mag_transform
+ *  lsd_nonlinear   non_velocity, non_codensity, non_composition, non_magnetic
+ *  lsd_predictor   *_predictor (tim_it == 0)
+ *  lsd_corrector   *_corrector; *dterr receives tim_dterr (tim_addcorr sums) */
+int lsd_transform(lsd_ctx* ctx);
+int lsd_nonlinear(lsd_ctx* ctx);
+int lsd_predictor(lsd_ctx* ctx);
+int lsd_corrector(lsd_ctx* ctx, double* dterr);
+
+/* nsteps full timesteps of program/main.F90:100-164 with fixed dt: one
+ * predictor and correctors until tim_check_cgce (modules/timestep.F90:791-827,
+ * i_time_step_control = 2) accepts. */
+int lsd_step(lsd_ctx* ctx, int nsteps, lsd_step_stats* stats);
+
+/* Refactorise all banded operators for a new dt (vel_matrices, cod_matrices,
+ * comp_matrices, mag_matrices; program/main.F90:126-135). */
+int lsd_set_dt(lsd_ctx* ctx, double dt);
+
+/* Drop-in transforms with host arrays in the reference spec/phys layouts
+ * (modules/transform.fftw3.F90:472,527,719,825,955). */
+int lsd_tra_spec2phys(lsd_ctx* ctx, const double* s_re, const double* s_im, double* p);
+int lsd_tra_phys2spec(lsd_ctx* ctx, const double* p, double* s_re, double* s_im);
+int lsd_tra_grad(lsd_ctx* ctx, const double* s_re, const double* s_im, double* t, double* p);
+int lsd_tra_qst2rtp(lsd_ctx* ctx,
+                    const double* q_re, const double* q_im,
+                    const double* s_re, const double* s_im,
+                    const double* t_re, const double* t_im,
+                    double* r, double* th, double* ph);
+int lsd_tra_rtp2qst(lsd_ctx* ctx, const double* r, const double* th, const double* ph,
+                    double* q_re, double* q_im, double* s_re, double* s_im,
+                    double* t_re, double* t_im);
+
+/* Wait for all work queued on the context's stream (get_time bracket,
+ * modules/dyn_mpi.F90:81-92). */
+int lsd_sync(lsd_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LSD_H */

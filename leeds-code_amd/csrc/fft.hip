@@ -1,0 +1,277 @@
+// Phi FFTs staged through LDS (modules/transform.fftw3.F90:280-466).
+//
+// Inverse (tra_s2p): each workgroup takes one (field, shell) and 2*NF theta
+// rows; rows (j, j+1) are packed into one complex sequence
+// Z = A + iB with Hermitian-extended spectra A, B (zero for m >= M, the
+// imaginary parts of the DC terms ignored as FFTW's c2r does), transformed
+// with an in-LDS Stockham FFT (radices 8, 4, 2, 3), and the real and
+// imaginary parts are the two physical rows.
+//
+// Forward (tra_p2s + the even/odd fold of tra_rtp2qst, :862-864): rows j and
+// Th-1-j (its mirror) are packed, scaled by 1/Ph as the reference does
+// before r2c, transformed, separated, and written as
+// G0 = X_j + X_mirror, G1 = X_j - X_mirror for m < M.
+#include "lsd_internal.h"
+
+#include <hip/hip_runtime.h>
+
+namespace lsd {
+
+struct FftPlan {
+    int n;        // transform length Ph
+    int nstage;
+    int radix[16];
+};
+
+constexpr int FFT_THREADS = 256;
+constexpr int FFT_EPT = 16;  // complex elements per thread per stage
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+// multiply by sign*i
+__device__ __forceinline__ double2 mul_si(double2 a, double sg) { return make_double2(-sg * a.y, sg * a.x); }
+
+template <int R>
+__device__ __forceinline__ void dft(double2* v, double sg) {
+    if (R == 2) {
+        double2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if (R == 3) {
+        const double h = 0.86602540378443864676;  // sqrt(3)/2
+        double2 t = cadd(v[1], v[2]);
+        double2 d = csub(v[1], v[2]);
+        double2 m = make_double2(v[0].x - 0.5 * t.x, v[0].y - 0.5 * t.y);
+        double2 e = mul_si(make_double2(h * d.x, h * d.y), sg);
+        v[0] = cadd(v[0], t);
+        v[1] = cadd(m, e);
+        v[2] = csub(m, e);
+    } else if (R == 4) {
+        double2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+        double2 t2 = cadd(v[1], v[3]), t3 = mul_si(csub(v[1], v[3]), sg);
+        v[0] = cadd(t0, t2);
+        v[2] = csub(t0, t2);
+        v[1] = cadd(t1, t3);
+        v[3] = csub(t1, t3);
+    } else if (R == 8) {
+        const double c = 0.70710678118654752440;
+        double2 e[4] = {v[0], v[2], v[4], v[6]};
+        double2 o[4] = {v[1], v[3], v[5], v[7]};
+        dft<4>(e, sg);
+        dft<4>(o, sg);
+        // w^k, w = exp(sg*2*pi*i/8)
+        double2 w1 = make_double2(c * (o[1].x - sg * o[1].y), c * (o[1].y + sg * o[1].x));
+        double2 w2 = mul_si(o[2], sg);
+        double2 w3 = make_double2(c * (-o[3].x - sg * o[3].y), c * (-o[3].y + sg * o[3].x));
+        v[0] = cadd(e[0], o[0]);
+        v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], w1);
+        v[5] = csub(e[1], w1);
+        v[2] = cadd(e[2], w2);
+        v[6] = csub(e[2], w2);
+        v[3] = cadd(e[3], w3);
+        v[7] = csub(e[3], w3);
+    }
+}
+
+// One Stockham stage over nf sequences of length n held in Z[p*n + i].
+template <int R>
+__device__ void stage(double2* Z, int n, int nf, int Ns, const double2* __restrict__ tw, double sg) {
+    constexpr int BPT = (FFT_EPT + R - 1) / R;
+    const int nb = n / R;
+    const int total = nf * nb;
+    const int step = n / (Ns * R);
+    double2 v[BPT][R];
+#pragma unroll
+    for (int q = 0; q < BPT; q++) {
+        int b = threadIdx.x + q * FFT_THREADS;
+        if (b < total) {
+            int p = b / nb, j = b - p * nb;
+            int jm = j % Ns;
+            const double2* src = Z + (size_t)p * n + j;
+#pragma unroll
+            for (int r = 0; r < R; r++) v[q][r] = src[r * nb];
+            if (jm) {
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    double2 w = tw[jm * r * step];
+                    w.y *= sg;
+                    v[q][r] = cmul(v[q][r], w);
+                }
+            }
+            dft<R>(v[q], sg);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < BPT; q++) {
+        int b = threadIdx.x + q * FFT_THREADS;
+        if (b < total) {
+            int p = b / nb, j = b - p * nb;
+            int jm = j % Ns;
+            double2* dst = Z + (size_t)p * n + (j / Ns) * Ns * R + jm;
+#pragma unroll
+            for (int r = 0; r < R; r++) dst[r * Ns] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+__device__ void fft_lds(double2* Z, const FftPlan& pl, int nf, const double2* __restrict__ tw, double sg) {
+    int Ns = 1;
+    for (int s = 0; s < pl.nstage; s++) {
+        int R = pl.radix[s];
+        switch (R) {
+        case 8: stage<8>(Z, pl.n, nf, Ns, tw, sg); break;
+        case 4: stage<4>(Z, pl.n, nf, Ns, tw, sg); break;
+        case 2: stage<2>(Z, pl.n, nf, Ns, tw, sg); break;
+        default: stage<3>(Z, pl.n, nf, Ns, tw, sg); break;
+        }
+        Ns *= R;
+    }
+}
+
+// fourier [f][n][M][Th] -> phys [f][n][Ph][Th]; grid (row blocks, N, fields).
+__global__ void __launch_bounds__(FFT_THREADS) k_fft_c2r(FftPlan pl, int nf, int N, int Th, int M,
+                                                         const double2* __restrict__ fourier,
+                                                         double* __restrict__ phys, const double2* __restrict__ tw) {
+    extern __shared__ double2 Z[];
+    const int n = pl.n;
+    const int f = blockIdx.z, sh = blockIdx.y;
+    const int j0 = blockIdx.x * 2 * nf;
+    const double2* X = fourier + ((size_t)f * N + sh) * (size_t)M * Th;
+    for (int i = threadIdx.x; i < nf * n; i += FFT_THREADS) Z[i] = make_double2(0.0, 0.0);
+    __syncthreads();
+    for (int t = threadIdx.x; t < M * nf; t += FFT_THREADS) {
+        int k = t / nf, p = t - k * nf;
+        int ja = j0 + 2 * p, jb = ja + 1;
+        double2 a = ja < Th ? X[(size_t)k * Th + ja] : make_double2(0.0, 0.0);
+        double2 b = jb < Th ? X[(size_t)k * Th + jb] : make_double2(0.0, 0.0);
+        double2* z = Z + (size_t)p * n;
+        if (k == 0) {
+            z[0] = make_double2(a.x, b.x);
+        } else {
+            z[k] = make_double2(a.x - b.y, a.y + b.x);
+            z[n - k] = make_double2(a.x + b.y, b.x - a.y);
+        }
+    }
+    __syncthreads();
+    fft_lds(Z, pl, nf, tw, 1.0);
+    double* P = phys + ((size_t)f * N + sh) * (size_t)n * Th;
+    const int rows = 2 * nf;
+    for (int t = threadIdx.x; t < n * rows; t += FFT_THREADS) {
+        int k = t / rows, r = t - k * rows;
+        int j = j0 + r;
+        if (j < Th) {
+            double2 z = Z[(size_t)(r >> 1) * n + k];
+            P[(size_t)k * Th + j] = (r & 1) ? z.y : z.x;
+        }
+    }
+}
+
+// phys [f][n][Ph][Th] -> folded [f][n][M][2][Th2]; grid (row blocks, N, fields).
+__global__ void __launch_bounds__(FFT_THREADS) k_fft_r2c_fold(FftPlan pl, int nf, int N, int Th, int Th2, int M,
+                                                              const double* __restrict__ phys,
+                                                              double2* __restrict__ folded,
+                                                              const double2* __restrict__ tw) {
+    extern __shared__ double2 Z[];
+    const int n = pl.n;
+    const int f = blockIdx.z, sh = blockIdx.y;
+    const int j0 = blockIdx.x * nf;
+    const double scale = 1.0 / (double)n;
+    const double* P = phys + ((size_t)f * N + sh) * (size_t)n * Th;
+    for (int t = threadIdx.x; t < n * nf; t += FFT_THREADS) {
+        int k = t / nf, p = t - k * nf;
+        int j = j0 + p;
+        double a = 0.0, b = 0.0;
+        if (j < Th2) {
+            a = P[(size_t)k * Th + j] * scale;
+            b = P[(size_t)k * Th + (Th - 1 - j)] * scale;
+        }
+        Z[(size_t)p * n + k] = make_double2(a, b);
+    }
+    __syncthreads();
+    fft_lds(Z, pl, nf, tw, -1.0);
+    double2* G = folded + ((size_t)f * N + sh) * (size_t)M * 2 * Th2;
+    for (int t = threadIdx.x; t < M * 2 * nf; t += FFT_THREADS) {
+        int k = t / (2 * nf), rem = t - k * 2 * nf;
+        int par = rem / nf, p = rem - par * nf;
+        int j = j0 + p;
+        if (j >= Th2) continue;
+        double2 zk = Z[(size_t)p * n + k];
+        double2 zc = Z[(size_t)p * n + (k ? n - k : 0)];
+        zc.y = -zc.y;
+        double2 A = make_double2(0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y));
+        double2 D = csub(zk, zc);
+        double2 B = make_double2(0.5 * D.y, -0.5 * D.x);
+        double2 out = par ? csub(A, B) : cadd(A, B);
+        G[((size_t)k * 2 + par) * Th2 + j] = out;
+    }
+}
+
+static FftPlan make_plan(int n) {
+    FftPlan p{};
+    p.n = n;
+    int rem = n;
+    while (rem % 8 == 0) { p.radix[p.nstage++] = 8; rem /= 8; }
+    while (rem % 4 == 0) { p.radix[p.nstage++] = 4; rem /= 4; }
+    while (rem % 2 == 0) { p.radix[p.nstage++] = 2; rem /= 2; }
+    while (rem % 3 == 0) { p.radix[p.nstage++] = 3; rem /= 3; }
+    if (rem != 1) p.nstage = -1;
+    return p;
+}
+
+// Twiddle table exp(+2 pi i k / Ph), k < Ph, built on the device.
+__global__ void k_twiddle(int n, double2* __restrict__ tw) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double s, c;
+    sincospi(2.0 * (double)k / (double)n, &s, &c);
+    tw[k] = make_double2(c, s);
+}
+
+int fft_setup(Ctx& c) {
+    FftPlan p = make_plan(c.Ph);
+    if (p.nstage < 0) {
+        set_error("Ph must be a product of 2s and 3s");
+        return LSD_E_CONFIG;
+    }
+    LSD_HIP(hipMalloc((void**)&c.d_tw, sizeof(double2) * c.Ph));
+    hipLaunchKernelGGL(k_twiddle, dim3((c.Ph + 255) / 256), dim3(256), 0, c.stream, c.Ph, c.d_tw);
+    LSD_HIP(hipGetLastError());
+    return LSD_OK;
+}
+
+static int nfft_per_block(const Ctx& c, int rows_needed) {
+    int nf = (FFT_EPT * FFT_THREADS) / c.Ph;
+    if (nf < 1) nf = 1;
+    if (nf > rows_needed) nf = rows_needed;
+    return nf;
+}
+
+int launch_fft_c2r(Ctx& c, const double2* fourier, double* phys, int nfield) {
+    FftPlan p = make_plan(c.Ph);
+    int nf = nfft_per_block(c, (c.Th + 1) / 2);
+    dim3 grid((c.Th + 2 * nf - 1) / (2 * nf), c.N, nfield);
+    size_t lds = sizeof(double2) * nf * c.Ph;
+    hipLaunchKernelGGL(k_fft_c2r, grid, dim3(FFT_THREADS), lds, c.stream, p, nf, c.N, c.Th, c.M, fourier, phys,
+                       c.d_tw);
+    LSD_HIP(hipGetLastError());
+    return LSD_OK;
+}
+
+int launch_fft_r2c_fold(Ctx& c, const double* phys, double2* folded, int nfield) {
+    FftPlan p = make_plan(c.Ph);
+    int nf = nfft_per_block(c, c.Th2);
+    dim3 grid((c.Th2 + nf - 1) / nf, c.N, nfield);
+    size_t lds = sizeof(double2) * nf * c.Ph;
+    hipLaunchKernelGGL(k_fft_r2c_fold, grid, dim3(FFT_THREADS), lds, c.stream, p, nf, c.N, c.Th, c.Th2, c.M, phys,
+                       folded, c.d_tw);
+    LSD_HIP(hipGetLastError());
+    return LSD_OK;
+}
+
+}  // namespace lsd

@@ -1,0 +1,619 @@
+// C-ABI entry points (include/lsd.h): context life cycle, field I/O and the
+// phases of the reference's main loop (program/main.F90:100-164).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lsd_internal.h"
+
+namespace lsd {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+static int check_config(const lsd_config* c, int* H, int* Th, int* Ph) {
+    if (!c) {
+        set_error("null config");
+        return LSD_E_ARG;
+    }
+    // Scripts/parameters.F90_template:274-288
+    if (c->L <= (c->M - 1) * c->Mp) { set_error("set L > (M-1)*Mp"); return LSD_E_CONFIG; }
+    if (c->M > 1 && (c->M % 2) == 1) { set_error("if(M>1) set M even"); return LSD_E_CONFIG; }
+    if (c->Mp < 1 || c->M < 1) { set_error("M, Mp must be positive"); return LSD_E_CONFIG; }
+    if (c->N < 7 || c->N > 1024) { set_error("N must be in [7, 1024]"); return LSD_E_CONFIG; }
+    int th = c->Th > 0 ? c->Th : c->L * 3 / 2;
+    int ph = c->Ph > 0 ? c->Ph : (c->M * 3 / 2) * 2;
+    if (th % 2) { set_error("Th must be even"); return LSD_E_CONFIG; }
+    if (ph < 2 * c->M || ph % 2) { set_error("Ph must be even and >= 2M"); return LSD_E_CONFIG; }
+    if (!c->vel) { set_error("vel must be time-stepped"); return LSD_E_CONFIG; }
+    if (c->vel_bc != 1 || c->cod_bc != 1 || c->comp_bc != 1) {
+        set_error("only i_vel_bc = i_cod_bc = i_comp_bc = 1 are supported");
+        return LSD_E_CONFIG;
+    }
+    if (c->dt <= 0.0) { set_error("dt must be > 0 (fixed step)"); return LSD_E_CONFIG; }
+    if (c->precision != LSD_PLAIN_F64) { set_error("only LSD_PLAIN_F64 is available in this build"); return LSD_E_CONFIG; }
+    if (H) *H = c->L * c->M - c->Mp * (c->M - 1) * c->M / 2;
+    if (Th) *Th = th;
+    if (Ph) *Ph = ph;
+    return LSD_OK;
+}
+
+// Folded harmonic order (Scripts/parallel.h_template:65-80).
+static void harmonics(int L, int M, int Mp, std::vector<int>& hl, std::vector<int>& hm) {
+    int H = L * M - Mp * (M - 1) * M / 2;
+    int Hc = (M + 1) / 2, Hr = H / Hc;
+    hl.resize(H);
+    hm.resize(H);
+    int nh = 0;
+    for (int jj = 1; jj <= Hc; jj++)
+        for (int ii = 1; ii <= Hr; ii++) {
+            int m = jj - 1, l = m * Mp + ii - 1;
+            if (l >= L) {
+                m = 2 * Hc - jj;
+                l = L - Hr + ii - 1;
+            }
+            hl[nh] = l;
+            hm[nh] = m * Mp;
+            nh++;
+        }
+}
+
+template <typename T>
+static int dalloc(T** p, size_t n) {
+    LSD_HIP(hipMalloc((void**)p, sizeof(T) * (n ? n : 1)));
+    return LSD_OK;
+}
+
+static void release(Ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    void* ptrs[] = {c->d_hl, c->d_hm, c->d_pidx, c->d_apos, c->d_slots, c->d_slot_order, c->d_gp, c->d_gw,
+                    c->d_sin, c->d_tab, c->d_atab, c->d_r, c->d_dr1, c->d_dr2, c->d_radlap, c->d_r1dr,
+                    c->d_intr2dr, c->d_ll, c->d_lu, c->d_piv, c->d_y, c->d_gre, c->d_invg, c->d_info,
+                    c->d_spec_in, c->d_spec_out, c->d_work, c->d_fourier, c->d_folded, c->d_phys, c->d_nlphys,
+                    c->d_tw, c->d_partial};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    for (int k = 0; k < LSD_NUM_COLL; k++)
+        if (c->d_f[k]) (void)hipFree(c->d_f[k]);
+    for (int k = 0; k < 6; k++)
+        if (c->d_hist[k]) (void)hipFree(c->d_hist[k]);
+    if (c->h_partial) (void)hipHostFree(c->h_partial);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+static int create(const lsd_config* cfg, int device, Ctx** out) {
+    int H, Th, Ph;
+    int rc = check_config(cfg, &H, &Th, &Ph);
+    if (rc) return rc;
+    Ctx* c = new Ctx();
+    c->cfg = *cfg;
+    c->device = device;
+    c->L = cfg->L;
+    c->M = cfg->M;
+    c->Mp = cfg->Mp;
+    c->N = cfg->N;
+    c->Th = Th;
+    c->Ph = Ph;
+    c->H = H;
+    c->Th2 = (Th + 1) / 2;
+    c->dt = cfg->dt;
+    *out = c;
+    LSD_HIP(hipSetDevice(device));
+    LSD_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+
+    // host index bookkeeping
+    harmonics(c->L, c->M, c->Mp, c->hl, c->hm);
+    for (int h = 0; h < H; h++) {
+        if (h == 0 || c->hm[h] != c->hm[h - 1]) c->slots.push_back(Slot{h, 0, c->hm[h], c->hm[h] / c->Mp});
+        c->slots.back().len++;
+    }
+    c->nslot = (int)c->slots.size();
+    c->slot_order.resize(c->nslot);
+    for (int s = 0; s < c->nslot; s++) c->slot_order[s] = s;
+    std::stable_sort(c->slot_order.begin(), c->slot_order.end(),
+                     [&](int a, int b) { return c->slots[a].len > c->slots[b].len; });
+    int L = c->L, N = c->N;
+    std::vector<int> pidx((size_t)L * (L + 1) / 2, -1), apos(H);
+    for (int h = 0; h < H; h++) pidx[(size_t)c->hl[h] * (c->hl[h] + 1) / 2 + c->hm[h]] = h;
+    for (auto& s : c->slots)
+        for (int k = 0; k < s.len; k++)
+            apos[s.h0 + k] = s.h0 + ((k & 1) ? (s.len + 1) / 2 + (k >> 1) : (k >> 1));
+
+    size_t HN = c->coll_elems();
+    if (dalloc(&c->d_hl, H) || dalloc(&c->d_hm, H) || dalloc(&c->d_pidx, pidx.size()) ||
+        dalloc(&c->d_apos, H) || dalloc(&c->d_slots, c->nslot) || dalloc(&c->d_slot_order, c->nslot) ||
+        dalloc(&c->d_gp, Th) || dalloc(&c->d_gw, Th) || dalloc(&c->d_sin, Th) ||
+        dalloc(&c->d_tab, (size_t)5 * H * c->Th2) || dalloc(&c->d_atab, (size_t)3 * H * c->Th2) ||
+        dalloc(&c->d_r, (size_t)7 * N) || dalloc(&c->d_dr1, (size_t)7 * N) || dalloc(&c->d_dr2, (size_t)7 * N) ||
+        dalloc(&c->d_radlap, (size_t)7 * N) || dalloc(&c->d_r1dr, (size_t)7 * N) || dalloc(&c->d_intr2dr, N) ||
+        dalloc(&c->d_ll, (size_t)4 * L) || dalloc(&c->d_lu, (size_t)NLU * L * N * LSD_LDAB) ||
+        dalloc(&c->d_piv, (size_t)NLU * L * N) || dalloc(&c->d_y, (size_t)NY * L * N * 7) ||
+        dalloc(&c->d_gre, (size_t)2 * L * N) || dalloc(&c->d_invg, (size_t)4 * L) ||
+        dalloc(&c->d_info, (size_t)NLU * L) || dalloc(&c->d_spec_in, (size_t)SI_COUNT * HN) ||
+        dalloc(&c->d_spec_out, (size_t)SO_COUNT * HN) || dalloc(&c->d_work, (size_t)6 * HN) ||
+        dalloc(&c->d_fourier, (size_t)LSD_NUM_PHYS * c->fourier_elems()) ||
+        dalloc(&c->d_folded, (size_t)NO_COUNT * c->folded_elems()) ||
+        dalloc(&c->d_phys, (size_t)LSD_NUM_PHYS * c->phys_elems()) ||
+        dalloc(&c->d_nlphys, (size_t)NO_COUNT * c->phys_elems()))
+        return LSD_E_HIP;
+    for (int k = 0; k < LSD_NUM_COLL; k++)
+        if (dalloc(&c->d_f[k], HN)) return LSD_E_HIP;
+    for (int k = 0; k < 6; k++)
+        if (dalloc(&c->d_hist[k], HN)) return LSD_E_HIP;
+    c->n_partial_blocks = (H + 63) / 64;
+    if (dalloc(&c->d_partial, (size_t)8 * c->n_partial_blocks)) return LSD_E_HIP;
+    LSD_HIP(hipHostMalloc((void**)&c->h_partial, sizeof(double) * 8 * c->n_partial_blocks, hipHostMallocDefault));
+
+    hipStream_t st = c->stream;
+    LSD_HIP(hipMemcpyAsync(c->d_hl, c->hl.data(), sizeof(int) * H, hipMemcpyHostToDevice, st));
+    LSD_HIP(hipMemcpyAsync(c->d_hm, c->hm.data(), sizeof(int) * H, hipMemcpyHostToDevice, st));
+    LSD_HIP(hipMemcpyAsync(c->d_pidx, pidx.data(), sizeof(int) * pidx.size(), hipMemcpyHostToDevice, st));
+    LSD_HIP(hipMemcpyAsync(c->d_apos, apos.data(), sizeof(int) * H, hipMemcpyHostToDevice, st));
+    LSD_HIP(hipMemcpyAsync(c->d_slots, c->slots.data(), sizeof(Slot) * c->nslot, hipMemcpyHostToDevice, st));
+    LSD_HIP(hipMemcpyAsync(c->d_slot_order, c->slot_order.data(), sizeof(int) * c->nslot, hipMemcpyHostToDevice, st));
+    LSD_HIP(hipMemsetAsync(c->d_tab, 0, sizeof(double) * 5 * H * c->Th2, st));
+    LSD_HIP(hipMemsetAsync(c->d_atab, 0, sizeof(double) * 3 * H * c->Th2, st));
+    LSD_HIP(hipMemsetAsync(c->d_lu, 0, sizeof(double) * NLU * L * N * LSD_LDAB, st));
+    LSD_HIP(hipMemsetAsync(c->d_piv, 0, sizeof(int) * NLU * L * N, st));
+    LSD_HIP(hipMemsetAsync(c->d_y, 0, sizeof(double) * NY * L * N * 7, st));
+    LSD_HIP(hipMemsetAsync(c->d_gre, 0, sizeof(double) * 2 * L * N, st));
+    LSD_HIP(hipMemsetAsync(c->d_invg, 0, sizeof(double) * 4 * L, st));
+    for (int k = 0; k < LSD_NUM_COLL; k++) LSD_HIP(hipMemsetAsync(c->d_f[k], 0, sizeof(double2) * HN, st));
+    for (int k = 0; k < 6; k++) LSD_HIP(hipMemsetAsync(c->d_hist[k], 0, sizeof(double2) * HN, st));
+    LSD_HIP(hipMemsetAsync(c->d_spec_in, 0, sizeof(double2) * SI_COUNT * HN, st));
+    LSD_HIP(hipMemsetAsync(c->d_spec_out, 0, sizeof(double2) * SO_COUNT * HN, st));
+    LSD_HIP(hipMemsetAsync(c->d_fourier, 0, sizeof(double2) * LSD_NUM_PHYS * c->fourier_elems(), st));
+    LSD_HIP(hipMemsetAsync(c->d_folded, 0, sizeof(double2) * NO_COUNT * c->folded_elems(), st));
+    LSD_HIP(hipMemsetAsync(c->d_phys, 0, sizeof(double) * LSD_NUM_PHYS * c->phys_elems(), st));
+    LSD_HIP(hipMemsetAsync(c->d_nlphys, 0, sizeof(double) * NO_COUNT * c->phys_elems(), st));
+
+    if ((rc = fft_setup(*c))) return rc;
+    if ((rc = precompute_mesh(*c))) return rc;
+    if ((rc = precompute_tables(*c))) return rc;
+    if ((rc = build_operators(*c))) return rc;
+    LSD_HIP(hipStreamSynchronize(st));
+    return LSD_OK;
+}
+
+// ------------------------------------------------------------- phases
+
+static double2* spec_in(Ctx& c, int k) { return c.d_spec_in + (size_t)k * c.coll_elems(); }
+static double2* spec_out(Ctx& c, int k) { return c.d_spec_out + (size_t)k * c.coll_elems(); }
+static double2* fourier(Ctx& c, int k) { return c.d_fourier + (size_t)k * c.fourier_elems(); }
+static double2* folded(Ctx& c, int k) { return c.d_folded + (size_t)k * c.folded_elems(); }
+static double* phys(Ctx& c, int k) { return c.d_phys + (size_t)k * c.phys_elems(); }
+static double* nlphys(Ctx& c, int k) { return c.d_nlphys + (size_t)k * c.phys_elems(); }
+
+#define RC(x)             \
+    do {                  \
+        int r_ = (x);     \
+        if (r_) return r_; \
+    } while (0)
+
+// vel_transform, cod_transform, comp_transform, mag_transform
+static int transform(Ctx& c) {
+    RC(launch_vec_prep(c, c.d_f[LSD_VEL_TOR], c.d_f[LSD_VEL_POL], spec_in(c, SI_UQ), spec_in(c, SI_US),
+                       spec_in(c, SI_UT), spec_in(c, SI_WQ), spec_in(c, SI_WS), spec_in(c, SI_WT)));
+    RC(launch_synthesis(c, SYN_SCALAR, spec_in(c, SI_UQ), nullptr, fourier(c, LSD_P_VEL_R), nullptr));
+    RC(launch_synthesis(c, SYN_VECTOR, spec_in(c, SI_US), spec_in(c, SI_UT), fourier(c, LSD_P_VEL_T),
+                        fourier(c, LSD_P_VEL_P)));
+    RC(launch_synthesis(c, SYN_SCALAR, spec_in(c, SI_WQ), nullptr, fourier(c, LSD_P_VCURL_R), nullptr));
+    RC(launch_synthesis(c, SYN_VECTOR, spec_in(c, SI_WS), spec_in(c, SI_WT), fourier(c, LSD_P_VCURL_T),
+                        fourier(c, LSD_P_VCURL_P)));
+    RC(launch_fft_c2r(c, fourier(c, 0), phys(c, 0), 6));
+    if (c.cfg.mag) {
+        RC(launch_vec_prep(c, c.d_f[LSD_MAG_TOR], c.d_f[LSD_MAG_POL], spec_in(c, SI_BQ), spec_in(c, SI_BS),
+                           spec_in(c, SI_BT), spec_in(c, SI_JQ), spec_in(c, SI_JS), spec_in(c, SI_JT)));
+        RC(launch_synthesis(c, SYN_SCALAR, spec_in(c, SI_BQ), nullptr, fourier(c, LSD_P_MAG_R), nullptr));
+        RC(launch_synthesis(c, SYN_VECTOR, spec_in(c, SI_BS), spec_in(c, SI_BT), fourier(c, LSD_P_MAG_T),
+                            fourier(c, LSD_P_MAG_P)));
+        RC(launch_synthesis(c, SYN_SCALAR, spec_in(c, SI_JQ), nullptr, fourier(c, LSD_P_MCURL_R), nullptr));
+        RC(launch_synthesis(c, SYN_VECTOR, spec_in(c, SI_JS), spec_in(c, SI_JT), fourier(c, LSD_P_MCURL_T),
+                            fourier(c, LSD_P_MCURL_P)));
+        RC(launch_fft_c2r(c, fourier(c, 6), phys(c, 6), 6));
+    }
+    if (c.cfg.cod) {
+        RC(launch_scal_prep(c, c.d_f[LSD_COD], spec_in(c, SI_DC)));
+        RC(launch_synthesis(c, SYN_SCALAR, spec_in(c, SI_DC), nullptr, fourier(c, LSD_P_COD_GR), nullptr));
+        RC(launch_synthesis(c, SYN_GRAD, c.d_f[LSD_COD], nullptr, fourier(c, LSD_P_COD_GT), fourier(c, LSD_P_COD_GP)));
+        RC(launch_fft_c2r(c, fourier(c, 12), phys(c, 12), 3));
+    }
+    if (c.cfg.comp) {
+        RC(launch_scal_prep(c, c.d_f[LSD_COMP], spec_in(c, SI_DX)));
+        RC(launch_synthesis(c, SYN_SCALAR, spec_in(c, SI_DX), nullptr, fourier(c, LSD_P_COMP_GR), nullptr));
+        RC(launch_synthesis(c, SYN_GRAD, c.d_f[LSD_COMP], nullptr, fourier(c, LSD_P_COMP_GT),
+                            fourier(c, LSD_P_COMP_GP)));
+        RC(launch_fft_c2r(c, fourier(c, 15), phys(c, 15), 3));
+    }
+    return LSD_OK;
+}
+
+// non_velocity, non_codensity, non_composition, non_magnetic
+static int nonlinear(Ctx& c) {
+    RC(launch_nonlinear(c));
+    RC(launch_fft_r2c_fold(c, nlphys(c, NO_UR), folded(c, NO_UR), 3));
+    RC(launch_analysis(c, ANA_VECTOR, folded(c, NO_UR), folded(c, NO_UT), folded(c, NO_UP), spec_out(c, SO_UQ),
+                       spec_out(c, SO_US), spec_out(c, SO_UT)));
+    if (c.cfg.cod) {
+        RC(launch_fft_r2c_fold(c, nlphys(c, NO_C), folded(c, NO_C), 1));
+        RC(launch_analysis(c, ANA_SCALAR, folded(c, NO_C), nullptr, nullptr, spec_out(c, SO_C), nullptr, nullptr));
+    }
+    if (c.cfg.comp) {
+        RC(launch_fft_r2c_fold(c, nlphys(c, NO_X), folded(c, NO_X), 1));
+        RC(launch_analysis(c, ANA_SCALAR, folded(c, NO_X), nullptr, nullptr, spec_out(c, SO_X), nullptr, nullptr));
+    }
+    if (c.cfg.mag) {
+        RC(launch_fft_r2c_fold(c, nlphys(c, NO_ER), folded(c, NO_ER), 3));
+        RC(launch_analysis(c, ANA_VECTOR, folded(c, NO_ER), folded(c, NO_ET), folded(c, NO_EP), spec_out(c, SO_EQ),
+                           spec_out(c, SO_ES), spec_out(c, SO_ET)));
+    }
+    RC(launch_spec_post(c));
+    return LSD_OK;
+}
+
+// tim_check_cgce, i_time_step_control = 2 (modules/timestep.F90:791-827).
+// Returns nonzero if the reference would stop the run.
+static int check_cgce(Ctx& c) {
+    int stop = 0;
+    if (c.it == 1) {
+        c.corr_dt = c.dt * std::sqrt(c.cfg.dterr / c.dterr);
+        c.lasterr = 1e99;
+    }
+    if (c.dt < 1e-12 && c.step > 30) {
+        stop = 1;
+        c.it = 0;
+    } else if (c.it > 10) {
+        stop = 1;
+        c.it = 0;
+    } else if (c.dterr > c.lasterr) {
+        if (c.dterr > 2.0 * c.cfg.dterr) stop = 1;
+        c.it = 0;
+    } else if (c.dterr > c.cfg.dterr) {
+        c.lasterr = c.dterr;
+        c.it = c.it + 1;
+    } else {
+        c.it = 0;
+    }
+    c.dterr = 0.0;
+    return stop;
+}
+
+static int step(Ctx& c, int nsteps, lsd_step_stats* st) {
+    lsd_step_stats z;
+    std::memset(&z, 0, sizeof z);
+    for (int k = 0; k < nsteps; k++) {
+        int its = 0;
+        double last = 0.0;
+        do {
+            RC(transform(c));
+            RC(nonlinear(c));
+            z.evaluations++;
+            if (c.it == 0) {
+                RC(launch_predictor(c));
+                c.it = 1;
+            } else {
+                double d = 0.0;
+                RC(launch_corrector(c, &d));
+                c.dterr += d;
+                last = c.dterr;
+                its++;
+                if (check_cgce(c)) {
+                    set_error("tim_check_cgce: run stopped");
+                    return LSD_E_DIVERGED;
+                }
+            }
+        } while (c.it != 0);
+        c.t = c.t + c.dt;
+        c.step++;
+        z.steps++;
+        z.last_its = its;
+        z.max_its = std::max(z.max_its, its);
+        z.last_dterr = last;
+    }
+    z.t = c.t;
+    if (st) *st = z;
+    return LSD_OK;
+}
+
+}  // namespace lsd
+
+using lsd::Ctx;
+
+struct lsd_ctx {
+    Ctx* c;
+};
+
+#define CTX_OR_FAIL(ctx)                    \
+    if (!(ctx) || !(ctx)->c) {              \
+        lsd::set_error("null context");     \
+        return LSD_E_ARG;                   \
+    }                                       \
+    Ctx& c = *(ctx)->c;                     \
+    if (hipSetDevice(c.device) != hipSuccess) { \
+        lsd::set_error("hipSetDevice failed"); \
+        return LSD_E_HIP;                   \
+    }
+
+extern "C" {
+
+const char* lsd_last_error(void) { return lsd::g_err.c_str(); }
+
+int lsd_config_sizes(const lsd_config* cfg, int* H, int* Th, int* Ph) { return lsd::check_config(cfg, H, Th, Ph); }
+
+int lsd_create(const lsd_config* cfg, int device, lsd_ctx** out) {
+    if (!out) {
+        lsd::set_error("null output handle");
+        return LSD_E_ARG;
+    }
+    *out = nullptr;
+    Ctx* c = nullptr;
+    int rc = lsd::create(cfg, device, &c);
+    if (rc) {
+        std::string keep = lsd::g_err;
+        lsd::release(c);
+        lsd::set_error(keep);
+        return rc;
+    }
+    *out = new lsd_ctx{c};
+    return LSD_OK;
+}
+
+void lsd_destroy(lsd_ctx* ctx) {
+    if (!ctx) return;
+    lsd::release(ctx->c);
+    delete ctx;
+}
+
+int lsd_sync(lsd_ctx* ctx) {
+    CTX_OR_FAIL(ctx);
+    LSD_HIP(hipStreamSynchronize(c.stream));
+    return LSD_OK;
+}
+
+int lsd_set_coll(lsd_ctx* ctx, int field, const double* re, const double* im) {
+    CTX_OR_FAIL(ctx);
+    if (field < 0 || field >= LSD_NUM_COLL || !re || !im) {
+        lsd::set_error("lsd_set_coll: bad field or null pointer");
+        return LSD_E_ARG;
+    }
+    size_t HN = c.coll_elems();
+    std::vector<double2> tmp(HN);
+    for (size_t q = 0; q < HN; q++) tmp[q] = make_double2(re[q], im[q]);
+    LSD_HIP(hipMemcpyAsync(c.d_f[field], tmp.data(), sizeof(double2) * HN, hipMemcpyHostToDevice, c.stream));
+    LSD_HIP(hipStreamSynchronize(c.stream));
+    return LSD_OK;
+}
+
+int lsd_get_coll(lsd_ctx* ctx, int field, double* re, double* im) {
+    CTX_OR_FAIL(ctx);
+    if (field < 0 || field >= LSD_NUM_COLL || !re || !im) {
+        lsd::set_error("lsd_get_coll: bad field or null pointer");
+        return LSD_E_ARG;
+    }
+    size_t HN = c.coll_elems();
+    std::vector<double2> tmp(HN);
+    LSD_HIP(hipMemcpyAsync(tmp.data(), c.d_f[field], sizeof(double2) * HN, hipMemcpyDeviceToHost, c.stream));
+    LSD_HIP(hipStreamSynchronize(c.stream));
+    for (size_t q = 0; q < HN; q++) {
+        re[q] = tmp[q].x;
+        im[q] = tmp[q].y;
+    }
+    return LSD_OK;
+}
+
+int lsd_get_phys(lsd_ctx* ctx, int field, double* p) {
+    CTX_OR_FAIL(ctx);
+    if (field < 0 || field >= LSD_NUM_PHYS || !p) {
+        lsd::set_error("lsd_get_phys: bad field or null pointer");
+        return LSD_E_ARG;
+    }
+    LSD_HIP(hipMemcpyAsync(p, lsd::phys(c, field), sizeof(double) * c.phys_elems(), hipMemcpyDeviceToHost,
+                           c.stream));
+    LSD_HIP(hipStreamSynchronize(c.stream));
+    return LSD_OK;
+}
+
+int lsd_get_table(lsd_ctx* ctx, int table, double* out) {
+    CTX_OR_FAIL(ctx);
+    if (!out) {
+        lsd::set_error("null output");
+        return LSD_E_ARG;
+    }
+    size_t HT = (size_t)c.H * c.Th2;
+    if (table == LSD_T_GP || table == LSD_T_GW) {
+        LSD_HIP(hipMemcpyAsync(out, table == LSD_T_GP ? c.d_gp : c.d_gw, sizeof(double) * c.Th,
+                               hipMemcpyDeviceToHost, c.stream));
+    } else if (table >= LSD_T_P && table <= LSD_T_LLSIN1MP) {
+        LSD_HIP(hipMemcpyAsync(out, c.d_tab + (size_t)(table - LSD_T_P) * HT, sizeof(double) * HT,
+                               hipMemcpyDeviceToHost, c.stream));
+    } else if (table >= LSD_T_GWP && table <= LSD_T_GWLLSIN1MP) {
+        std::vector<double> tmp(HT);
+        LSD_HIP(hipMemcpyAsync(tmp.data(), c.d_atab + (size_t)(table - LSD_T_GWP) * HT, sizeof(double) * HT,
+                               hipMemcpyDeviceToHost, c.stream));
+        LSD_HIP(hipStreamSynchronize(c.stream));
+        for (auto& s : c.slots)
+            for (int k = 0; k < s.len; k++) {
+                int ap = s.h0 + ((k & 1) ? (s.len + 1) / 2 + (k >> 1) : (k >> 1));
+                for (int j = 0; j < c.Th2; j++) out[(size_t)(s.h0 + k) * c.Th2 + j] = tmp[(size_t)j * c.H + ap];
+            }
+    } else {
+        lsd::set_error("lsd_get_table: unknown table");
+        return LSD_E_ARG;
+    }
+    LSD_HIP(hipStreamSynchronize(c.stream));
+    return LSD_OK;
+}
+
+int lsd_get_mesh(lsd_ctx* ctx, int which, double* out) {
+    CTX_OR_FAIL(ctx);
+    if (!out) {
+        lsd::set_error("null output");
+        return LSD_E_ARG;
+    }
+    const double* src = nullptr;
+    size_t n = 7 * (size_t)c.N;
+    switch (which) {
+    case LSD_M_R: src = c.d_r + 4 * c.N; n = c.N; break;
+    case LSD_M_DR1: src = c.d_dr1; break;
+    case LSD_M_DR2: src = c.d_dr2; break;
+    case LSD_M_RADLAP: src = c.d_radlap; break;
+    case LSD_M_R1DR: src = c.d_r1dr; break;
+    case LSD_M_INTR2DR: src = c.d_intr2dr; n = c.N; break;
+    default: lsd::set_error("lsd_get_mesh: unknown array"); return LSD_E_ARG;
+    }
+    LSD_HIP(hipMemcpyAsync(out, src, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    LSD_HIP(hipStreamSynchronize(c.stream));
+    return LSD_OK;
+}
+
+int lsd_transform(lsd_ctx* ctx) {
+    CTX_OR_FAIL(ctx);
+    return lsd::transform(c);
+}
+int lsd_nonlinear(lsd_ctx* ctx) {
+    CTX_OR_FAIL(ctx);
+    return lsd::nonlinear(c);
+}
+int lsd_predictor(lsd_ctx* ctx) {
+    CTX_OR_FAIL(ctx);
+    int rc = lsd::launch_predictor(c);
+    if (rc) return rc;
+    LSD_HIP(hipStreamSynchronize(c.stream));
+    return LSD_OK;
+}
+int lsd_corrector(lsd_ctx* ctx, double* dterr) {
+    CTX_OR_FAIL(ctx);
+    double d = 0.0;
+    int rc = lsd::launch_corrector(c, &d);
+    if (rc) return rc;
+    if (dterr) *dterr = d;
+    return LSD_OK;
+}
+
+int lsd_step(lsd_ctx* ctx, int nsteps, lsd_step_stats* stats) {
+    CTX_OR_FAIL(ctx);
+    int rc = lsd::step(c, nsteps, stats);
+    if (rc) return rc;
+    LSD_HIP(hipStreamSynchronize(c.stream));
+    return LSD_OK;
+}
+
+int lsd_set_dt(lsd_ctx* ctx, double dt) {
+    CTX_OR_FAIL(ctx);
+    if (!(dt > 0.0)) {
+        lsd::set_error("dt must be > 0");
+        return LSD_E_ARG;
+    }
+    c.dt = dt;
+    c.cfg.dt = dt;
+    return lsd::build_operators(c);
+}
+
+// ---------------------------------------------------- drop-in transforms
+
+static int upload_spec(Ctx& c, const double* re, const double* im, double2* dst) {
+    size_t HN = c.coll_elems();
+    double* tmp = reinterpret_cast<double*>(lsd::nlphys(c, 0));  // staging, >= 2*H*N doubles
+    LSD_HIP(hipMemcpyAsync(tmp, re, sizeof(double) * HN, hipMemcpyHostToDevice, c.stream));
+    LSD_HIP(hipMemcpyAsync(tmp + HN, im, sizeof(double) * HN, hipMemcpyHostToDevice, c.stream));
+    return lsd::launch_layout_spec2coll(c, tmp, tmp + HN, dst);
+}
+
+static int download_spec(Ctx& c, const double2* src, double* re, double* im) {
+    size_t HN = c.coll_elems();
+    double* tmp = reinterpret_cast<double*>(lsd::nlphys(c, 0));
+    int rc = lsd::launch_layout_coll2spec(c, src, tmp, tmp + HN);
+    if (rc) return rc;
+    LSD_HIP(hipMemcpyAsync(re, tmp, sizeof(double) * HN, hipMemcpyDeviceToHost, c.stream));
+    LSD_HIP(hipMemcpyAsync(im, tmp + HN, sizeof(double) * HN, hipMemcpyDeviceToHost, c.stream));
+    LSD_HIP(hipStreamSynchronize(c.stream));
+    return LSD_OK;
+}
+
+static int download_phys(Ctx& c, int k, double* p) {
+    LSD_HIP(hipMemcpyAsync(p, lsd::phys(c, k), sizeof(double) * c.phys_elems(), hipMemcpyDeviceToHost, c.stream));
+    return LSD_OK;
+}
+
+int lsd_tra_spec2phys(lsd_ctx* ctx, const double* s_re, const double* s_im, double* p) {
+    CTX_OR_FAIL(ctx);
+    int rc;
+    double2* in = lsd::spec_in(c, lsd::SI_DC);
+    if ((rc = upload_spec(c, s_re, s_im, in))) return rc;
+    if ((rc = lsd::launch_synthesis(c, lsd::SYN_SCALAR, in, nullptr, lsd::fourier(c, 0), nullptr))) return rc;
+    if ((rc = lsd::launch_fft_c2r(c, lsd::fourier(c, 0), lsd::phys(c, 0), 1))) return rc;
+    if ((rc = download_phys(c, 0, p))) return rc;
+    LSD_HIP(hipStreamSynchronize(c.stream));
+    return LSD_OK;
+}
+
+int lsd_tra_grad(lsd_ctx* ctx, const double* s_re, const double* s_im, double* t, double* p) {
+    CTX_OR_FAIL(ctx);
+    int rc;
+    double2* in = lsd::spec_in(c, lsd::SI_DC);
+    if ((rc = upload_spec(c, s_re, s_im, in))) return rc;
+    if ((rc = lsd::launch_synthesis(c, lsd::SYN_GRAD, in, nullptr, lsd::fourier(c, 0), lsd::fourier(c, 1))))
+        return rc;
+    if ((rc = lsd::launch_fft_c2r(c, lsd::fourier(c, 0), lsd::phys(c, 0), 2))) return rc;
+    if ((rc = download_phys(c, 0, t))) return rc;
+    if ((rc = download_phys(c, 1, p))) return rc;
+    LSD_HIP(hipStreamSynchronize(c.stream));
+    return LSD_OK;
+}
+
+int lsd_tra_qst2rtp(lsd_ctx* ctx, const double* q_re, const double* q_im, const double* s_re, const double* s_im,
+                    const double* t_re, const double* t_im, double* r, double* th, double* ph) {
+    CTX_OR_FAIL(ctx);
+    int rc;
+    double2* q = lsd::spec_in(c, lsd::SI_UQ);
+    double2* s = lsd::spec_in(c, lsd::SI_US);
+    double2* t = lsd::spec_in(c, lsd::SI_UT);
+    if ((rc = upload_spec(c, q_re, q_im, q)) || (rc = upload_spec(c, s_re, s_im, s)) ||
+        (rc = upload_spec(c, t_re, t_im, t)))
+        return rc;
+    if ((rc = lsd::launch_synthesis(c, lsd::SYN_SCALAR, q, nullptr, lsd::fourier(c, 0), nullptr))) return rc;
+    if ((rc = lsd::launch_synthesis(c, lsd::SYN_VECTOR, s, t, lsd::fourier(c, 1), lsd::fourier(c, 2)))) return rc;
+    if ((rc = lsd::launch_fft_c2r(c, lsd::fourier(c, 0), lsd::phys(c, 0), 3))) return rc;
+    if ((rc = download_phys(c, 0, r)) || (rc = download_phys(c, 1, th)) || (rc = download_phys(c, 2, ph))) return rc;
+    LSD_HIP(hipStreamSynchronize(c.stream));
+    return LSD_OK;
+}
+
+int lsd_tra_phys2spec(lsd_ctx* ctx, const double* p, double* s_re, double* s_im) {
+    CTX_OR_FAIL(ctx);
+    int rc;
+    double* x = lsd::phys(c, 0);
+    LSD_HIP(hipMemcpyAsync(x, p, sizeof(double) * c.phys_elems(), hipMemcpyHostToDevice, c.stream));
+    if ((rc = lsd::launch_fft_r2c_fold(c, x, lsd::folded(c, 0), 1))) return rc;
+    double2* o = lsd::spec_out(c, lsd::SO_C);
+    if ((rc = lsd::launch_analysis(c, lsd::ANA_SCALAR, lsd::folded(c, 0), nullptr, nullptr, o, nullptr, nullptr)))
+        return rc;
+    return download_spec(c, o, s_re, s_im);
+}
+
+int lsd_tra_rtp2qst(lsd_ctx* ctx, const double* r, const double* th, const double* ph, double* q_re, double* q_im,
+                    double* s_re, double* s_im, double* t_re, double* t_im) {
+    CTX_OR_FAIL(ctx);
+    int rc;
+    size_t np = c.phys_elems();
+    LSD_HIP(hipMemcpyAsync(lsd::phys(c, 0), r, sizeof(double) * np, hipMemcpyHostToDevice, c.stream));
+    LSD_HIP(hipMemcpyAsync(lsd::phys(c, 1), th, sizeof(double) * np, hipMemcpyHostToDevice, c.stream));
+    LSD_HIP(hipMemcpyAsync(lsd::phys(c, 2), ph, sizeof(double) * np, hipMemcpyHostToDevice, c.stream));
+    if ((rc = lsd::launch_fft_r2c_fold(c, lsd::phys(c, 0), lsd::folded(c, 0), 3))) return rc;
+    double2* o0 = lsd::spec_out(c, lsd::SO_UQ);
+    double2* o1 = lsd::spec_out(c, lsd::SO_US);
+    double2* o2 = lsd::spec_out(c, lsd::SO_UT);
+    if ((rc = lsd::launch_analysis(c, lsd::ANA_VECTOR, lsd::folded(c, 0), lsd::folded(c, 1), lsd::folded(c, 2), o0,
+                                   o1, o2)))
+        return rc;
+    if ((rc = download_spec(c, o0, q_re, q_im)) || (rc = download_spec(c, o1, s_re, s_im)) ||
+        (rc = download_spec(c, o2, t_re, t_im)))
+        return rc;
+    return LSD_OK;
+}
+
+}  // extern "C"

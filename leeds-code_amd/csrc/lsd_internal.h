@@ -1,0 +1,174 @@
+// Internal declarations of the MI355X per-timestep library.
+//
+// Device layouts (all resident in HBM for the life of a context):
+//   coll     double2 [h][n]            complex, radius fastest (the reference's
+//                                       coll type with Re/Im interleaved)
+//   fourier  double2 [f][n][m][j]      j = 0..Th-1 fastest, m = FFT index
+//   folded   double2 [f][n][m][p][j]   j < Th2, p = equatorial parity bucket
+//                                       (north+south, south-north)
+//   phys     double  [f][n][k][j]      the reference's (Th, Ph, N) layout
+//   tables   synthesis  double [h][j]  (j < Th2 fastest, like leg_P(j,nh))
+//            analysis   double [j][h]  (h fastest, so lanes over harmonics
+//                                       read coalesced)
+//   bands    mesh  double [n][7]       M(KL+1+n-j, j) stored per column j
+//            lu    double [l][n][10]   LAPACK band storage + int pivots
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/lsd.h"
+
+#define LSD_KL 3
+#define LSD_LDAB 10
+#define LSD_MBW 7
+
+namespace lsd {
+
+void set_error(const std::string& msg);
+
+#define LSD_HIP(call)                                                         \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess) {                                               \
+            lsd::set_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
+            return LSD_E_HIP;                                                 \
+        }                                                                     \
+    } while (0)
+
+// One m-slot: a run of harmonics with equal m (the reference's _loop_sumth).
+struct Slot {
+    int h0;    // first harmonic index
+    int len;   // number of harmonics (l = m .. L-1)
+    int m;     // actual order m_
+    int mfft;  // FFT index m_/Mp
+};
+
+// Synthesis job kinds.
+enum SynthKind { SYN_SCALAR = 0, SYN_VECTOR = 1, SYN_GRAD = 2 };
+
+// Analysis job kinds.
+enum AnalKind { ANA_SCALAR = 0, ANA_VECTOR = 1 };
+
+// Banded operator kinds (LU): 0 vel tor, 1 vel pol, 2 vel green, 3 cod,
+// 4 comp, 5 mag tor, 6 mag pol.  Y kinds: 0 vel tor, 1 vel pol, 2 cod,
+// 3 comp, 4 mag tor, 5 mag pol.
+constexpr int NLU = 7;
+constexpr int NY = 6;
+
+// Synthesis input (coll-layout spectral) buffer ids.
+enum SpecIn {
+    SI_UQ = 0, SI_US, SI_UT, SI_WQ, SI_WS, SI_WT,
+    SI_BQ, SI_BS, SI_BT, SI_JQ, SI_JS, SI_JT,
+    SI_DC, SI_DX, SI_COUNT
+};
+
+// Analysis output (coll-layout) buffer ids.
+enum SpecOut { SO_UQ = 0, SO_US, SO_UT, SO_C, SO_X, SO_EQ, SO_ES, SO_ET, SO_COUNT };
+
+// Physical output field ids of the nonlinear products (folded buffers).
+enum NlOut { NO_UR = 0, NO_UT, NO_UP, NO_C, NO_X, NO_ER, NO_ET, NO_EP, NO_COUNT };
+
+struct Ctx {
+    lsd_config cfg;
+    int device = 0;
+    int L, M, Mp, N, Th, Ph, H, Th2, nslot;
+    double dt;
+    hipStream_t stream = nullptr;
+
+    std::vector<int> hl, hm;       // per harmonic l, m_
+    std::vector<Slot> slots;       // in harmonic order
+    std::vector<int> slot_order;   // slots sorted by decreasing length
+
+    // device: index data
+    int* d_hl = nullptr;           // [H]
+    int* d_hm = nullptr;           // [H]
+    int* d_pidx = nullptr;         // packed P_l^m index -> h, [(L)(L+1)/2], -1 if absent
+    int* d_apos = nullptr;         // analysis-table position of each harmonic [H]
+    Slot* d_slots = nullptr;       // [nslot]
+    int* d_slot_order = nullptr;   // [nslot]
+
+    // device: quadrature and tables
+    double* d_gp = nullptr;        // [Th]
+    double* d_gw = nullptr;        // [Th]
+    double* d_sin = nullptr;       // [Th]
+    double* d_tab = nullptr;       // 5 synthesis tables [5][H][Th2]
+    double* d_atab = nullptr;      // 3 analysis tables [3][Th2][H]
+    // device: radial mesh
+    double* d_r = nullptr;         // [7][N]  r^p, p = -3..3
+    double* d_dr1 = nullptr;       // [N][7]
+    double* d_dr2 = nullptr;
+    double* d_radlap = nullptr;
+    double* d_r1dr = nullptr;
+    double* d_intr2dr = nullptr;   // [N]
+    double* d_ll = nullptr;        // [4][L] ll1, ll11, rtll1, rtll11
+    // device: operators
+    double* d_lu = nullptr;        // [NLU][L][N][10]
+    int* d_piv = nullptr;          // [NLU][L][N]
+    double* d_y = nullptr;         // [NY][L][N][7]
+    double* d_gre = nullptr;       // [L][2][N]
+    double* d_invg = nullptr;      // [L][4]
+    int* d_info = nullptr;         // [NLU][L] dgbtrf info
+
+    // device: fields
+    double2* d_f[LSD_NUM_COLL] = {};   // state + current N
+    double2* d_hist[6] = {};           // saved N^n
+    double2* d_spec_in = nullptr;      // [SI_COUNT][H][N]
+    double2* d_spec_out = nullptr;     // [SO_COUNT][H][N]
+    double2* d_work = nullptr;         // solve scratch [6][H][N]
+    double2* d_fourier = nullptr;      // [18][N][M][Th]
+    double2* d_folded = nullptr;       // [8][N][M][2][Th2]
+    double* d_phys = nullptr;          // [18][N][Ph][Th]
+    double* d_nlphys = nullptr;        // [8][N][Ph][Th]
+    double2* d_tw = nullptr;           // FFT twiddles exp(2 pi i k/Ph) [Ph]
+    double* d_partial = nullptr;       // corrector norm partials
+    double* h_partial = nullptr;       // pinned copy
+    int n_partial_blocks = 0;
+
+    // host time-stepping state (modules/timestep.F90:12-20)
+    double t = 0.0;
+    int it = 0;
+    int step = 0;
+    double dterr = 0.0, lasterr = 1e99, corr_dt = 0.0;
+
+    size_t coll_elems() const { return (size_t)H * N; }
+    size_t phys_elems() const { return (size_t)Th * Ph * N; }
+    size_t fourier_elems() const { return (size_t)N * M * Th; }
+    size_t folded_elems() const { return (size_t)N * M * 2 * Th2; }
+};
+
+// precompute.hip
+int precompute_tables(Ctx& c);
+int precompute_mesh(Ctx& c);
+int build_operators(Ctx& c);
+
+// legendre.hip
+int launch_synthesis(Ctx& c, int kind, const double2* in0, const double2* in1, double2* out0,
+                     double2* out1);
+int launch_analysis(Ctx& c, int kind, const double2* g0, const double2* g1, const double2* g2,
+                     double2* o0, double2* o1, double2* o2);
+
+// fft.hip
+int fft_setup(Ctx& c);
+int launch_fft_c2r(Ctx& c, const double2* fourier, double* phys, int nfield);
+int launch_fft_r2c_fold(Ctx& c, const double* phys, double2* folded, int nfield);
+
+// spectral.hip
+int launch_vec_prep(Ctx& c, const double2* T, const double2* P, double2* q, double2* s, double2* t,
+                    double2* oq, double2* os, double2* ot);
+int launch_scal_prep(Ctx& c, const double2* C, double2* dC);
+int launch_spec_post(Ctx& c);
+int launch_layout_spec2coll(Ctx& c, const double* re, const double* im, double2* out);
+int launch_layout_coll2spec(Ctx& c, const double2* in, double* re, double* im);
+
+// nonlinear.hip
+int launch_nonlinear(Ctx& c);
+
+// solve.hip
+int launch_predictor(Ctx& c);
+int launch_corrector(Ctx& c, double* dterr);
+
+}  // namespace lsd

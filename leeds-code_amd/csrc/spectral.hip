@@ -1,0 +1,215 @@
+// Spectral-side operators on coll fields (radius fastest): the 7-point
+// radial stencils of var_coll_meshmult (modules/variables.F90:476-505),
+// the toroidal/poloidal -> qst map, the qst curl, and the r/(l(l+1))
+// r-hat.curl / r-hat.curl.curl projections of the nonlinear terms.
+// One work-item per (harmonic, radial point); neighbours along r come from
+// the same contiguous column so a wave reads one coalesced run.
+#include "lsd_internal.h"
+
+namespace lsd {
+
+// Row n of A.x for a band A stored per column j: A(n,j) = band[j*7 + 3+n-j];
+// contributions in ascending j, as var_coll_meshmult accumulates them.
+__device__ __forceinline__ double2 bandrow(const double* __restrict__ band, const double2* __restrict__ x, int n,
+                                           int N) {
+    int lo = n - LSD_KL > 0 ? n - LSD_KL : 0;
+    int hi = n + LSD_KL < N - 1 ? n + LSD_KL : N - 1;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int j = lo; j <= hi; j++) {
+        double a = band[j * 7 + (LSD_KL + n - j)];
+        acc.x = fma(a, x[j].x, acc.x);
+        acc.y = fma(a, x[j].y, acc.y);
+    }
+    return acc;
+}
+
+// var_coll_TorPol2qst_mag (variables.F90:565-605) followed by
+// var_coll_qstcurl (variables.F90:713-754).
+__global__ void k_vec_prep(int H, int N, const int* __restrict__ hl, const double* __restrict__ r,
+                           const double* __restrict__ r1dr, const double* __restrict__ ll,
+                           const double2* __restrict__ T, const double2* __restrict__ P, double2* __restrict__ q,
+                           double2* __restrict__ s, double2* __restrict__ t, double2* __restrict__ oq,
+                           double2* __restrict__ os, double2* __restrict__ ot, int L) {
+    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)H * N) return;
+    int h = (int)(idx / N), n = (int)(idx - (size_t)h * N);
+    int l = hl[h];
+    const double* rinv = r + 2 * N;
+    double ll1 = ll[l], rt = ll[2 * L + l];
+    const double2* Pc = P + (size_t)h * N;
+    const double2* Tc = T + (size_t)h * N;
+    double a = ll1 * rinv[n];
+    double2 qn = make_double2(Pc[n].x * a, Pc[n].y * a);
+    // s at the stencil of n, t likewise
+    int lo = n - LSD_KL > 0 ? n - LSD_KL : 0;
+    int hi = n + LSD_KL < N - 1 ? n + LSD_KL : N - 1;
+    double2 otn = make_double2(0.0, 0.0), osn = make_double2(0.0, 0.0), sn = make_double2(0.0, 0.0);
+    for (int j = lo; j <= hi; j++) {
+        double2 sj = bandrow(r1dr, Pc, j, N);
+        sj.x = sj.x * rt;
+        sj.y = sj.y * rt;
+        double2 tj = make_double2(-Tc[j].x * rt, -Tc[j].y * rt);
+        double b = r1dr[j * 7 + (LSD_KL + n - j)];
+        otn.x = fma(b, sj.x, otn.x);
+        otn.y = fma(b, sj.y, otn.y);
+        osn.x = fma(b, tj.x, osn.x);
+        osn.y = fma(b, tj.y, osn.y);
+        if (j == n) sn = sj;
+    }
+    double2 tn = make_double2(-Tc[n].x * rt, -Tc[n].y * rt);
+    double c = rt * rinv[n];
+    size_t o = (size_t)h * N + n;
+    q[o] = qn;
+    s[o] = sn;
+    t[o] = tn;
+    ot[o] = make_double2(otn.x - c * qn.x, otn.y - c * qn.y);
+    os[o] = make_double2(-osn.x, -osn.y);
+    double d = -rt * rinv[n];
+    oq[o] = make_double2(tn.x * d, tn.y * d);
+}
+
+// dC/dr by the first-derivative stencil (codensity.F90:207).
+__global__ void k_scal_prep(int H, int N, const double* __restrict__ dr1, const double2* __restrict__ C,
+                            double2* __restrict__ dC) {
+    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)H * N) return;
+    int h = (int)(idx / N), n = (int)(idx - (size_t)h * N);
+    dC[idx] = bandrow(dr1, C + (size_t)h * N, n, N);
+}
+
+struct PostParams {
+    double Ra, Ra_comp, Pm2, Pr, Sc;
+    double src_c, src_x;  // d_source, d_source_comp
+    int cod, comp, mag;
+};
+
+// Nonlinear terms in coll form (nonlinear.F90:148-197, 277-288, 341-348,
+// 432-471): buoyancy on q, r/(l(l+1)) r.curl and r.curl.curl projections,
+// l = 0 zeroed, sources on the l = 0 scalar modes.
+__global__ void k_spec_post(int H, int N, int L, const int* __restrict__ hl, const double* __restrict__ r,
+                            const double* __restrict__ r1dr, const double* __restrict__ ll, PostParams pp,
+                            const double2* __restrict__ so, const double2* __restrict__ Cst,
+                            const double2* __restrict__ Xst, double2* __restrict__ NuT, double2* __restrict__ NuP,
+                            double2* __restrict__ NC, double2* __restrict__ NX, double2* __restrict__ NbT,
+                            double2* __restrict__ NbP) {
+    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t HN = (size_t)H * N;
+    if (idx >= HN) return;
+    int h = (int)(idx / N), n = (int)(idx - (size_t)h * N);
+    int l = hl[h];
+    const double* rinv = r + 2 * N;
+    double rn = r[4 * N + n];
+    double rt11 = ll[3 * L + l];
+    const double2* uq = so + SO_UQ * HN;
+    const double2* us = so + SO_US * HN + (size_t)h * N;
+    const double2* ut = so + SO_UT * HN;
+    // buoyancy (nonlinear.F90:158-185)
+    double2 cq = uq[idx];
+    double d2 = pp.Ra * rn * pp.Pm2 / pp.Pr;
+    cq.x = cq.x + d2 * Cst[idx].x;
+    cq.y = cq.y + d2 * Cst[idx].y;
+    d2 = pp.Ra_comp * rn * pp.Pm2 / pp.Sc;
+    cq.x = cq.x + d2 * Xst[idx].x;
+    cq.y = cq.y + d2 * Xst[idx].y;
+    double2 nt = make_double2(-ut[idx].x * rt11, -ut[idx].y * rt11);
+    double2 v = bandrow(r1dr, us, n, N);
+    v.x = v.x * rt11;
+    v.y = v.y * rt11;
+    double2 np = make_double2(cq.x * rinv[n] - v.x, cq.y * rinv[n] - v.y);
+    if (h == 0) nt = np = make_double2(0.0, 0.0);
+    NuT[idx] = nt;
+    NuP[idx] = np;
+    if (pp.cod) {
+        double2 c = so[SO_C * HN + idx];
+        if (h == 0) c.x = c.x + pp.src_c;
+        NC[idx] = c;
+    }
+    if (pp.comp) {
+        double2 c = so[SO_X * HN + idx];
+        if (h == 0) c.x = c.x + pp.src_x;
+        NX[idx] = c;
+    }
+    if (pp.mag) {
+        const double2* eq = so + SO_EQ * HN;
+        const double2* es = so + SO_ES * HN + (size_t)h * N;
+        const double2* et = so + SO_ET * HN;
+        double2 bp = make_double2(-et[idx].x * rt11, -et[idx].y * rt11);
+        double2 w = bandrow(r1dr, es, n, N);
+        w.x = w.x * rt11;
+        w.y = w.y * rt11;
+        double2 bt = make_double2(eq[idx].x * rinv[n] - w.x, eq[idx].y * rinv[n] - w.y);
+        if (h == 0) bp = bt = make_double2(0.0, 0.0);
+        NbP[idx] = bp;
+        NbT[idx] = bt;
+    }
+}
+
+// spec re[n*H+h], im -> coll double2 [h][n] and back (the serial
+// var_coll2spec / var_spec2coll transposes, variables.F90:246-268, 365-389).
+__global__ void k_spec2coll(int H, int N, const double* __restrict__ re, const double* __restrict__ im,
+                            double2* __restrict__ out) {
+    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)H * N) return;
+    int h = (int)(idx / N), n = (int)(idx - (size_t)h * N);
+    out[idx] = make_double2(re[(size_t)n * H + h], im[(size_t)n * H + h]);
+}
+__global__ void k_coll2spec(int H, int N, const double2* __restrict__ in, double* __restrict__ re,
+                            double* __restrict__ im) {
+    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)H * N) return;
+    int n = (int)(idx / H), h = (int)(idx - (size_t)n * H);
+    double2 v = in[(size_t)h * N + n];
+    re[idx] = v.x;
+    im[idx] = v.y;
+}
+
+static dim3 grid1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+int launch_vec_prep(Ctx& c, const double2* T, const double2* P, double2* q, double2* s, double2* t, double2* oq,
+                    double2* os, double2* ot) {
+    hipLaunchKernelGGL(k_vec_prep, grid1(c.coll_elems()), dim3(256), 0, c.stream, c.H, c.N, c.d_hl, c.d_r,
+                       c.d_r1dr, c.d_ll, T, P, q, s, t, oq, os, ot, c.L);
+    LSD_HIP(hipGetLastError());
+    return LSD_OK;
+}
+
+int launch_scal_prep(Ctx& c, const double2* C, double2* dC) {
+    hipLaunchKernelGGL(k_scal_prep, grid1(c.coll_elems()), dim3(256), 0, c.stream, c.H, c.N, c.d_dr1, C, dC);
+    LSD_HIP(hipGetLastError());
+    return LSD_OK;
+}
+
+int launch_spec_post(Ctx& c) {
+    PostParams pp;
+    const lsd_config& k = c.cfg;
+    pp.Ra = k.Ra;
+    pp.Ra_comp = k.Ra_comp;
+    pp.Pm2 = k.Pm * k.Pm;
+    pp.Pr = k.Pr;
+    pp.Sc = k.Sc;
+    pp.src_c = k.source;
+    pp.src_x = k.source_comp;
+    pp.cod = k.cod;
+    pp.comp = k.comp;
+    pp.mag = k.mag;
+    hipLaunchKernelGGL(k_spec_post, grid1(c.coll_elems()), dim3(256), 0, c.stream, c.H, c.N, c.L, c.d_hl, c.d_r,
+                       c.d_r1dr, c.d_ll, pp, c.d_spec_out, c.d_f[LSD_COD], c.d_f[LSD_COMP], c.d_f[LSD_N_VEL_TOR],
+                       c.d_f[LSD_N_VEL_POL], c.d_f[LSD_N_COD], c.d_f[LSD_N_COMP], c.d_f[LSD_N_MAG_TOR],
+                       c.d_f[LSD_N_MAG_POL]);
+    LSD_HIP(hipGetLastError());
+    return LSD_OK;
+}
+
+int launch_layout_spec2coll(Ctx& c, const double* re, const double* im, double2* out) {
+    hipLaunchKernelGGL(k_spec2coll, grid1(c.coll_elems()), dim3(256), 0, c.stream, c.H, c.N, re, im, out);
+    LSD_HIP(hipGetLastError());
+    return LSD_OK;
+}
+
+int launch_layout_coll2spec(Ctx& c, const double2* in, double* re, double* im) {
+    hipLaunchKernelGGL(k_coll2spec, grid1(c.coll_elems()), dim3(256), 0, c.stream, c.H, c.N, in, re, im);
+    LSD_HIP(hipGetLastError());
+    return LSD_OK;
+}
+
+}  // namespace lsd

@@ -1,0 +1,104 @@
+"""Run parameters of the per-timestep path (ctypes mirror of ``lsd_config``).
+
+Field names and meanings follow the reference's compile-time parameters
+(Scripts/parameters.F90_template:19-264); here they are runtime values so one
+build serves every truncation.  ``Th``/``Ph`` are decoupled from ``L``/``M``:
+0 selects the reference's 3/2 rule (parameters.F90_template:257-259).
+"""
+import ctypes
+import math
+
+PLAIN_F64 = 0
+MIXED_F32 = 1
+MIXED_BF16X3 = 2
+
+
+class LsdConfig(ctypes.Structure):
+    _fields_ = [
+        ("L", ctypes.c_int), ("M", ctypes.c_int), ("Mp", ctypes.c_int), ("N", ctypes.c_int),
+        ("Th", ctypes.c_int), ("Ph", ctypes.c_int),
+        ("vel", ctypes.c_int), ("cod", ctypes.c_int), ("comp", ctypes.c_int), ("mag", ctypes.c_int),
+        ("vel_bc", ctypes.c_int), ("cod_bc", ctypes.c_int), ("comp_bc", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("rratio", ctypes.c_double), ("Ra", ctypes.c_double), ("Ra_comp", ctypes.c_double),
+        ("Pr", ctypes.c_double), ("Sc", ctypes.c_double), ("Pm", ctypes.c_double),
+        ("E", ctypes.c_double), ("dt", ctypes.c_double), ("implicit", ctypes.c_double),
+        ("dterr", ctypes.c_double), ("source", ctypes.c_double), ("source_comp", ctypes.c_double),
+    ]
+
+    def sizes(self):
+        """(H, Th, Ph, Th2) with the reference's rules."""
+        H = self.L * self.M - self.Mp * (self.M - 1) * self.M // 2
+        Th = self.Th if self.Th > 0 else self.L * 3 // 2
+        Ph = self.Ph if self.Ph > 0 else (self.M * 3 // 2) * 2
+        return H, Th, Ph, (Th + 1) // 2
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def make_config(**kw):
+    """Defaults are the reference template's Boussinesq values
+    (Scripts/parameters.F90_template) with the test cases' d_implicit = 0.5
+    (tests/parameters-case*.F90)."""
+    base = dict(L=32, M=32, Mp=1, N=32, Th=0, Ph=0, vel=1, cod=1, comp=0, mag=1,
+                vel_bc=1, cod_bc=1, comp_bc=1, precision=PLAIN_F64,
+                rratio=0.35, Ra=6.5e4, Ra_comp=1.0, Pr=1.0, Sc=1.0, Pm=5.0, E=1.0e-3,
+                dt=1.0e-4, implicit=0.5, dterr=0.025, source=0.0, source_comp=0.0)
+    base.update(kw)
+    c = LsdConfig()
+    for k, v in base.items():
+        setattr(c, k, v)
+    return c
+
+
+# Critical modified Rayleigh number at E = 1e-3, Pr = 1, r_i/r_o = 0.35
+# (Christensen et al. 2001, quoted beside the benchmark's Ra = 100 in
+# Benchmark_files/Boussinesq/Christensen_case0_benchmark/INFO), scaled to
+# other E with the asymptotic E^(-1/3) law of rotating convection onset.
+RA_MOD_CRIT_E1EM3 = 55.91
+
+
+def unified_ra(ra_mod, E, rratio=0.35):
+    """Code Rayleigh number d_Ra from a modified Rayleigh number:
+    d_Ra = Ra_mod / (r_o E) with r_o = 1/(1 - rratio) (the benchmark's
+    100 -> 6.5e4 conversion, Christensen_case0_benchmark/INFO:3-5)."""
+    ro = 1.0 / (1.0 - rratio)
+    return ra_mod / (ro * E)
+
+
+def ra_crit_estimate(E, rratio=0.35):
+    return unified_ra(RA_MOD_CRIT_E1EM3 * (1e-3 / E) ** (1.0 / 3.0), E, rratio)
+
+
+def config1():
+    """Plain method, Boussinesq MHD, no composition: E=1e-3, Ra_mod=100
+    (d_Ra = 6.5e4), Pr=1, Pm=5, N=32, L=M=32, 48x96 grid, dt=1e-4."""
+    return make_config(L=32, M=32, N=32, Th=48, Ph=96, comp=0, Ra=6.5e4, Pm=5.0, E=1e-3, dt=1e-4)
+
+
+def config3(precision=PLAIN_F64):
+    """Full timestep: Boussinesq with codensity and composition, L=M=170,
+    N=96, 256x512, E=1e-4, Ra = 5 x critical (estimate above), Pr=1, Pm=2,
+    dt=2e-6."""
+    ra = 5.0 * ra_crit_estimate(1e-4)
+    return make_config(L=170, M=170, N=96, Th=256, Ph=512, comp=1, Ra=ra, Ra_comp=ra,
+                       Pr=1.0, Sc=1.0, Pm=2.0, E=1e-4, dt=2e-6, precision=precision)
+
+
+def config_small(**kw):
+    """A reduced case for quick parity checks (odd sizes on purpose)."""
+    base = dict(L=12, M=12, N=16, Th=18, Ph=36, comp=1, Ra=6.5e4, Ra_comp=2.0e4, Pm=5.0,
+                E=1e-3, dt=1e-4)
+    base.update(kw)
+    return make_config(**base)
+
+
+def ra_table():
+    return {"E=1e-4": ra_crit_estimate(1e-4), "E=1e-3": ra_crit_estimate(1e-3)}
+
+
+if __name__ == "__main__":
+    for name, c in (("config1", config1()), ("config3", config3())):
+        print(name, c.sizes(), "Ra=%.6g" % c.Ra)
+    print(math.pi)

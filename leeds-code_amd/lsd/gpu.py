@@ -1,0 +1,216 @@
+"""ctypes binding of liblsdgpu.so (include/lsd.h).
+
+Mirrors the reference's module interfaces for the per-timestep path:
+``transform`` / ``nonlinear`` / ``predictor`` / ``corrector`` are the phases
+of program/main.F90:101-157, ``step`` is the whole loop body, and the
+``tra_*`` methods are the drop-ins for modules/transform.fftw3.F90's
+tra_spec2phys, tra_phys2spec, tra_grad, tra_qst2rtp and tra_rtp2qst.
+Errors raise ``LsdError`` with the library's message (the reference stops
+the program with the same text).
+
+There is no CPU fallback: if the HIP library or a GPU is missing, every
+call fails.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .config import LsdConfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "liblsdgpu.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+EXPORTS = (
+    "lsd_last_error", "lsd_config_sizes", "lsd_create", "lsd_destroy", "lsd_set_coll", "lsd_get_coll",
+    "lsd_get_phys", "lsd_get_table", "lsd_get_mesh", "lsd_transform", "lsd_nonlinear", "lsd_predictor",
+    "lsd_corrector", "lsd_step", "lsd_set_dt", "lsd_tra_spec2phys", "lsd_tra_phys2spec", "lsd_tra_grad",
+    "lsd_tra_qst2rtp", "lsd_tra_rtp2qst", "lsd_sync",
+)
+
+
+class LsdError(RuntimeError):
+    pass
+
+
+class StepStats(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int), ("evaluations", ctypes.c_int), ("last_its", ctypes.c_int),
+                ("max_its", ctypes.c_int), ("last_dterr", ctypes.c_double), ("t", ctypes.c_double)]
+
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load the HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise LsdError("liblsdgpu.so not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(path)
+    lib.lsd_last_error.restype = ctypes.c_char_p
+    lib.lsd_config_sizes.argtypes = [ctypes.POINTER(LsdConfig)] + [ctypes.POINTER(ctypes.c_int)] * 3
+    lib.lsd_create.argtypes = [ctypes.POINTER(LsdConfig), ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+    lib.lsd_destroy.argtypes = [ctypes.c_void_p]
+    lib.lsd_destroy.restype = None
+    for n in ("lsd_set_coll", "lsd_get_coll"):
+        getattr(lib, n).argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+    for n in ("lsd_get_phys", "lsd_get_table", "lsd_get_mesh"):
+        getattr(lib, n).argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+    for n in ("lsd_transform", "lsd_nonlinear", "lsd_predictor", "lsd_sync"):
+        getattr(lib, n).argtypes = [ctypes.c_void_p]
+    lib.lsd_corrector.argtypes = [ctypes.c_void_p, _dp]
+    lib.lsd_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(StepStats)]
+    lib.lsd_set_dt.argtypes = [ctypes.c_void_p, ctypes.c_double]
+    lib.lsd_tra_spec2phys.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+    lib.lsd_tra_phys2spec.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+    lib.lsd_tra_grad.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
+    lib.lsd_tra_qst2rtp.argtypes = [ctypes.c_void_p] + [_dp] * 9
+    lib.lsd_tra_rtp2qst.argtypes = [ctypes.c_void_p] + [_dp] * 9
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    if a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise LsdError("arrays must be C-contiguous float64")
+    return a.ctypes.data_as(_dp)
+
+
+def config_sizes(cfg):
+    lib = load()
+    H, Th, Ph = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    c = LsdConfig.from_buffer_copy(cfg)
+    rc = lib.lsd_config_sizes(ctypes.byref(c), ctypes.byref(H), ctypes.byref(Th), ctypes.byref(Ph))
+    if rc:
+        raise LsdError(lib.lsd_last_error().decode())
+    return H.value, Th.value, Ph.value
+
+
+class Context:
+    """One device-resident run (lsd_ctx)."""
+
+    def __init__(self, cfg, device=0):
+        self.lib = load()
+        self.cfg = LsdConfig.from_buffer_copy(cfg)
+        self.H, self.Th, self.Ph, self.Th2 = cfg.sizes()
+        self.N = cfg.N
+        h = ctypes.c_void_p()
+        self._chk(self.lib.lsd_create(ctypes.byref(self.cfg), device, ctypes.byref(h)), "lsd_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.lib.lsd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise LsdError("%s: %s (code %d)" % (what, self.lib.lsd_last_error().decode(), rc))
+
+    # --- fields
+    def set_coll(self, field, re, im):
+        re = np.ascontiguousarray(re, dtype=np.float64)
+        im = np.ascontiguousarray(im, dtype=np.float64)
+        assert re.size == self.H * self.N and im.size == self.H * self.N
+        self._chk(self.lib.lsd_set_coll(self.h, field, _p(re), _p(im)), "lsd_set_coll")
+
+    def get_coll(self, field):
+        re = np.zeros((self.H, self.N))
+        im = np.zeros((self.H, self.N))
+        self._chk(self.lib.lsd_get_coll(self.h, field, _p(re), _p(im)), "lsd_get_coll")
+        return re, im
+
+    def set_state(self, state):
+        from .initial import FIELD_ID
+        for name, (re, im) in state.items():
+            self.set_coll(FIELD_ID[name], re, im)
+
+    def get_phys(self, field):
+        p = np.zeros((self.N, self.Ph, self.Th))
+        self._chk(self.lib.lsd_get_phys(self.h, field, _p(p)), "lsd_get_phys")
+        return p
+
+    def table(self, t):
+        n = self.Th if t in (0, 1) else self.H * self.Th2
+        out = np.zeros(n)
+        self._chk(self.lib.lsd_get_table(self.h, t, _p(out)), "lsd_get_table")
+        return out if t in (0, 1) else out.reshape(self.H, self.Th2)
+
+    def mesh(self, w):
+        n = self.N if w in (0, 5) else 7 * self.N
+        out = np.zeros(n)
+        self._chk(self.lib.lsd_get_mesh(self.h, w, _p(out)), "lsd_get_mesh")
+        return out if w in (0, 5) else out.reshape(self.N, 7)
+
+    # --- phases
+    def transform(self):
+        self._chk(self.lib.lsd_transform(self.h), "lsd_transform")
+
+    def nonlinear(self):
+        self._chk(self.lib.lsd_nonlinear(self.h), "lsd_nonlinear")
+
+    def predictor(self):
+        self._chk(self.lib.lsd_predictor(self.h), "lsd_predictor")
+
+    def corrector(self):
+        d = ctypes.c_double(0.0)
+        self._chk(self.lib.lsd_corrector(self.h, ctypes.byref(d)), "lsd_corrector")
+        return d.value
+
+    def step(self, n=1):
+        st = StepStats()
+        self._chk(self.lib.lsd_step(self.h, n, ctypes.byref(st)), "lsd_step")
+        return st
+
+    def sync(self):
+        self._chk(self.lib.lsd_sync(self.h), "lsd_sync")
+
+    def set_dt(self, dt):
+        self._chk(self.lib.lsd_set_dt(self.h, dt), "lsd_set_dt")
+
+    # --- drop-in transforms (spec re[n, h]; phys [n, k, j])
+    def spec2phys(self, sre, sim):
+        p = np.zeros((self.N, self.Ph, self.Th))
+        self._chk(self.lib.lsd_tra_spec2phys(self.h, _p(sre), _p(sim), _p(p)), "lsd_tra_spec2phys")
+        return p
+
+    def phys2spec(self, p):
+        sre = np.zeros((self.N, self.H))
+        sim = np.zeros((self.N, self.H))
+        self._chk(self.lib.lsd_tra_phys2spec(self.h, _p(np.ascontiguousarray(p)), _p(sre), _p(sim)),
+                  "lsd_tra_phys2spec")
+        return sre, sim
+
+    def grad(self, sre, sim):
+        t = np.zeros((self.N, self.Ph, self.Th))
+        p = np.zeros((self.N, self.Ph, self.Th))
+        self._chk(self.lib.lsd_tra_grad(self.h, _p(sre), _p(sim), _p(t), _p(p)), "lsd_tra_grad")
+        return t, p
+
+    def qst2rtp(self, q, s, t):
+        out = [np.zeros((self.N, self.Ph, self.Th)) for _ in range(3)]
+        args = [_p(a) for a in (q[0], q[1], s[0], s[1], t[0], t[1])] + [_p(o) for o in out]
+        self._chk(self.lib.lsd_tra_qst2rtp(self.h, *args), "lsd_tra_qst2rtp")
+        return out
+
+    def rtp2qst(self, r, th, ph):
+        out = [np.zeros((self.N, self.H)) for _ in range(6)]
+        args = [_p(np.ascontiguousarray(a)) for a in (r, th, ph)] + [_p(o) for o in out]
+        self._chk(self.lib.lsd_tra_rtp2qst(self.h, *args), "lsd_tra_rtp2qst")
+        return (out[0], out[1]), (out[2], out[3]), (out[4], out[5])
