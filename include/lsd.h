@@ -197,6 +197,24 @@ int lsd_tra_rtp2qst(lsd_ctx* ctx, const double* r, const double* th, const doubl
  * modules/dyn_mpi.F90:81-92). */
 int lsd_sync(lsd_ctx* ctx);
 
+/* Kernel classes for per-launch timing. */
+#define LSD_K_SYNTH    0   /* Legendre synthesis                                */
+#define LSD_K_ANAL     1   /* Legendre analysis                                 */
+#define LSD_K_FFT_INV  2   /* phi FFT, Fourier -> physical                      */
+#define LSD_K_FFT_FWD  3   /* phi FFT, physical -> folded Fourier               */
+#define LSD_K_NONLIN   4   /* pointwise nonlinear products                      */
+#define LSD_K_SPEC     5   /* radial stencils on spectral fields                */
+#define LSD_K_SOLVE    6   /* banded predictor / corrector solves               */
+#define LSD_K_COUNT    7
+
+/* Timing hooks (no counterpart in the reference beyond get_time): between
+ * begin and end, every launch of the given class is bracketed by HIP events
+ * on the context's stream; end returns the summed kernel time, the launch
+ * count and the algorithmic work of those launches (flops for SYNTH/ANAL,
+ * HBM bytes for the other classes). */
+int lsd_profile_begin(lsd_ctx* ctx, int kernel_class);
+int lsd_profile_end(lsd_ctx* ctx, double* total_ms, int* launches, double* work);
+
 #ifdef __cplusplus
 }
 #endif

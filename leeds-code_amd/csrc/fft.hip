@@ -257,6 +257,7 @@ int launch_fft_c2r(Ctx& c, const double2* fourier, double* phys, int nfield) {
     int nf = nfft_per_block(c, (c.Th + 1) / 2);
     dim3 grid((c.Th + 2 * nf - 1) / (2 * nf), c.N, nfield);
     size_t lds = sizeof(double2) * nf * c.Ph;
+    ProfScope ps(c, LSD_K_FFT_INV, (double)nfield * c.N * ((double)c.M * c.Th * 16 + (double)c.Ph * c.Th * 8));
     hipLaunchKernelGGL(k_fft_c2r, grid, dim3(FFT_THREADS), lds, c.stream, p, nf, c.N, c.Th, c.M, fourier, phys,
                        c.d_tw);
     LSD_HIP(hipGetLastError());
@@ -268,6 +269,7 @@ int launch_fft_r2c_fold(Ctx& c, const double* phys, double2* folded, int nfield)
     int nf = nfft_per_block(c, c.Th2);
     dim3 grid((c.Th2 + nf - 1) / nf, c.N, nfield);
     size_t lds = sizeof(double2) * nf * c.Ph;
+    ProfScope ps(c, LSD_K_FFT_FWD, (double)nfield * c.N * ((double)c.Ph * c.Th * 8 + (double)c.M * 2 * c.Th2 * 16));
     hipLaunchKernelGGL(k_fft_r2c_fold, grid, dim3(FFT_THREADS), lds, c.stream, p, nf, c.N, c.Th, c.Th2, c.M, phys,
                        folded, c.d_tw);
     LSD_HIP(hipGetLastError());

@@ -171,6 +171,9 @@ constexpr int SYN_S = 8;
 int launch_synthesis(Ctx& c, int kind, const double2* in0, const double2* in1, double2* out0, double2* out1) {
     size_t HT = (size_t)c.H * c.Th2;
     dim3 grid(c.nslot, (c.Th2 + 63) / 64, (c.N + SYN_S - 1) / SYN_S);
+    // 4 flops per (harmonic, theta point, shell) and product unit
+    const double unit = 4.0 * (double)c.H * c.Th2 * c.N;
+    ProfScope ps(c, LSD_K_SYNTH, unit * (kind == SYN_SCALAR ? 1 : kind == SYN_VECTOR ? 4 : 2));
     const double* rinv = c.d_r + 2 * c.N;
     if (kind == SYN_SCALAR) {
         hipLaunchKernelGGL((k_synth_scalar<SYN_S>), grid, dim3(64), 0, c.stream, c.N, c.Th, c.Th2, c.M,
@@ -280,6 +283,8 @@ int launch_analysis(Ctx& c, int kind, const double2* g0, const double2* g1, cons
     for (auto& s : c.slots) maxlen = maxlen > s.len ? maxlen : s.len;
     int ktiles = ((maxlen + 1) / 2 + 63) / 64;
     dim3 grid(c.nslot, 2 * ktiles, (c.N + ANA_S - 1) / ANA_S);
+    const double unit = 4.0 * (double)c.H * c.Th2 * c.N;
+    ProfScope ps(c, LSD_K_ANAL, unit * (kind == ANA_SCALAR ? 1 : 5));
     if (kind == ANA_SCALAR) {
         hipLaunchKernelGGL((k_anal<ANA_S, false>), grid, dim3(64), 0, c.stream, c.N, c.Th2, c.M, c.H, c.d_slots,
                            c.d_slot_order, c.d_atab, g0, g0, g0, o0, o0, o0);

@@ -81,6 +81,7 @@ static void release(Ctx* c) {
     for (int k = 0; k < 6; k++)
         if (c->d_hist[k]) (void)hipFree(c->d_hist[k]);
     if (c->h_partial) (void)hipHostFree(c->h_partial);
+    for (hipEvent_t e : c->prof_events) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -177,6 +178,30 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
     if ((rc = build_operators(*c))) return rc;
     LSD_HIP(hipStreamSynchronize(st));
     return LSD_OK;
+}
+
+// ------------------------------------------------------------ timing
+
+ProfScope::ProfScope(Ctx& ctx, int cls, double work) : c(ctx), on(ctx.prof_class == cls), idx(-1) {
+    if (!on) return;
+    if ((size_t)(c.prof_used + 2) > c.prof_events.size()) {
+        for (int k = 0; k < 64; k++) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) {
+                on = false;
+                return;
+            }
+            c.prof_events.push_back(e);
+        }
+    }
+    idx = c.prof_used;
+    c.prof_used += 2;
+    c.prof_work += work;
+    (void)hipEventRecord(c.prof_events[idx], c.stream);
+}
+
+ProfScope::~ProfScope() {
+    if (on && idx >= 0) (void)hipEventRecord(c.prof_events[idx + 1], c.stream);
 }
 
 // ------------------------------------------------------------- phases
@@ -499,6 +524,37 @@ int lsd_step(lsd_ctx* ctx, int nsteps, lsd_step_stats* stats) {
     int rc = lsd::step(c, nsteps, stats);
     if (rc) return rc;
     LSD_HIP(hipStreamSynchronize(c.stream));
+    return LSD_OK;
+}
+
+int lsd_profile_begin(lsd_ctx* ctx, int kernel_class) {
+    CTX_OR_FAIL(ctx);
+    if (kernel_class < 0 || kernel_class >= LSD_K_COUNT) {
+        lsd::set_error("lsd_profile_begin: unknown kernel class");
+        return LSD_E_ARG;
+    }
+    LSD_HIP(hipStreamSynchronize(c.stream));
+    c.prof_class = kernel_class;
+    c.prof_used = 0;
+    c.prof_work = 0.0;
+    return LSD_OK;
+}
+
+int lsd_profile_end(lsd_ctx* ctx, double* total_ms, int* launches, double* work) {
+    CTX_OR_FAIL(ctx);
+    LSD_HIP(hipStreamSynchronize(c.stream));
+    double tot = 0.0;
+    for (int k = 0; k + 1 < c.prof_used; k += 2) {
+        float ms = 0.f;
+        LSD_HIP(hipEventElapsedTime(&ms, c.prof_events[k], c.prof_events[k + 1]));
+        tot += ms;
+    }
+    if (total_ms) *total_ms = tot;
+    if (launches) *launches = c.prof_used / 2;
+    if (work) *work = c.prof_work;
+    c.prof_class = -1;
+    c.prof_used = 0;
+    c.prof_work = 0.0;
     return LSD_OK;
 }
 

@@ -128,6 +128,12 @@ struct Ctx {
     double* h_partial = nullptr;       // pinned copy
     int n_partial_blocks = 0;
 
+    // per-launch timing of one kernel class (lsd_profile_begin/end)
+    int prof_class = -1;
+    std::vector<hipEvent_t> prof_events;  // pool, pairs (start, stop)
+    int prof_used = 0;
+    double prof_work = 0.0;
+
     // host time-stepping state (modules/timestep.F90:12-20)
     double t = 0.0;
     int it = 0;
@@ -138,6 +144,16 @@ struct Ctx {
     size_t phys_elems() const { return (size_t)Th * Ph * N; }
     size_t fourier_elems() const { return (size_t)N * M * Th; }
     size_t folded_elems() const { return (size_t)N * M * 2 * Th2; }
+};
+
+// Brackets one launch of a kernel class with events when that class is
+// being timed; `work` is the launch's algorithmic flops or bytes.
+struct ProfScope {
+    Ctx& c;
+    bool on;
+    int idx;
+    ProfScope(Ctx& ctx, int cls, double work);
+    ~ProfScope();
 };
 
 // precompute.hip

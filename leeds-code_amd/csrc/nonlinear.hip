@@ -64,6 +64,9 @@ int launch_nonlinear(Ctx& c) {
     p.comp = k.comp;
     p.mag = k.mag;
     size_t np = c.phys_elems();
+    int fin = 6 + (k.mag ? 6 : 0) + (k.cod ? 3 : 0) + (k.comp ? 3 : 0);
+    int fout = 3 + (k.mag ? 3 : 0) + (k.cod ? 1 : 0) + (k.comp ? 1 : 0);
+    ProfScope ps(c, LSD_K_NONLIN, (double)np * 8.0 * (fin + fout));
     hipLaunchKernelGGL(k_nonlinear, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c.stream, np, c.Th, p,
                        c.d_gp, c.d_sin, c.d_phys, c.d_nlphys);
     LSD_HIP(hipGetLastError());

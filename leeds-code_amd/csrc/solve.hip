@@ -256,8 +256,14 @@ static SolveArgs make_args(Ctx& c) {
     return a;
 }
 
+static int active_fields(const Ctx& c) {
+    return 2 + (c.cfg.cod ? 1 : 0) + (c.cfg.comp ? 1 : 0) + (c.cfg.mag ? 2 : 0);
+}
+
 int launch_predictor(Ctx& c) {
     SolveArgs a = make_args(c);
+    // per field: read state and N, write state and saved N
+    ProfScope ps(c, LSD_K_SOLVE, 4.0 * 16.0 * c.coll_elems() * active_fields(c));
     dim3 grid((c.H + 63) / 64, 6);
     hipLaunchKernelGGL(k_predict, grid, dim3(64), 0, c.stream, a);
     LSD_HIP(hipGetLastError());
@@ -272,7 +278,11 @@ int launch_corrector(Ctx& c, double* dterr) {
     int nb = (c.H + 63) / 64;
     LSD_HIP(hipMemsetAsync(c.d_partial, 0, sizeof(double) * 8 * nb, c.stream));
     dim3 grid(nb, 6);
-    hipLaunchKernelGGL(k_correct, grid, dim3(64), 0, c.stream, a);
+    {
+        // per field: read state, N and saved N, write state and saved N
+        ProfScope ps(c, LSD_K_SOLVE, 5.0 * 16.0 * c.coll_elems() * active_fields(c));
+        hipLaunchKernelGGL(k_correct, grid, dim3(64), 0, c.stream, a);
+    }
     LSD_HIP(hipGetLastError());
     LSD_HIP(hipMemcpyAsync(c.h_partial, c.d_partial, sizeof(double) * 8 * nb, hipMemcpyDeviceToHost, c.stream));
     LSD_HIP(hipStreamSynchronize(c.stream));

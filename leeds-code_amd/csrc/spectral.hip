@@ -167,6 +167,7 @@ static dim3 grid1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
 int launch_vec_prep(Ctx& c, const double2* T, const double2* P, double2* q, double2* s, double2* t, double2* oq,
                     double2* os, double2* ot) {
+    ProfScope ps(c, LSD_K_SPEC, 8.0 * 16.0 * c.coll_elems());
     hipLaunchKernelGGL(k_vec_prep, grid1(c.coll_elems()), dim3(256), 0, c.stream, c.H, c.N, c.d_hl, c.d_r,
                        c.d_r1dr, c.d_ll, T, P, q, s, t, oq, os, ot, c.L);
     LSD_HIP(hipGetLastError());
@@ -174,6 +175,7 @@ int launch_vec_prep(Ctx& c, const double2* T, const double2* P, double2* q, doub
 }
 
 int launch_scal_prep(Ctx& c, const double2* C, double2* dC) {
+    ProfScope ps(c, LSD_K_SPEC, 2.0 * 16.0 * c.coll_elems());
     hipLaunchKernelGGL(k_scal_prep, grid1(c.coll_elems()), dim3(256), 0, c.stream, c.H, c.N, c.d_dr1, C, dC);
     LSD_HIP(hipGetLastError());
     return LSD_OK;
@@ -182,6 +184,10 @@ int launch_scal_prep(Ctx& c, const double2* C, double2* dC) {
 int launch_spec_post(Ctx& c) {
     PostParams pp;
     const lsd_config& k = c.cfg;
+    // reads: 3 u terms + C + X (+ C, X, E terms), writes the N fields
+    int nrd = 5 + (k.cod ? 1 : 0) + (k.comp ? 1 : 0) + (k.mag ? 3 : 0);
+    int nwr = 2 + (k.cod ? 1 : 0) + (k.comp ? 1 : 0) + (k.mag ? 2 : 0);
+    ProfScope ps(c, LSD_K_SPEC, (double)(nrd + nwr) * 16.0 * c.coll_elems());
     pp.Ra = k.Ra;
     pp.Ra_comp = k.Ra_comp;
     pp.Pm2 = k.Pm * k.Pm;

@@ -27,8 +27,12 @@ EXPORTS = (
     "lsd_last_error", "lsd_config_sizes", "lsd_create", "lsd_destroy", "lsd_set_coll", "lsd_get_coll",
     "lsd_get_phys", "lsd_get_table", "lsd_get_mesh", "lsd_transform", "lsd_nonlinear", "lsd_predictor",
     "lsd_corrector", "lsd_step", "lsd_set_dt", "lsd_tra_spec2phys", "lsd_tra_phys2spec", "lsd_tra_grad",
-    "lsd_tra_qst2rtp", "lsd_tra_rtp2qst", "lsd_sync",
+    "lsd_tra_qst2rtp", "lsd_tra_rtp2qst", "lsd_sync", "lsd_profile_begin", "lsd_profile_end",
 )
+
+# kernel classes for lsd_profile_begin
+K_SYNTH, K_ANAL, K_FFT_INV, K_FFT_FWD, K_NONLIN, K_SPEC, K_SOLVE = range(7)
+K_NAMES = ("synth", "anal", "fft_inv", "fft_fwd", "nonlin", "spec", "solve")
 
 
 class LsdError(RuntimeError):
@@ -70,6 +74,8 @@ def load(path=LIB_PATH):
     lib.lsd_tra_grad.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
     lib.lsd_tra_qst2rtp.argtypes = [ctypes.c_void_p] + [_dp] * 9
     lib.lsd_tra_rtp2qst.argtypes = [ctypes.c_void_p] + [_dp] * 9
+    lib.lsd_profile_begin.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.lsd_profile_end.argtypes = [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_int), _dp]
     _lib = lib
     return lib
 
@@ -180,6 +186,16 @@ class Context:
 
     def sync(self):
         self._chk(self.lib.lsd_sync(self.h), "lsd_sync")
+
+    def profile_begin(self, kernel_class):
+        self._chk(self.lib.lsd_profile_begin(self.h, kernel_class), "lsd_profile_begin")
+
+    def profile_end(self):
+        """(total kernel ms, launches, algorithmic work) since profile_begin."""
+        ms, n, w = ctypes.c_double(), ctypes.c_int(), ctypes.c_double()
+        self._chk(self.lib.lsd_profile_end(self.h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(w)),
+                  "lsd_profile_end")
+        return ms.value, n.value, w.value
 
     def set_dt(self, dt):
         self._chk(self.lib.lsd_set_dt(self.h, dt), "lsd_set_dt")

@@ -1,11 +1,22 @@
 """Seeded synthetic initial states shared by the GPU path, the oracle and the
 benchmark, so every consumer sees identical bytes.
 
-Coefficients: Re, Im ~ N(0,1) * A * (l+1)^-2 for every (l, m, r_n), drawn
-with numpy PCG64(seed); Im = 0 for m = 0; l = 0 is zero for the toroidal and
-poloidal scalars of u and B, while codensity and composition keep it.
-Draw order: fields uT, uP, C, X, BT, BP, each as one (2, H, N) normal array
-(Re then Im) in the reference coll layout re[h, n].
+Coefficient of harmonic (l, m) at radius r_n:
+    Re, Im = z_{lm} * A * (l+1)^-2 * phi_f(x_n),   x = r - r_i in [0, 1]
+with z ~ N(0,1) drawn with numpy PCG64(seed) once per (l, m), Im = 0 for
+m = 0, and a smooth radial shape per field:
+    phi = sin(pi x)        toroidal u, toroidal B, codensity, composition
+    phi = sin(pi x)^2      poloidal u (P = dP/dr = 0 at both walls)
+    phi = sin(pi x)        poloidal B
+l = 0 is zero for u and B; the codensity and composition l = 0 modes carry
+the conductive profile r_i (r_o - r) / r (C = 1 at r_i, 0 at r_o, the
+reference's i_cod_bc = 1 values) plus the random part.
+
+Randomness is in (l, m) only: independent noise at every radial point of a
+Chebyshev grid (spacing ~ 1/N^2 at the walls) gives curls of order N^4 A and
+makes the fixed-dt predictor-corrector of tim_check_cgce diverge at the
+E = 1e-4, dt = 2e-6 case, which the reference would abort.
+Draw order: fields uT, uP, C, X, BT, BP, each one (2, H) normal array.
 """
 import numpy as np
 
@@ -33,26 +44,38 @@ def harmonics(L, M, Mp=1):
     return np.array(ls, dtype=np.int64), np.array(ms, dtype=np.int64)
 
 
+def radial_grid(N, rratio):
+    """Chebyshev collocation points (modules/meshs.F90:135-143)."""
+    n = np.arange(1, N + 1)
+    ri = rratio / (1.0 - rratio)
+    return 0.5 * (1.0 + np.cos(np.pi * (N - n) / (N - 1.0))) + ri, ri
+
+
 def seeded_state(cfg, seed, amp):
-    """dict field -> (re[H,N], im[H,N]) float64 arrays."""
+    """dict field -> (re[H,N], im[H,N]) float64 arrays (reference coll layout)."""
     H = cfg.sizes()[0]
-    N = cfg.N
     l, m = harmonics(cfg.L, cfg.M, cfg.Mp)
+    r, ri = radial_grid(cfg.N, cfg.rratio)
+    ro = ri + 1.0
+    x = r - ri
+    s1 = np.sin(np.pi * x)
+    shapes = {"uT": s1, "uP": s1 * s1, "C": s1, "X": s1, "BT": s1, "BP": s1}
     scale = amp * (l.astype(np.float64) + 1.0) ** -2
     rng = np.random.Generator(np.random.PCG64(seed))
     out = {}
     for name in FIELD_ORDER:
-        z = rng.standard_normal((2, H, N))
-        re = z[0] * scale[:, None]
-        im = z[1] * scale[:, None]
+        z = rng.standard_normal((2, H))
+        re = (z[0] * scale)[:, None] * shapes[name][None, :]
+        im = (z[1] * scale)[:, None] * shapes[name][None, :]
         im[m == 0, :] = 0.0
         if name in ("uT", "uP", "BT", "BP"):
             re[l == 0, :] = 0.0
             im[l == 0, :] = 0.0
-        if name == "X" and not cfg.comp:
-            re[:] = 0.0
-            im[:] = 0.0
-        if name in ("BT", "BP") and not cfg.mag:
+        if name in ("C", "X"):
+            re[(l == 0) & (m == 0), :] += ri * (ro - r) / r
+        off = (name == "X" and not cfg.comp) or (name in ("BT", "BP") and not cfg.mag) or \
+              (name == "C" and not cfg.cod)
+        if off:
             re[:] = 0.0
             im[:] = 0.0
         out[name] = (np.ascontiguousarray(re), np.ascontiguousarray(im))
