@@ -1,0 +1,151 @@
+"""GPU parity: the HIP library against the CPU oracle on the same seeded
+inputs, through the C-ABI (include/lsd.h).
+
+Tolerances: tables, mesh, transforms and single phases agree to a few ulps
+scaled by the contraction length.  A full step involves banded solves whose
+operators mix c1/dt with radLap entries of order 1/h^2 at the walls; at
+L=M=170, N=96, dt=2e-6 the l=0 codensity/composition solve moves by ~1e-11
+relative when the oracle itself is compiled with and without fused
+multiply-adds, so that mode gets the oracle's own rounding spread as its
+tolerance and every other harmonic 1e-12.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from lsd import config1, config3, config_small, seeded_state
+from lsd.gpu import Context
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b, mask=None):
+    dre, dim = a[0] - b[0], a[1] - b[1]
+    if mask is not None:
+        dre, dim, b = dre[mask], dim[mask], (b[0][mask], b[1][mask])
+    num = np.sqrt((dre ** 2 + dim ** 2).sum())
+    den = np.sqrt((b[0] ** 2 + b[1] ** 2).sum())
+    return num / den if den > 0 else num
+
+
+def rel_max(a, b):
+    s = np.abs(b).max()
+    return np.abs(a - b).max() / s if s > 0 else np.abs(a - b).max()
+
+
+CFGS = {"small": config_small, "config1": config1}
+
+
+@pytest.fixture(scope="module", params=["small", "config1"])
+def pair(request):
+    cfg = CFGS[request.param]()
+    print("case", request.param, cfg.sizes(), flush=True)
+    g = Context(cfg)
+    o = O.Oracle(cfg)
+    yield cfg, g, o
+    g.close()
+
+
+def test_tables_and_mesh(pair):
+    cfg, g, o = pair
+    for t in range(10):
+        assert rel_max(g.table(t), o.table(t)) < 4e-15, t
+    for w in range(6):
+        assert rel_max(g.mesh(w), o.mesh(w)) < 2e-14, w
+
+
+def _spec(cfg, seed, lmin=0):
+    from lsd.initial import harmonics
+    H = cfg.sizes()[0]
+    l, m = harmonics(cfg.L, cfg.M, cfg.Mp)
+    rng = np.random.default_rng(seed)
+    re = rng.standard_normal((cfg.N, H)) / (l + 1.0) ** 2
+    im = rng.standard_normal((cfg.N, H)) / (l + 1.0) ** 2
+    im[:, m == 0] = 0.0
+    re[:, l < lmin] = 0.0
+    im[:, l < lmin] = 0.0
+    return np.ascontiguousarray(re), np.ascontiguousarray(im)
+
+
+def test_dropin_transforms(pair):
+    cfg, g, o = pair
+    s = _spec(cfg, 1)
+    assert rel_max(g.spec2phys(*s), o.spec2phys(*s)) < 1e-13
+    p = o.spec2phys(*s)
+    for a, b in zip(g.phys2spec(p), o.phys2spec(p)):
+        assert rel_max(a, b) < 1e-13
+    for a, b in zip(g.grad(*s), o.grad(*s)):
+        assert rel_max(a, b) < 1e-13
+    q, sv, t = _spec(cfg, 2), _spec(cfg, 3, 1), _spec(cfg, 4, 1)
+    for a, b in zip(g.qst2rtp(q, sv, t), o.qst2rtp(q, sv, t)):
+        assert rel_max(a, b) < 1e-13
+    r, th, ph = o.qst2rtp(q, sv, t)
+    for a, b in zip(g.rtp2qst(r, th, ph), o.rtp2qst(r, th, ph)):
+        assert rel_max(a[0], b[0]) < 1e-13 and rel_max(a[1], b[1]) < 1e-12
+
+
+def test_phases(pair):
+    cfg, g, o = pair
+    st = seeded_state(cfg, 11, 1e-3)
+    g.set_state(st)
+    o.set_state(st)
+    g.transform()
+    o.transform()
+    for k in range(18):
+        assert rel_max(g.get_phys(k), o.get_phys(k)) < 1e-13, k
+    g.nonlinear()
+    o.nonlinear()
+    for k in range(6, 12):
+        assert rel_l2(g.get_coll(k), o.get_coll(k)) < 1e-13, k
+    g.predictor()
+    o.predictor()
+    for k in range(6):
+        assert rel_l2(g.get_coll(k), o.get_coll(k)) < 1e-12, k
+    g.transform(); g.nonlinear()
+    o.transform(); o.nonlinear()
+    dg, do = g.corrector(), o.corrector()
+    assert abs(dg - do) <= 1e-12 * do
+    for k in range(6):
+        assert rel_l2(g.get_coll(k), o.get_coll(k)) < 1e-12, k
+
+
+def test_steps_config1():
+    # 100 timesteps of the plain method at config 1, same corrector counts
+    cfg = config1()
+    st = seeded_state(cfg, 20261201, 1e-3)
+    g = Context(cfg)
+    o = O.Oracle(cfg)
+    g.set_state(st)
+    o.set_state(st)
+    worst = 0.0
+    for block in range(10):
+        sg = g.step(10)
+        so = o.step(10)
+        assert sg.evaluations == so.evaluations
+        for k in range(6):
+            e = rel_l2(g.get_coll(k), o.get_coll(k))
+            worst = max(worst, e)
+        print("after %d steps: worst rel L2 %.3e" % (10 * (block + 1), worst), flush=True)
+    g.close()
+    assert worst < 1e-10
+
+
+def test_one_step_config3():
+    cfg = config3()
+    st = seeded_state(cfg, 20261203, 1e-2)
+    g = Context(cfg)
+    g.set_state(st)
+    sg = g.step(1)
+    o = O.Oracle(cfg)
+    o.set_state(st)
+    so = o.step(1)
+    assert sg.evaluations == so.evaluations == 2
+    assert abs(sg.last_dterr - so.last_dterr) < 1e-10 * so.last_dterr
+    H = o.H
+    not_l0 = np.ones(H, dtype=bool)
+    not_l0[0] = False
+    for k in range(6):
+        a, b = g.get_coll(k), o.get_coll(k)
+        assert rel_l2(a, b, not_l0) < 1e-12, k
+        assert rel_l2(a, b) < 5e-11, k
+    g.close()

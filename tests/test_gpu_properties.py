@@ -1,0 +1,101 @@
+"""Size-independent properties at the full sizes, on the GPU:
+synthesis -> analysis round trips of the transforms at lmax = 255 on
+384 x 768 with 128 shells (L = M = 256 in the reference's l in [0, L)
+convention; M must be even), and boundary conditions after steps at
+L = M = 170, N = 96.
+
+The round-trip tolerance grows with l: the reference's GAUWTS stops its
+Newton iteration at |dz| <= 1e-13 (auxiliary/gauwts.f:38,63), so the
+quadrature weights carry ~1e-13 relative error that the analysis factor
+(2l+1)/2 multiplies; the CPU oracle shows the same 4.5e-11 round-trip error
+at l = 254, m = 0.
+"""
+import numpy as np
+import pytest
+
+from lsd import config3, make_config, seeded_state
+from lsd.gpu import Context
+from lsd.initial import harmonics, FIELD_ID
+
+pytestmark = pytest.mark.gpu
+
+
+def _spec(cfg, seed, lmin=0):
+    H = cfg.sizes()[0]
+    l, m = harmonics(cfg.L, cfg.M, cfg.Mp)
+    rng = np.random.default_rng(seed)
+    re = rng.standard_normal((cfg.N, H)) / (l + 1.0) ** 2
+    im = rng.standard_normal((cfg.N, H)) / (l + 1.0) ** 2
+    im[:, m == 0] = 0.0
+    re[:, l < lmin] = 0.0
+    im[:, l < lmin] = 0.0
+    return np.ascontiguousarray(re), np.ascontiguousarray(im)
+
+
+def _tol(cfg, b):
+    l, _ = harmonics(cfg.L, cfg.M, cfg.Mp)
+    return np.abs(b).max() * (1e-12 + 4e-13 * l[None, :])
+
+
+@pytest.fixture(scope="module")
+def big():
+    cfg = make_config(L=256, M=256, N=128, Th=384, Ph=768, comp=1)
+    print("case L=M=256 384x768 N=128", flush=True)
+    g = Context(cfg)
+    yield cfg, g
+    g.close()
+
+
+def test_scalar_round_trip_full_size(big):
+    cfg, g = big
+    s = _spec(cfg, 1)
+    back = g.phys2spec(g.spec2phys(*s))
+    for a, b in zip(back, s):
+        assert (np.abs(a - b) <= _tol(cfg, b)).all()
+
+
+def test_vector_round_trip_full_size(big):
+    cfg, g = big
+    q, sv, t = _spec(cfg, 2), _spec(cfg, 3, 1), _spec(cfg, 4, 1)
+    back = g.rtp2qst(*g.qst2rtp(q, sv, t))
+    for a, b in zip(back, (q, sv, t)):
+        assert (np.abs(a[0] - b[0]) <= _tol(cfg, b[0])).all()
+        assert (np.abs(a[1] - b[1]) <= _tol(cfg, b[1])).all()
+
+
+def test_boundary_conditions_after_steps_config3():
+    cfg = config3()
+    g = Context(cfg)
+    g.set_state(seeded_state(cfg, 20261203, 1e-2))
+    st = g.step(3)
+    assert st.steps == 3
+    dr1 = g.mesh(1)
+    r = g.mesh(0)
+    l, _ = harmonics(cfg.L, cfg.M, cfg.Mp)
+
+    def ddr(f, rows):
+        out = []
+        for n in rows:
+            acc = 0.0
+            for j in range(max(0, n - 3), min(cfg.N, n + 4)):
+                acc = acc + dr1[j, 3 + n - j] * f[:, j]
+            out.append(acc)
+        return out
+
+    for name in ("uT", "BT"):
+        for part in g.get_coll(FIELD_ID[name]):
+            assert np.abs(part[:, [0, -1]]).max() < 1e-12 * np.abs(part).max()
+    for part in g.get_coll(FIELD_ID["uP"]):
+        assert np.abs(part[:, [0, -1]]).max() < 1e-12 * np.abs(part).max()
+        d0, d1 = ddr(part, (0, cfg.N - 1))
+        assert max(np.abs(d0).max(), np.abs(d1).max()) < 1e-9 * np.abs(part).max() * cfg.N ** 2
+    for part in g.get_coll(FIELD_ID["BP"]):
+        d0, d1 = ddr(part, (0, cfg.N - 1))
+        inner = d0 - l / r[0] * part[:, 0]
+        outer = d1 + (l + 1) / r[-1] * part[:, -1]
+        scale = np.abs(d0).max() + np.abs(d1).max()
+        assert np.abs(inner[l > 0]).max() < 1e-9 * scale
+        assert np.abs(outer[l > 0]).max() < 1e-9 * scale
+    c = g.get_coll(FIELD_ID["C"])[0]
+    assert abs(c[0, 0] - 1.0) < 1e-12 and np.abs(c[1:, [0, -1]]).max() < 1e-12
+    g.close()
