@@ -1,299 +1,434 @@
-// Gauss-Legendre synthesis and analysis as per-m contractions.
+// Gauss-Legendre synthesis and analysis as per-m contractions on the matrix
+// cores (v_mfma_*_16x16x4: 16x4 by 4x16 per wave instruction).
 //
-// Synthesis (modules/transform.fftw3.F90:472-520, 719-818, 955-1000): for an
-// m-slot with K harmonics (l = m..L-1, equatorial parity s = (l+m) mod 2 =
-// k mod 2 for the k-th harmonic of the slot),
-//     Y_p[j, n] = sum_{k : parity p} T[h0+k, j] * c[h0+k, n]        j < Th2
+// Synthesis (modules/transform.fftw3.F90:472-520, 719-818, 955-1000).  For
+// an m-slot (harmonics h0..h0+K-1, l = m..L-1, equatorial parity of the
+// k-th harmonic = k mod 2):
+//     Y_p[c, j] = sum_{k: parity p} coef[h0+k, c] * T[h0+k, j]   j < Th2
 //     F[j] = Y0 + Y1,  F[Th-1-j] = Y0 - Y1
-// One wave per (slot, 64 theta points, S shells): lanes run over theta so
-// table reads are coalesced; the coefficients of the wave's shells are
-// wave-uniform and come through the scalar path.
+// with columns c = 2n + (re, im).  The MFMA A operand is the coefficient
+// block (16 columns x 4 harmonics, read straight from the coll layout), the B
+// operand the table block (4 harmonics x 16 theta points, tables stored
+// [h][j]); each lane ends with 4 consecutive columns (two complex values)
+// of one theta point, so the epilogue writes 16 consecutive theta points
+// per shell.  Parity-flipped tables (dtheta P) accumulate into the other
+// bucket; the i and -i rotations of the vector/gradient terms swap the
+// re/im column of a lane pair (one lane exchange).
 //
-// Analysis (modules/transform.fftw3.F90:527-602, 825-949): the folded
-// Fourier data G_p[j, n] (north+south, south-north, from the forward FFT)
-// are contracted with the quadrature-weighted tables,
-//     out[h0+k, n] = sum_j Tw[j, h0+k] * G_{parity}[j, n]
-// One wave per (slot, parity class, 64 harmonics, S shells): lanes run over
-// harmonics of one parity class, the analysis tables are stored with each
-// slot's even-k harmonics first, so reads are coalesced.
+// Analysis (modules/transform.fftw3.F90:527-602, 825-949).  The forward FFT
+// leaves folded data G_p[j, c] (north+south, south-north).  A workgroup
+// takes one slot and 32 columns, stages 16-theta chunks of G through LDS,
+// and each wave contracts 16-harmonic tiles of one parity class against
+// them: D[k, c] = sum_j Tw[j, k] G[j, c] with the quadrature-weighted tables
+// stored [j][h] and each slot's even-k harmonics before its odd-k ones, so
+// table reads and result writes are 128-byte runs.
+#include <type_traits>
+
 #include "lsd_internal.h"
 
 namespace lsd {
 
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+    typedef v4d acc;
+    static __device__ __forceinline__ acc run(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct Mfma<float> {
+    typedef v4f acc;
+    static __device__ __forceinline__ acc run(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// -------------------------------------------------------------- synthesis
-
-// SCALAR: one table (P), one input, one output, no parity flip.
-template <int S>
-__global__ void __launch_bounds__(64) k_synth_scalar(int N, int Th, int Th2, int M, const Slot* __restrict__ slots,
-                                                     const int* __restrict__ order, const double* __restrict__ tab,
-                                                     const double2* __restrict__ in, double2* __restrict__ out) {
-    const int sl = uniform(order[blockIdx.x]);
-    const Slot s = slots[sl];
-    const int j = blockIdx.y * 64 + threadIdx.x;
-    const int n0 = uniform(blockIdx.z * S);
-    const int jj = j < Th2 ? j : Th2 - 1;
-    double2 y0[S], y1[S];
-#pragma unroll
-    for (int i = 0; i < S; i++) y0[i] = y1[i] = make_double2(0.0, 0.0);
-    const double* T = tab + (size_t)s.h0 * Th2 + jj;
-    const double2* C = in + (size_t)s.h0 * N + n0;
-    int k = 0;
-    for (; k + 1 < s.len; k += 2) {
-        double t0 = T[(size_t)k * Th2];
-        double t1 = T[(size_t)(k + 1) * Th2];
-#pragma unroll
-        for (int i = 0; i < S; i++) {
-            if (n0 + i < N) {
-                double2 c0 = C[(size_t)k * N + i];
-                double2 c1 = C[(size_t)(k + 1) * N + i];
-                y0[i].x = fma(t0, c0.x, y0[i].x);
-                y0[i].y = fma(t0, c0.y, y0[i].y);
-                y1[i].x = fma(t1, c1.x, y1[i].x);
-                y1[i].y = fma(t1, c1.y, y1[i].y);
-            }
-        }
-    }
-    if (k < s.len) {
-        double t0 = T[(size_t)k * Th2];
-#pragma unroll
-        for (int i = 0; i < S; i++) {
-            if (n0 + i < N) {
-                double2 c0 = C[(size_t)k * N + i];
-                y0[i].x = fma(t0, c0.x, y0[i].x);
-                y0[i].y = fma(t0, c0.y, y0[i].y);
-            }
-        }
-    }
-    if (j >= Th2) return;
-#pragma unroll
-    for (int i = 0; i < S; i++) {
-        if (n0 + i >= N) break;
-        double2* F = out + (((size_t)(n0 + i) * M + s.mfft) * Th);
-        F[j] = make_double2(y0[i].x + y1[i].x, y0[i].y + y1[i].y);
-        if (Th - 1 - j >= Th2) F[Th - 1 - j] = make_double2(y0[i].x - y1[i].x, y0[i].y - y1[i].y);
-    }
+template <typename T>
+__device__ __forceinline__ T partner(T v) {
+    return __shfl_xor(v, 1);
 }
 
-// VECTOR: tables llP_ (parity flipped) and llsin1mP, inputs s and t,
-// outputs A_theta and A_phi:
-//   A_th = sum s * llP_ + (-i t) * llsin1mP,  A_ph = sum (i s) * llsin1mP + t * llP_
-// GRAD: tables P_ (flipped) and sin1mP, input c, outputs scaled by 1/r:
-//   A_th = (1/r) sum c * P_,  A_ph = (1/r) sum (i c) * sin1mP
-template <int S, bool GRAD>
-__global__ void __launch_bounds__(64) k_synth_pair(int N, int Th, int Th2, int M, const Slot* __restrict__ slots,
-                                                   const int* __restrict__ order, const double* __restrict__ tabA,
-                                                   const double* __restrict__ tabB, const double2* __restrict__ in_s,
-                                                   const double2* __restrict__ in_t, const double* __restrict__ rinv,
-                                                   double2* __restrict__ out_th, double2* __restrict__ out_ph) {
-    const int sl = uniform(order[blockIdx.x]);
-    const Slot s = slots[sl];
-    const int j = blockIdx.y * 64 + threadIdx.x;
-    const int n0 = uniform(blockIdx.z * S);
-    const int jj = j < Th2 ? j : Th2 - 1;
-    // accumulators [output][parity bucket][shell]
-    double2 a0[S], a1[S], b0[S], b1[S];
+template <typename T>
+struct SynArgs {
+    int N, Th, Th2, M, H;
+    const Slot* slots;
+    const int* order;
+    const T* tabA;   // [h][Th2]
+    const T* tabB;   // second table (vector / gradient)
+    const T* in0;    // coll [h][2N] (re, im interleaved)
+    const T* in1;
+    const double* rinv;
+    T* out0;         // fourier [n][M][Th] complex
+    T* out1;
+};
+
+// KIND 0: scalar (P);  1: vector (llP_ flipped, llsin1mP);  2: gradient
+// (P_ flipped, sin1mP, scaled by 1/r).
+template <typename T, int KIND>
+__global__ void __launch_bounds__(64) k_syn(SynArgs<T> a) {
+    typedef typename Mfma<T>::acc acc_t;
+    constexpr int NOUT = KIND == 0 ? 1 : 2;
+    constexpr int CB = KIND == 0 ? 2 : 1;
+    const int sl = uniform(a.order[blockIdx.x]);
+    const Slot s = a.slots[sl];
+    const int lane = threadIdx.x, l16 = lane & 15, q4 = lane >> 4;
+    const int j0 = blockIdx.y * 64;
+    const int c0 = blockIdx.z * 16 * CB;
+    const int C2 = 2 * a.N;
+    acc_t Y[NOUT][2][CB][4];
 #pragma unroll
-    for (int i = 0; i < S; i++) a0[i] = a1[i] = b0[i] = b1[i] = make_double2(0.0, 0.0);
-    const double* TA = tabA + (size_t)s.h0 * Th2 + jj;
-    const double* TB = tabB + (size_t)s.h0 * Th2 + jj;
-    const double2* CS = in_s + (size_t)s.h0 * N + n0;
-    const double2* CT = GRAD ? CS : in_t + (size_t)s.h0 * N + n0;
-    for (int k = 0; k < s.len; k++) {
-        const double ta = TA[(size_t)k * Th2];  // flipped parity table
-        const double tb = TB[(size_t)k * Th2];
-        const bool odd = (k & 1);
+    for (int o = 0; o < NOUT; o++)
 #pragma unroll
-        for (int i = 0; i < S; i++) {
-            if (n0 + i < N) {
-                double2 cs = CS[(size_t)k * N + i];
-                double2 ct = GRAD ? cs : CT[(size_t)k * N + i];
-                // flip-parity contributions go to bucket (1 - k%2), the others to k%2
-                if (GRAD) {
-                    if (!odd) {
-                        a1[i].x = fma(ta, cs.x, a1[i].x);
-                        a1[i].y = fma(ta, cs.y, a1[i].y);
-                        b0[i].x = fma(-tb, cs.y, b0[i].x);
-                        b0[i].y = fma(tb, cs.x, b0[i].y);
-                    } else {
-                        a0[i].x = fma(ta, cs.x, a0[i].x);
-                        a0[i].y = fma(ta, cs.y, a0[i].y);
-                        b1[i].x = fma(-tb, cs.y, b1[i].x);
-                        b1[i].y = fma(tb, cs.x, b1[i].y);
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+            for (int cb = 0; cb < CB; cb++)
+#pragma unroll
+                for (int jb = 0; jb < 4; jb++) Y[o][p][cb][jb] = acc_t{0, 0, 0, 0};
+    const bool odd_col = (l16 & 1);
+    for (int k0 = 0; k0 < s.len; k0 += 8) {
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            const int k = k0 + 2 * q4 + p;
+            const bool kv = k < s.len;
+            const size_t hrow = (size_t)s.h0 + (kv ? k : 0);
+            T ta[4], tb[4];
+#pragma unroll
+            for (int jb = 0; jb < 4; jb++) {
+                int j = j0 + 16 * jb + l16;
+                int jc = j < a.Th2 ? j : a.Th2 - 1;
+                ta[jb] = kv ? a.tabA[hrow * a.Th2 + jc] : T(0);
+                tb[jb] = (KIND != 0 && kv) ? a.tabB[hrow * a.Th2 + jc] : T(0);
+            }
+#pragma unroll
+            for (int cb = 0; cb < CB; cb++) {
+                const int c = c0 + 16 * cb + l16;
+                const bool cv = kv && c < C2;
+                const T x = cv ? a.in0[hrow * C2 + c] : T(0);
+                if (KIND == 0) {
+#pragma unroll
+                    for (int jb = 0; jb < 4; jb++) Y[0][p][cb][jb] = Mfma<T>::run(x, ta[jb], Y[0][p][cb][jb]);
+                } else if (KIND == 2) {
+                    // theta: c * dP (bucket 1-p);  phi: (i c) * mP/sin (bucket p)
+                    T xr = partner(x);
+                    xr = odd_col ? xr : -xr;
+#pragma unroll
+                    for (int jb = 0; jb < 4; jb++) {
+                        Y[0][1 - p][cb][jb] = Mfma<T>::run(x, ta[jb], Y[0][1 - p][cb][jb]);
+                        Y[NOUT - 1][p][cb][jb] = Mfma<T>::run(xr, tb[jb], Y[NOUT - 1][p][cb][jb]);
                     }
                 } else {
-                    if (!odd) {
-                        a1[i].x = fma(ta, cs.x, a1[i].x);
-                        a1[i].y = fma(ta, cs.y, a1[i].y);
-                        b1[i].x = fma(ta, ct.x, b1[i].x);
-                        b1[i].y = fma(ta, ct.y, b1[i].y);
-                        a0[i].x = fma(tb, ct.y, a0[i].x);
-                        a0[i].y = fma(-tb, ct.x, a0[i].y);
-                        b0[i].x = fma(-tb, cs.y, b0[i].x);
-                        b0[i].y = fma(tb, cs.x, b0[i].y);
-                    } else {
-                        a0[i].x = fma(ta, cs.x, a0[i].x);
-                        a0[i].y = fma(ta, cs.y, a0[i].y);
-                        b0[i].x = fma(ta, ct.x, b0[i].x);
-                        b0[i].y = fma(ta, ct.y, b0[i].y);
-                        a1[i].x = fma(tb, ct.y, a1[i].x);
-                        a1[i].y = fma(-tb, ct.x, a1[i].y);
-                        b1[i].x = fma(-tb, cs.y, b1[i].x);
-                        b1[i].y = fma(tb, cs.x, b1[i].y);
+                    const T t = cv ? a.in1[hrow * C2 + c] : T(0);
+                    // (-i t): re <- t_im, im <- -t_re ;  (i s): re <- -s_im, im <- s_re
+                    T tr = partner(t), sr = partner(x);
+                    tr = odd_col ? -tr : tr;
+                    sr = odd_col ? sr : -sr;
+#pragma unroll
+                    for (int jb = 0; jb < 4; jb++) {
+                        Y[0][1 - p][cb][jb] = Mfma<T>::run(x, ta[jb], Y[0][1 - p][cb][jb]);
+                        Y[0][p][cb][jb] = Mfma<T>::run(tr, tb[jb], Y[0][p][cb][jb]);
+                        Y[NOUT - 1][p][cb][jb] = Mfma<T>::run(sr, tb[jb], Y[NOUT - 1][p][cb][jb]);
+                        Y[NOUT - 1][1 - p][cb][jb] = Mfma<T>::run(t, ta[jb], Y[NOUT - 1][1 - p][cb][jb]);
                     }
                 }
             }
         }
     }
-    if (j >= Th2) return;
+    // Output layout of v_mfma_*_16x16x4: the lane holds column j = l16 of
+    // rows mrow(q4, r); rows are our columns c (re/im of a shell), columns
+    // are theta points j0 + 16jb + l16.
+    typedef typename std::conditional<sizeof(T) == 8, double2, float2>::type T2;
 #pragma unroll
-    for (int i = 0; i < S; i++) {
-        const int n = n0 + i;
-        if (n >= N) break;
-        const double f = GRAD ? rinv[n] : 1.0;
-        double2* Fa = out_th + (((size_t)n * M + s.mfft) * Th);
-        double2* Fb = out_ph + (((size_t)n * M + s.mfft) * Th);
-        Fa[j] = make_double2(f * (a0[i].x + a1[i].x), f * (a0[i].y + a1[i].y));
-        Fb[j] = make_double2(f * (b0[i].x + b1[i].x), f * (b0[i].y + b1[i].y));
-        if (Th - 1 - j >= Th2) {
-            Fa[Th - 1 - j] = make_double2(f * (a0[i].x - a1[i].x), f * (a0[i].y - a1[i].y));
-            Fb[Th - 1 - j] = make_double2(f * (b0[i].x - b1[i].x), f * (b0[i].y - b1[i].y));
+    for (int o = 0; o < NOUT; o++) {
+        T2* F = reinterpret_cast<T2*>(o == 0 ? a.out0 : a.out1);
+#pragma unroll
+        for (int cb = 0; cb < CB; cb++) {
+#pragma unroll
+            for (int jb = 0; jb < 4; jb++) {
+                const int j = j0 + 16 * jb + l16;
+                T re0[2], im0[2], re1[2], im1[2];
+                int shell[2];
+                if constexpr (sizeof(T) == 8) {
+                    // rows q4 + 4r: even q4 holds re, odd q4 the im of the same
+                    // shells; pair with lane ^ 16, each lane writes two shells
+                    const bool lo = !(q4 & 1);
+#pragma unroll
+                    for (int hp = 0; hp < 2; hp++) {
+                        const int r0 = hp, r1 = 2 + hp;
+                        // values this lane sends: the half its partner writes
+                        const T s0 = lo ? Y[o][0][cb][jb][r1] : Y[o][0][cb][jb][r0];
+                        const T s1 = lo ? Y[o][1][cb][jb][r1] : Y[o][1][cb][jb][r0];
+                        const T g0 = __shfl_xor(s0, 16);
+                        const T g1 = __shfl_xor(s1, 16);
+                        const int r = lo ? r0 : r1;
+                        re0[hp] = lo ? Y[o][0][cb][jb][r] : g0;
+                        im0[hp] = lo ? g0 : Y[o][0][cb][jb][r];
+                        re1[hp] = lo ? Y[o][1][cb][jb][r] : g1;
+                        im1[hp] = lo ? g1 : Y[o][1][cb][jb][r];
+                        shell[hp] = (q4 >> 1) + 2 * r;
+                    }
+                } else {
+                    // rows 4q4 + r: the lane holds two whole complex values
+#pragma unroll
+                    for (int hp = 0; hp < 2; hp++) {
+                        re0[hp] = Y[o][0][cb][jb][2 * hp];
+                        im0[hp] = Y[o][0][cb][jb][2 * hp + 1];
+                        re1[hp] = Y[o][1][cb][jb][2 * hp];
+                        im1[hp] = Y[o][1][cb][jb][2 * hp + 1];
+                        shell[hp] = 2 * q4 + hp;
+                    }
+                }
+                if (j >= a.Th2) continue;
+#pragma unroll
+                for (int hp = 0; hp < 2; hp++) {
+                    const int n = ((c0 + 16 * cb) >> 1) + shell[hp];
+                    if (n >= a.N) continue;
+                    const T f = KIND == 2 ? (T)a.rinv[n] : T(1);
+                    T2* row = F + ((size_t)n * a.M + s.mfft) * a.Th;
+                    T2 v;
+                    v.x = f * (re0[hp] + re1[hp]);
+                    v.y = f * (im0[hp] + im1[hp]);
+                    row[j] = v;
+                    if (a.Th - 1 - j >= a.Th2) {
+                        v.x = f * (re0[hp] - re1[hp]);
+                        v.y = f * (im0[hp] - im1[hp]);
+                        row[a.Th - 1 - j] = v;
+                    }
+                }
+            }
         }
     }
 }
 
-constexpr int SYN_S = 8;
+template <typename T>
+static int syn_launch(Ctx& c, int kind, const T* tA, const T* tB, const T* in0, const T* in1, T* out0, T* out1) {
+    SynArgs<T> a;
+    a.N = c.N;
+    a.Th = c.Th;
+    a.Th2 = c.Th2;
+    a.M = c.M;
+    a.H = c.H;
+    a.slots = c.d_slots;
+    a.order = c.d_slot_order;
+    a.tabA = tA;
+    a.tabB = tB;
+    a.in0 = in0;
+    a.in1 = in1;
+    a.rinv = c.d_r + 2 * c.N;
+    a.out0 = out0;
+    a.out1 = out1;
+    const int cb = kind == SYN_SCALAR ? 2 : 1;
+    dim3 grid(c.nslot, (c.Th2 + 63) / 64, (2 * c.N + 16 * cb - 1) / (16 * cb));
+    if (kind == SYN_SCALAR) hipLaunchKernelGGL((k_syn<T, 0>), grid, dim3(64), 0, c.stream, a);
+    else if (kind == SYN_VECTOR) hipLaunchKernelGGL((k_syn<T, 1>), grid, dim3(64), 0, c.stream, a);
+    else hipLaunchKernelGGL((k_syn<T, 2>), grid, dim3(64), 0, c.stream, a);
+    LSD_HIP(hipGetLastError());
+    return LSD_OK;
+}
 
 // kind SYN_SCALAR: in0 -> out0 (table P)
 // kind SYN_VECTOR: (in0 = s, in1 = t) -> (out0 = theta, out1 = phi)
 // kind SYN_GRAD:   in0 -> (out0 = theta, out1 = phi)
 int launch_synthesis(Ctx& c, int kind, const double2* in0, const double2* in1, double2* out0, double2* out1) {
     size_t HT = (size_t)c.H * c.Th2;
-    dim3 grid(c.nslot, (c.Th2 + 63) / 64, (c.N + SYN_S - 1) / SYN_S);
-    // 4 flops per (harmonic, theta point, shell) and product unit
-    const double unit = 4.0 * (double)c.H * c.Th2 * c.N;
+    const double unit = 4.0 * (double)c.H * c.Th2 * c.N;  // flops per product unit
     ProfScope ps(c, LSD_K_SYNTH, unit * (kind == SYN_SCALAR ? 1 : kind == SYN_VECTOR ? 4 : 2));
-    const double* rinv = c.d_r + 2 * c.N;
-    if (kind == SYN_SCALAR) {
-        hipLaunchKernelGGL((k_synth_scalar<SYN_S>), grid, dim3(64), 0, c.stream, c.N, c.Th, c.Th2, c.M,
-                           c.d_slots, c.d_slot_order, c.d_tab + 0 * HT, in0, out0);
-    } else if (kind == SYN_VECTOR) {
-        hipLaunchKernelGGL((k_synth_pair<SYN_S, false>), grid, dim3(64), 0, c.stream, c.N, c.Th, c.Th2, c.M,
-                           c.d_slots, c.d_slot_order, c.d_tab + 3 * HT, c.d_tab + 4 * HT, in0, in1, rinv,
-                           out0, out1);
-    } else {
-        hipLaunchKernelGGL((k_synth_pair<SYN_S, true>), grid, dim3(64), 0, c.stream, c.N, c.Th, c.Th2, c.M,
-                           c.d_slots, c.d_slot_order, c.d_tab + 1 * HT, c.d_tab + 2 * HT, in0, in0, rinv,
-                           out0, out1);
-    }
-    LSD_HIP(hipGetLastError());
-    return LSD_OK;
+    const double* t0 = c.d_tab + (kind == SYN_SCALAR ? 0 : kind == SYN_VECTOR ? 3 : 1) * HT;
+    const double* t1 = c.d_tab + (kind == SYN_VECTOR ? 4 : 2) * HT;
+    return syn_launch<double>(c, kind, t0, t1, reinterpret_cast<const double*>(in0),
+                              reinterpret_cast<const double*>(in1), reinterpret_cast<double*>(out0),
+                              reinterpret_cast<double*>(out1));
 }
 
 // --------------------------------------------------------------- analysis
 
-// Position of the k-th harmonic of a slot in the analysis tables: even k
-// first, then odd k.
-__host__ __device__ inline int ana_pos(int h0, int len, int k) {
-    return h0 + ((k & 1) ? (len + 1) / 2 + (k >> 1) : (k >> 1));
-}
+template <typename T>
+struct AnaArgs {
+    int N, Th2, M, H;
+    const Slot* slots;
+    const int* order;
+    const T* tP;   // gwP        [Th2][H] (parity-grouped h)
+    const T* tL;   // gwllP_
+    const T* tS;   // gwllsin1mP
+    const T* g1;   // folded [n][M][2][Th2] complex
+    const T* g2;
+    const T* g3;
+    T* o1;         // coll [h][2N]
+    T* o2;
+    T* o3;
+};
 
-// Folded layout: G[n][m][p][j], p = 0 north+south, p = 1 south-north.
-template <int S, bool VEC>
-__global__ void __launch_bounds__(64) k_anal(int N, int Th2, int M, int H, const Slot* __restrict__ slots,
-                                             const int* __restrict__ order, const double* __restrict__ atab,
-                                             const double2* __restrict__ g1, const double2* __restrict__ g2,
-                                             const double2* __restrict__ g3, double2* __restrict__ o1,
-                                             double2* __restrict__ o2, double2* __restrict__ o3) {
-    const int sl = uniform(order[blockIdx.x]);
-    const Slot s = slots[sl];
-    const int par = uniform(blockIdx.y & 1);  // parity class = k mod 2
-    const int lane_k = (blockIdx.y >> 1) * 64 + threadIdx.x;
-    const int k = 2 * lane_k + par;
-    const int n0 = uniform(blockIdx.z * S);
-    const int ncls = par ? s.len / 2 : (s.len + 1) / 2;
-    if (uniform((blockIdx.y >> 1) * 64) >= ncls) return;
-    const bool active = lane_k < ncls;
-    const int ap = active ? ana_pos(s.h0, s.len, k) : s.h0;
-    size_t HT = (size_t)H * Th2;
-    const double* TP = atab + 0 * HT + ap;   // gwP
-    const double* TL = atab + 1 * HT + ap;   // gwllP_
-    const double* TS = atab + 2 * HT + ap;   // gwllsin1mP
-    // parity buckets: own = par, other = 1 - par
-    double2 q[S], sv[S], tv[S];
+constexpr int ANA_JC = 16;     // theta points per LDS chunk
+constexpr int ANA_COLS = 32;   // columns per workgroup (16 shells)
+constexpr int ANA_TPW = 4;     // max harmonic tiles per wave
+
+// VEC: q, s, t from three folded fields (tra_rtp2qst); else one field
+// (tra_phys2spec).  Workgroup: 4 waves, one slot, 32 columns.
+template <typename T, bool VEC>
+__global__ void __launch_bounds__(256) k_ana(AnaArgs<T> a) {
+    typedef typename Mfma<T>::acc acc_t;
+    constexpr int NF = VEC ? 3 : 1;
+    constexpr int NO = VEC ? 3 : 1;
+    // LDS: [field][parity][theta][column]
+    __shared__ T G[NF][2][ANA_JC][ANA_COLS + 1];
+    const int sl = uniform(a.order[blockIdx.x]);
+    const Slot s = a.slots[sl];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, q4 = lane >> 4;
+    const int c0 = blockIdx.y * ANA_COLS;
+    const int C2 = 2 * a.N;
+    const int n_even = (s.len + 1) / 2, n_odd = s.len / 2;
+    const int t_even = (n_even + 15) / 16, t_odd = (n_odd + 15) / 16;
+    const int ntiles = t_even + t_odd;
+    acc_t acc[ANA_TPW][NO][2];
 #pragma unroll
-    for (int i = 0; i < S; i++) q[i] = sv[i] = tv[i] = make_double2(0.0, 0.0);
-    const size_t gstride = 2 * (size_t)Th2;  // per m
-    for (int j = 0; j < Th2; j++) {
-        const double tp = TP[(size_t)j * H];
-        double tl = 0.0, ts = 0.0;
-        if (VEC) {
-            tl = TL[(size_t)j * H];
-            ts = TS[(size_t)j * H];
+    for (int t = 0; t < ANA_TPW; t++)
+#pragma unroll
+        for (int o = 0; o < NO; o++) acc[t][o][0] = acc[t][o][1] = acc_t{0, 0, 0, 0};
+    const size_t gstride = (size_t)2 * a.Th2;  // complex values per (n, m)
+    for (int jc0 = 0; jc0 < a.Th2; jc0 += ANA_JC) {
+        __syncthreads();
+        {   // stage: thread -> (shell in tile 0..15, theta in chunk 0..15)
+            const int sh = tid >> 4, jj = tid & 15;
+            const int n = (c0 >> 1) + sh;
+            const int j = jc0 + jj;
+            const bool ok = n < a.N && j < a.Th2;
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+                const T* g = f == 0 ? a.g1 : (f == 1 ? a.g2 : a.g3);
+#pragma unroll
+                for (int p = 0; p < 2; p++) {
+                    T re = T(0), im = T(0);
+                    if (ok) {
+                        size_t idx = (((size_t)n * a.M + s.mfft) * gstride + (size_t)p * a.Th2 + j) * 2;
+                        re = g[idx];
+                        im = g[idx + 1];
+                    }
+                    G[f][p][jj][2 * sh] = re;
+                    G[f][p][jj][2 * sh + 1] = im;
+                }
+            }
         }
+        __syncthreads();
 #pragma unroll
-        for (int i = 0; i < S; i++) {
-            const int n = n0 + i;
-            if (n < N) {
-                const size_t base = ((size_t)n * M + s.mfft) * gstride + j;
-                const double2 y1 = g1[base + (size_t)par * Th2];
-                q[i].x = fma(y1.x, tp, q[i].x);
-                q[i].y = fma(y1.y, tp, q[i].y);
-                if (VEC) {
-                    const double2 y2o = g2[base + (size_t)(1 - par) * Th2];  // Y2(:,m,1-s)
-                    const double2 y2s = g2[base + (size_t)par * Th2];        // Y2(:,m,s)
-                    const double2 y3s = g3[base + (size_t)par * Th2];        // Y3(:,m,s)
-                    const double2 y3o = g3[base + (size_t)(1 - par) * Th2];  // Y3(:,m,1-s)
-                    sv[i].x = fma(y2o.x, tl, sv[i].x);
-                    sv[i].x = fma(y3s.y, ts, sv[i].x);
-                    sv[i].y = fma(y2o.y, tl, sv[i].y);
-                    sv[i].y = fma(-y3s.x, ts, sv[i].y);
-                    tv[i].x = fma(-y2s.y, ts, tv[i].x);
-                    tv[i].x = fma(y3o.x, tl, tv[i].x);
-                    tv[i].y = fma(y2s.x, ts, tv[i].y);
-                    tv[i].y = fma(y3o.y, tl, tv[i].y);
+        for (int js = 0; js < ANA_JC / 4; js++) {
+            const int jl = 4 * js + q4;  // chunk-local theta of this lane's K index
+            const int j = jc0 + jl;
+            const bool jv = j < a.Th2;
+            const size_t trow = (size_t)(jv ? j : a.Th2 - 1) * a.H;
+#pragma unroll
+            for (int t = 0; t < ANA_TPW; t++) {
+                const int tile = wave + 4 * t;
+                if (tile >= ntiles) break;
+                const int par = tile < t_even ? 0 : 1;
+                const int kt = par ? tile - t_even : tile;  // tile within the class
+                const int ncls = par ? n_odd : n_even;
+                const int kc = kt * 16 + l16;               // class-local row of this lane
+                const int ap = (kc < ncls) ? s.h0 + (par ? n_even : 0) + kc : s.h0;
+                const T wP = jv ? a.tP[trow + ap] : T(0);
+                const T wL = (VEC && jv) ? a.tL[trow + ap] : T(0);
+                const T wS = (VEC && jv) ? a.tS[trow + ap] : T(0);
+#pragma unroll
+                for (int cb = 0; cb < 2; cb++) {
+                    const int col = 16 * cb + l16;
+                    const bool evn = !(l16 & 1);
+                    acc[t][0][cb] = Mfma<T>::run(wP, G[0][par][jl][col], acc[t][0][cb]);
+                    if constexpr (VEC) {
+                        const T y2o = G[NF - 2][1 - par][jl][col];
+                        const T y3s_p = G[NF - 1][par][jl][col ^ 1];
+                        const T y2s_p = G[NF - 2][par][jl][col ^ 1];
+                        const T y3o = G[NF - 1][1 - par][jl][col];
+                        // s.re += Y2o.re gl + Y3s.im gs ; s.im += Y2o.im gl - Y3s.re gs
+                        acc[t][NO - 2][cb] = Mfma<T>::run(wL, y2o, acc[t][NO - 2][cb]);
+                        acc[t][NO - 2][cb] = Mfma<T>::run(wS, evn ? y3s_p : -y3s_p, acc[t][NO - 2][cb]);
+                        // t.re = -Y2s.im gs + Y3o.re gl ; t.im = Y2s.re gs + Y3o.im gl
+                        acc[t][NO - 1][cb] = Mfma<T>::run(wS, evn ? -y2s_p : y2s_p, acc[t][NO - 1][cb]);
+                        acc[t][NO - 1][cb] = Mfma<T>::run(wL, y3o, acc[t][NO - 1][cb]);
+                    }
                 }
             }
         }
     }
-    if (!active) return;
-    const int h = s.h0 + k;
-    const bool m0 = (s.m == 0);
+    // the lane holds class-local rows 16*kt + mrow of columns c0 + 16cb + l16
+    // (mrow = q4 + 4r for the FP64 instruction, 4q4 + r for FP32)
 #pragma unroll
-    for (int i = 0; i < S; i++) {
-        const int n = n0 + i;
-        if (n >= N) break;
-        size_t o = (size_t)h * N + n;
-        if (VEC) {
-            o1[o] = q[i];  // the reference leaves q%Im(m=0) as summed
-            o2[o] = make_double2(sv[i].x, m0 ? 0.0 : sv[i].y);
-            o3[o] = make_double2(tv[i].x, m0 ? 0.0 : tv[i].y);
-        } else {
-            o1[o] = make_double2(q[i].x, m0 ? 0.0 : q[i].y);
+    for (int t = 0; t < ANA_TPW; t++) {
+        const int tile = wave + 4 * t;
+        if (tile >= ntiles) break;
+        const int par = tile < t_even ? 0 : 1;
+        const int kt = par ? tile - t_even : tile;
+        const int ncls = par ? n_odd : n_even;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int kc = kt * 16 + (sizeof(T) == 8 ? q4 + 4 * r : 4 * q4 + r);
+            if (kc >= ncls) continue;
+            const size_t h = (size_t)s.h0 + 2 * kc + par;
+#pragma unroll
+            for (int cb = 0; cb < 2; cb++) {
+                const int c = c0 + 16 * cb + l16;
+                if (c >= C2) continue;
+                const bool zero_im = (s.m == 0) && (c & 1);
+                const size_t o = h * C2 + c;
+                if constexpr (VEC) {
+                    a.o1[o] = acc[t][0][cb][r];  // the reference leaves q%Im(m=0) as summed
+                    a.o2[o] = zero_im ? T(0) : acc[t][1][cb][r];
+                    a.o3[o] = zero_im ? T(0) : acc[t][2][cb][r];
+                } else {
+                    a.o1[o] = zero_im ? T(0) : acc[t][0][cb][r];
+                }
+            }
         }
     }
 }
 
-constexpr int ANA_S = 8;
+template <typename T>
+static int ana_launch(Ctx& c, int kind, const T* tP, const T* tL, const T* tS, const T* g0, const T* g1,
+                      const T* g2, T* o0, T* o1, T* o2) {
+    int maxlen = 0;
+    for (auto& s : c.slots) maxlen = maxlen > s.len ? maxlen : s.len;
+    int tiles = ((maxlen + 1) / 2 + 15) / 16 + (maxlen / 2 + 15) / 16;
+    if (tiles > 4 * ANA_TPW) {
+        set_error("analysis: L too large for the tile plan");
+        return LSD_E_CONFIG;
+    }
+    AnaArgs<T> a;
+    a.N = c.N;
+    a.Th2 = c.Th2;
+    a.M = c.M;
+    a.H = c.H;
+    a.slots = c.d_slots;
+    a.order = c.d_slot_order;
+    a.tP = tP;
+    a.tL = tL;
+    a.tS = tS;
+    a.g1 = g0;
+    a.g2 = g1;
+    a.g3 = g2;
+    a.o1 = o0;
+    a.o2 = o1;
+    a.o3 = o2;
+    dim3 grid(c.nslot, (2 * c.N + ANA_COLS - 1) / ANA_COLS);
+    if (kind == ANA_SCALAR) hipLaunchKernelGGL((k_ana<T, false>), grid, dim3(256), 0, c.stream, a);
+    else hipLaunchKernelGGL((k_ana<T, true>), grid, dim3(256), 0, c.stream, a);
+    LSD_HIP(hipGetLastError());
+    return LSD_OK;
+}
 
 int launch_analysis(Ctx& c, int kind, const double2* g0, const double2* g1, const double2* g2, double2* o0,
                     double2* o1, double2* o2) {
-    int maxlen = 0;
-    for (auto& s : c.slots) maxlen = maxlen > s.len ? maxlen : s.len;
-    int ktiles = ((maxlen + 1) / 2 + 63) / 64;
-    dim3 grid(c.nslot, 2 * ktiles, (c.N + ANA_S - 1) / ANA_S);
+    size_t HT = (size_t)c.H * c.Th2;
     const double unit = 4.0 * (double)c.H * c.Th2 * c.N;
     ProfScope ps(c, LSD_K_ANAL, unit * (kind == ANA_SCALAR ? 1 : 5));
-    if (kind == ANA_SCALAR) {
-        hipLaunchKernelGGL((k_anal<ANA_S, false>), grid, dim3(64), 0, c.stream, c.N, c.Th2, c.M, c.H, c.d_slots,
-                           c.d_slot_order, c.d_atab, g0, g0, g0, o0, o0, o0);
-    } else {
-        hipLaunchKernelGGL((k_anal<ANA_S, true>), grid, dim3(64), 0, c.stream, c.N, c.Th2, c.M, c.H, c.d_slots,
-                           c.d_slot_order, c.d_atab, g0, g1, g2, o0, o1, o2);
-    }
-    LSD_HIP(hipGetLastError());
-    return LSD_OK;
+    return ana_launch<double>(c, kind, c.d_atab, c.d_atab + HT, c.d_atab + 2 * HT,
+                              reinterpret_cast<const double*>(g0), reinterpret_cast<const double*>(g1),
+                              reinterpret_cast<const double*>(g2), reinterpret_cast<double*>(o0),
+                              reinterpret_cast<double*>(o1), reinterpret_cast<double*>(o2));
 }
 
 }  // namespace lsd

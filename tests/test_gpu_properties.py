@@ -9,6 +9,10 @@ Newton iteration at |dz| <= 1e-13 (auxiliary/gauwts.f:38,63), so the
 quadrature weights carry ~1e-13 relative error that the analysis factor
 (2l+1)/2 multiplies; the CPU oracle shows the same 4.5e-11 round-trip error
 at l = 254, m = 0.
+
+Boundary rows come out of pivoted banded solves whose operators reach
+c Pm radLap ~ 1e7 at N = 96, so they hold to ~1e-9 of the field, not to the
+ulp.
 """
 import numpy as np
 import pytest
@@ -84,9 +88,9 @@ def test_boundary_conditions_after_steps_config3():
 
     for name in ("uT", "BT"):
         for part in g.get_coll(FIELD_ID[name]):
-            assert np.abs(part[:, [0, -1]]).max() < 1e-12 * np.abs(part).max()
+            assert np.abs(part[:, [0, -1]]).max() < 1e-9 * np.abs(part).max()
     for part in g.get_coll(FIELD_ID["uP"]):
-        assert np.abs(part[:, [0, -1]]).max() < 1e-12 * np.abs(part).max()
+        assert np.abs(part[:, [0, -1]]).max() < 1e-9 * np.abs(part).max()
         d0, d1 = ddr(part, (0, cfg.N - 1))
         assert max(np.abs(d0).max(), np.abs(d1).max()) < 1e-9 * np.abs(part).max() * cfg.N ** 2
     for part in g.get_coll(FIELD_ID["BP"]):
@@ -97,5 +101,5 @@ def test_boundary_conditions_after_steps_config3():
         assert np.abs(inner[l > 0]).max() < 1e-9 * scale
         assert np.abs(outer[l > 0]).max() < 1e-9 * scale
     c = g.get_coll(FIELD_ID["C"])[0]
-    assert abs(c[0, 0] - 1.0) < 1e-12 and np.abs(c[1:, [0, -1]]).max() < 1e-12
+    assert abs(c[0, 0] - 1.0) < 1e-9 and np.abs(c[1:, [0, -1]]).max() < 1e-9
     g.close()
