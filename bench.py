@@ -6,9 +6,14 @@ tim_check_cgce accepts) of a Boussinesq MHD shell with codensity and
 composition at L = M = 170, N = 96, Th x Ph = 256 x 512, E = 1e-4, Pm = 2,
 dt = 2e-6, from a seeded synthetic state resident in HBM.
 
-Prints one JSON line.  ``roofline`` reports the dominant kernel class,
-timed live with HIP events on the context's stream over the timed region;
-``cpu_baseline`` times the CPU oracle on the host for one timestep.
+``value`` is the mixed-precision form (FP32 Legendre on MFMA, FP32 FFTs and
+products, FP32 solves with an FP64-residual refinement sweep, FP64 state);
+the plain FP64 form runs the same steps from the same state for the ratio,
+and the per-field relative error of the mixed form against it after all
+steps is reported.  ``roofline`` reports the mixed form's dominant kernel
+class, timed live with HIP events on the context's stream over the timed
+region; ``cpu_baseline`` times the CPU oracle on the host for one timestep.
+Prints one JSON line.
 """
 import argparse
 import json
@@ -20,7 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "leeds-code_amd"))
 
 PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
-PEAK_FP32_TFLOPS = 157.3
+PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 matrix
 PEAK_HBM_GBS = 8000.0
 
 
@@ -55,13 +60,52 @@ def load_traffic(kernel_class):
         return None
 
 
+def per_class(ctx, names):
+    """Kernel time of each class over one step each."""
+    out = {}
+    for k, name in enumerate(names):
+        ctx.profile_begin(k)
+        ctx.step(1)
+        ms, n, w = ctx.profile_end()
+        out[name] = {"ms": ms, "launches": n, "work": w}
+    return out
+
+
+def timed(ctx, steps, cls, sync_dev):
+    ctx.sync()
+    if sync_dev:
+        sync_dev()
+    if cls is not None:
+        ctx.profile_begin(cls)
+    t0 = time.perf_counter()
+    st = ctx.step(steps)
+    ctx.sync()
+    if sync_dev:
+        sync_dev()
+    t1 = time.perf_counter()
+    prof = ctx.profile_end() if cls is not None else None
+    return t1 - t0, st, prof
+
+
+def field_errors(a, b, cfg):
+    import numpy as np
+    from lsd.initial import FIELD_ID
+    out = {}
+    for name in ("uT", "uP", "C", "X", "BT", "BP"):
+        x, y = a.get_coll(FIELD_ID[name]), b.get_coll(FIELD_ID[name])
+        num = np.sqrt(sum(((p - q) ** 2).sum() for p, q in zip(x, y)))
+        den = np.sqrt(sum((q ** 2).sum() for q in y))
+        out[name] = float(num / den) if den > 0 else float(num)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", default="plain", choices=["plain"])
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison run")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -69,26 +113,8 @@ def main():
               "(multi-GPU sharding is design-only); rerun with --gpus 1", file=sys.stderr)
         return 2
 
-    from lsd import config3, seeded_state, SEEDS, AMPS, PLAIN_F64
+    from lsd import config3, seeded_state, SEEDS, AMPS, PLAIN_F64, MIXED_F32
     from lsd.gpu import Context, K_NAMES, K_SYNTH, K_ANAL
-
-    cfg = config3(PLAIN_F64)
-    H, Th, Ph, Th2 = cfg.sizes()
-    state = seeded_state(cfg, SEEDS[3], AMPS[3])
-    ctx = Context(cfg)
-    ctx.set_state(state)
-
-    ctx.step(args.warmup)
-    ctx.sync()
-    # per-class kernel time over one step each, to pick the dominant class
-    per_class = {}
-    for k, name in enumerate(K_NAMES):
-        ctx.profile_begin(k)
-        ctx.step(1)
-        ms, n, w = ctx.profile_end()
-        per_class[name] = {"ms": ms, "launches": n, "work": w}
-    dom = max(range(len(K_NAMES)), key=lambda k: per_class[K_NAMES[k]]["ms"])
-    log("per-class kernel ms per step:", {k: round(v["ms"], 4) for k, v in per_class.items()})
 
     try:
         import torch
@@ -96,27 +122,42 @@ def main():
     except Exception:
         sync_dev = None
 
-    ctx.sync()
-    if sync_dev:
-        sync_dev()
-    ctx.profile_begin(dom)
-    t0 = time.perf_counter()
-    st = ctx.step(args.steps)
-    ctx.sync()
-    if sync_dev:
-        sync_dev()
-    t1 = time.perf_counter()
-    kms, nl, work = ctx.profile_end()
-    elapsed = t1 - t0
+    cfg = config3(MIXED_F32)
+    H, Th, Ph, Th2 = cfg.sizes()
+    state = seeded_state(cfg, SEEDS[3], AMPS[3])
+
+    mixed = Context(cfg)
+    mixed.set_state(state)
+    mixed.step(args.warmup)
+    pc = per_class(mixed, K_NAMES)
+    dom = max(range(len(K_NAMES)), key=lambda k: pc[K_NAMES[k]]["ms"])
+    log("mixed per-class kernel ms per step:", {k: round(v["ms"], 4) for k, v in pc.items()})
+    elapsed, st, (kms, nl, work) = timed(mixed, args.steps, dom, sync_dev)
     value = args.steps / elapsed
+
+    plain_out, errs = None, None
+    if not args.no_plain:
+        plain = Context(config3(PLAIN_F64))
+        plain.set_state(state)
+        plain.step(args.warmup)
+        ppc = per_class(plain, K_NAMES)
+        log("plain per-class kernel ms per step:", {k: round(v["ms"], 4) for k, v in ppc.items()})
+        pel, pst, _ = timed(plain, args.steps, None, sync_dev)
+        errs = field_errors(mixed, plain, cfg)
+        log("mixed vs plain FP64 after %d steps:" % (args.warmup + len(K_NAMES) + args.steps),
+            {k: "%.2e" % v for k, v in errs.items()})
+        plain_out = {"value": args.steps / pel, "ms_per_step": 1e3 * pel / args.steps,
+                     "evaluations_per_step": pst.evaluations / max(pst.steps, 1),
+                     "kernel_ms_per_step": {k: v["ms"] for k, v in ppc.items()}}
+        plain.close()
 
     compute_bound = dom in (K_SYNTH, K_ANAL)
     avg_s = (kms / 1e3) / max(nl, 1)
     per_launch = work / max(nl, 1)
     if compute_bound:
         achieved = per_launch / avg_s / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP64_TFLOPS}
+        roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP32_TFLOPS}
     else:
         achieved = per_launch / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
@@ -129,7 +170,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(cfg, state, threads)
+        cpu = cpu_baseline(config3(PLAIN_F64), state, threads)
 
     out = {
         "metric": "timesteps/s at L=170,N=96: mixed-precision vs plain FP64 GPU, <=1e-6 rel err",
@@ -142,18 +183,21 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 transforms and solves, f64 state and refinement",
         "data": "synthetic: seeded (l+1)^-2 spectrum, smooth radial shapes, resident in HBM",
-        "config": {"workload": "Boussinesq MHD IMEX timestep with codensity and composition, plain FP64",
+        "config": {"workload": "Boussinesq MHD IMEX timestep with codensity and composition, mixed FP32",
                    "L": cfg.L, "M": cfg.M, "N": cfg.N, "Th": Th, "Ph": Ph, "H": H, "E": cfg.E,
                    "Pm": cfg.Pm, "dt": cfg.dt, "evaluations_per_step": st.evaluations / max(st.steps, 1),
                    "parallelism": "single GPU"},
+        "plain_f64": plain_out,
+        "speedup_vs_plain_f64": (value / plain_out["value"]) if plain_out else None,
+        "rel_err_vs_plain_f64": errs,
         "roofline": roof,
         "cpu_baseline": cpu,
-        "kernel_ms_per_step": {k: v["ms"] for k, v in per_class.items()},
+        "kernel_ms_per_step": {k: v["ms"] for k, v in pc.items()},
     }
     print(json.dumps(out), flush=True)
-    ctx.close()
+    mixed.close()
     return 0
 
 

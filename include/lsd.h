@@ -175,6 +175,11 @@ int lsd_corrector(lsd_ctx* ctx, double* dterr);
  * i_time_step_control = 2) accepts. */
 int lsd_step(lsd_ctx* ctx, int nsteps, lsd_step_stats* stats);
 
+/* Whether lsd_step replays the timestep from captured HIP graphs (default 1)
+ * or launches every kernel from the host (0).  No counterpart in the
+ * reference; results are identical either way. */
+int lsd_set_graph(lsd_ctx* ctx, int on);
+
 /* Refactorise all banded operators for a new dt (vel_matrices, cod_matrices,
  * comp_matrices, mag_matrices; program/main.F90:126-135). */
 int lsd_set_dt(lsd_ctx* ctx, double dt);
@@ -205,7 +210,8 @@ int lsd_sync(lsd_ctx* ctx);
 #define LSD_K_NONLIN   4   /* pointwise nonlinear products                      */
 #define LSD_K_SPEC     5   /* radial stencils on spectral fields                */
 #define LSD_K_SOLVE    6   /* banded predictor / corrector solves               */
-#define LSD_K_COUNT    7
+#define LSD_K_PHYS     7   /* fused inverse FFT + products + forward FFT (step) */
+#define LSD_K_COUNT    8
 
 /* Timing hooks (no counterpart in the reference beyond get_time): between
  * begin and end, every launch of the given class is bracketed by HIP events

@@ -12,127 +12,9 @@
 // before r2c, transformed, separated, and written as
 // G0 = X_j + X_mirror, G1 = X_j - X_mirror for m < M.
 #include "lsd_internal.h"
-
-#include <hip/hip_runtime.h>
+#include "fft_lds.h"
 
 namespace lsd {
-
-struct FftPlan {
-    int n;        // transform length Ph
-    int nstage;
-    int radix[16];
-};
-
-constexpr int FFT_THREADS = 256;
-constexpr int FFT_EPT = 16;  // complex elements per thread per stage
-
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-// multiply by sign*i
-__device__ __forceinline__ double2 mul_si(double2 a, double sg) { return make_double2(-sg * a.y, sg * a.x); }
-
-template <int R>
-__device__ __forceinline__ void dft(double2* v, double sg) {
-    if (R == 2) {
-        double2 a = v[0], b = v[1];
-        v[0] = cadd(a, b);
-        v[1] = csub(a, b);
-    } else if (R == 3) {
-        const double h = 0.86602540378443864676;  // sqrt(3)/2
-        double2 t = cadd(v[1], v[2]);
-        double2 d = csub(v[1], v[2]);
-        double2 m = make_double2(v[0].x - 0.5 * t.x, v[0].y - 0.5 * t.y);
-        double2 e = mul_si(make_double2(h * d.x, h * d.y), sg);
-        v[0] = cadd(v[0], t);
-        v[1] = cadd(m, e);
-        v[2] = csub(m, e);
-    } else if (R == 4) {
-        double2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
-        double2 t2 = cadd(v[1], v[3]), t3 = mul_si(csub(v[1], v[3]), sg);
-        v[0] = cadd(t0, t2);
-        v[2] = csub(t0, t2);
-        v[1] = cadd(t1, t3);
-        v[3] = csub(t1, t3);
-    } else if (R == 8) {
-        const double c = 0.70710678118654752440;
-        double2 e[4] = {v[0], v[2], v[4], v[6]};
-        double2 o[4] = {v[1], v[3], v[5], v[7]};
-        dft<4>(e, sg);
-        dft<4>(o, sg);
-        // w^k, w = exp(sg*2*pi*i/8)
-        double2 w1 = make_double2(c * (o[1].x - sg * o[1].y), c * (o[1].y + sg * o[1].x));
-        double2 w2 = mul_si(o[2], sg);
-        double2 w3 = make_double2(c * (-o[3].x - sg * o[3].y), c * (-o[3].y + sg * o[3].x));
-        v[0] = cadd(e[0], o[0]);
-        v[4] = csub(e[0], o[0]);
-        v[1] = cadd(e[1], w1);
-        v[5] = csub(e[1], w1);
-        v[2] = cadd(e[2], w2);
-        v[6] = csub(e[2], w2);
-        v[3] = cadd(e[3], w3);
-        v[7] = csub(e[3], w3);
-    }
-}
-
-// One Stockham stage over nf sequences of length n held in Z[p*n + i].
-template <int R>
-__device__ void stage(double2* Z, int n, int nf, int Ns, const double2* __restrict__ tw, double sg) {
-    constexpr int BPT = (FFT_EPT + R - 1) / R;
-    const int nb = n / R;
-    const int total = nf * nb;
-    const int step = n / (Ns * R);
-    double2 v[BPT][R];
-#pragma unroll
-    for (int q = 0; q < BPT; q++) {
-        int b = threadIdx.x + q * FFT_THREADS;
-        if (b < total) {
-            int p = b / nb, j = b - p * nb;
-            int jm = j % Ns;
-            const double2* src = Z + (size_t)p * n + j;
-#pragma unroll
-            for (int r = 0; r < R; r++) v[q][r] = src[r * nb];
-            if (jm) {
-#pragma unroll
-                for (int r = 1; r < R; r++) {
-                    double2 w = tw[jm * r * step];
-                    w.y *= sg;
-                    v[q][r] = cmul(v[q][r], w);
-                }
-            }
-            dft<R>(v[q], sg);
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < BPT; q++) {
-        int b = threadIdx.x + q * FFT_THREADS;
-        if (b < total) {
-            int p = b / nb, j = b - p * nb;
-            int jm = j % Ns;
-            double2* dst = Z + (size_t)p * n + (j / Ns) * Ns * R + jm;
-#pragma unroll
-            for (int r = 0; r < R; r++) dst[r * Ns] = v[q][r];
-        }
-    }
-    __syncthreads();
-}
-
-__device__ void fft_lds(double2* Z, const FftPlan& pl, int nf, const double2* __restrict__ tw, double sg) {
-    int Ns = 1;
-    for (int s = 0; s < pl.nstage; s++) {
-        int R = pl.radix[s];
-        switch (R) {
-        case 8: stage<8>(Z, pl.n, nf, Ns, tw, sg); break;
-        case 4: stage<4>(Z, pl.n, nf, Ns, tw, sg); break;
-        case 2: stage<2>(Z, pl.n, nf, Ns, tw, sg); break;
-        default: stage<3>(Z, pl.n, nf, Ns, tw, sg); break;
-        }
-        Ns *= R;
-    }
-}
 
 // fourier [f][n][M][Th] -> phys [f][n][Ph][Th]; grid (row blocks, N, fields).
 __global__ void __launch_bounds__(FFT_THREADS) k_fft_c2r(FftPlan pl, int nf, int N, int Th, int M,
@@ -159,7 +41,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_c2r(FftPlan pl, int nf, int
         }
     }
     __syncthreads();
-    fft_lds(Z, pl, nf, tw, 1.0);
+    fft_lds<double2>(Z, pl, nf, tw, 1.0);
     double* P = phys + ((size_t)f * N + sh) * (size_t)n * Th;
     const int rows = 2 * nf;
     for (int t = threadIdx.x; t < n * rows; t += FFT_THREADS) {
@@ -194,7 +76,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_r2c_fold(FftPlan pl, int nf
         Z[(size_t)p * n + k] = make_double2(a, b);
     }
     __syncthreads();
-    fft_lds(Z, pl, nf, tw, -1.0);
+    fft_lds<double2>(Z, pl, nf, tw, -1.0);
     double2* G = folded + ((size_t)f * N + sh) * (size_t)M * 2 * Th2;
     for (int t = threadIdx.x; t < M * 2 * nf; t += FFT_THREADS) {
         int k = t / (2 * nf), rem = t - k * 2 * nf;
@@ -205,14 +87,14 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_r2c_fold(FftPlan pl, int nf
         double2 zc = Z[(size_t)p * n + (k ? n - k : 0)];
         zc.y = -zc.y;
         double2 A = make_double2(0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y));
-        double2 D = csub(zk, zc);
+        double2 D = csubv(zk, zc);
         double2 B = make_double2(0.5 * D.y, -0.5 * D.x);
-        double2 out = par ? csub(A, B) : cadd(A, B);
+        double2 out = par ? csubv(A, B) : caddv(A, B);
         G[((size_t)k * 2 + par) * Th2 + j] = out;
     }
 }
 
-static FftPlan make_plan(int n) {
+FftPlan make_fft_plan(int n) {
     FftPlan p{};
     p.n = n;
     int rem = n;
@@ -225,22 +107,24 @@ static FftPlan make_plan(int n) {
 }
 
 // Twiddle table exp(+2 pi i k / Ph), k < Ph, built on the device.
-__global__ void k_twiddle(int n, double2* __restrict__ tw) {
+__global__ void k_twiddle(int n, double2* __restrict__ tw, float2* __restrict__ tw32) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     double s, c;
     sincospi(2.0 * (double)k / (double)n, &s, &c);
     tw[k] = make_double2(c, s);
+    tw32[k] = make_float2((float)c, (float)s);
 }
 
 int fft_setup(Ctx& c) {
-    FftPlan p = make_plan(c.Ph);
+    FftPlan p = make_fft_plan(c.Ph);
     if (p.nstage < 0) {
         set_error("Ph must be a product of 2s and 3s");
         return LSD_E_CONFIG;
     }
     LSD_HIP(hipMalloc((void**)&c.d_tw, sizeof(double2) * c.Ph));
-    hipLaunchKernelGGL(k_twiddle, dim3((c.Ph + 255) / 256), dim3(256), 0, c.stream, c.Ph, c.d_tw);
+    LSD_HIP(hipMalloc((void**)&c.d_tw32, sizeof(float2) * c.Ph));
+    hipLaunchKernelGGL(k_twiddle, dim3((c.Ph + 255) / 256), dim3(256), 0, c.stream, c.Ph, c.d_tw, c.d_tw32);
     LSD_HIP(hipGetLastError());
     return LSD_OK;
 }
@@ -253,7 +137,7 @@ static int nfft_per_block(const Ctx& c, int rows_needed) {
 }
 
 int launch_fft_c2r(Ctx& c, const double2* fourier, double* phys, int nfield) {
-    FftPlan p = make_plan(c.Ph);
+    FftPlan p = make_fft_plan(c.Ph);
     int nf = nfft_per_block(c, (c.Th + 1) / 2);
     dim3 grid((c.Th + 2 * nf - 1) / (2 * nf), c.N, nfield);
     size_t lds = sizeof(double2) * nf * c.Ph;
@@ -265,7 +149,7 @@ int launch_fft_c2r(Ctx& c, const double2* fourier, double* phys, int nfield) {
 }
 
 int launch_fft_r2c_fold(Ctx& c, const double* phys, double2* folded, int nfield) {
-    FftPlan p = make_plan(c.Ph);
+    FftPlan p = make_fft_plan(c.Ph);
     int nf = nfft_per_block(c, c.Th2);
     dim3 grid((c.Th2 + nf - 1) / nf, c.N, nfield);
     size_t lds = sizeof(double2) * nf * c.Ph;

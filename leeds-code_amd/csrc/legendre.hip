@@ -235,15 +235,24 @@ static int syn_launch(Ctx& c, int kind, const T* tA, const T* tB, const T* in0, 
 // kind SYN_SCALAR: in0 -> out0 (table P)
 // kind SYN_VECTOR: (in0 = s, in1 = t) -> (out0 = theta, out1 = phi)
 // kind SYN_GRAD:   in0 -> (out0 = theta, out1 = phi)
-int launch_synthesis(Ctx& c, int kind, const double2* in0, const double2* in1, double2* out0, double2* out1) {
+// f32: FP32 tables, inputs and outputs (float2 spectral / Fourier layouts).
+int launch_synthesis_p(Ctx& c, int kind, const void* in0, const void* in1, void* out0, void* out1, bool f32) {
     size_t HT = (size_t)c.H * c.Th2;
     const double unit = 4.0 * (double)c.H * c.Th2 * c.N;  // flops per product unit
     ProfScope ps(c, LSD_K_SYNTH, unit * (kind == SYN_SCALAR ? 1 : kind == SYN_VECTOR ? 4 : 2));
-    const double* t0 = c.d_tab + (kind == SYN_SCALAR ? 0 : kind == SYN_VECTOR ? 3 : 1) * HT;
-    const double* t1 = c.d_tab + (kind == SYN_VECTOR ? 4 : 2) * HT;
-    return syn_launch<double>(c, kind, t0, t1, reinterpret_cast<const double*>(in0),
+    const size_t o0 = (kind == SYN_SCALAR ? 0 : kind == SYN_VECTOR ? 3 : 1) * HT;
+    const size_t o1 = (kind == SYN_VECTOR ? 4 : 2) * HT;
+    if (f32)
+        return syn_launch<float>(c, kind, c.d_tab32 + o0, c.d_tab32 + o1, reinterpret_cast<const float*>(in0),
+                                 reinterpret_cast<const float*>(in1), reinterpret_cast<float*>(out0),
+                                 reinterpret_cast<float*>(out1));
+    return syn_launch<double>(c, kind, c.d_tab + o0, c.d_tab + o1, reinterpret_cast<const double*>(in0),
                               reinterpret_cast<const double*>(in1), reinterpret_cast<double*>(out0),
                               reinterpret_cast<double*>(out1));
+}
+
+int launch_synthesis(Ctx& c, int kind, const double2* in0, const double2* in1, double2* out0, double2* out1) {
+    return launch_synthesis_p(c, kind, in0, in1, out0, out1, false);
 }
 
 // --------------------------------------------------------------- analysis
@@ -420,15 +429,25 @@ static int ana_launch(Ctx& c, int kind, const T* tP, const T* tL, const T* tS, c
     return LSD_OK;
 }
 
-int launch_analysis(Ctx& c, int kind, const double2* g0, const double2* g1, const double2* g2, double2* o0,
-                    double2* o1, double2* o2) {
+int launch_analysis_p(Ctx& c, int kind, const void* g0, const void* g1, const void* g2, void* o0, void* o1,
+                      void* o2, bool f32) {
     size_t HT = (size_t)c.H * c.Th2;
     const double unit = 4.0 * (double)c.H * c.Th2 * c.N;
     ProfScope ps(c, LSD_K_ANAL, unit * (kind == ANA_SCALAR ? 1 : 5));
+    if (f32)
+        return ana_launch<float>(c, kind, c.d_atab32, c.d_atab32 + HT, c.d_atab32 + 2 * HT,
+                                 reinterpret_cast<const float*>(g0), reinterpret_cast<const float*>(g1),
+                                 reinterpret_cast<const float*>(g2), reinterpret_cast<float*>(o0),
+                                 reinterpret_cast<float*>(o1), reinterpret_cast<float*>(o2));
     return ana_launch<double>(c, kind, c.d_atab, c.d_atab + HT, c.d_atab + 2 * HT,
                               reinterpret_cast<const double*>(g0), reinterpret_cast<const double*>(g1),
                               reinterpret_cast<const double*>(g2), reinterpret_cast<double*>(o0),
                               reinterpret_cast<double*>(o1), reinterpret_cast<double*>(o2));
+}
+
+int launch_analysis(Ctx& c, int kind, const double2* g0, const double2* g1, const double2* g2, double2* o0,
+                    double2* o1, double2* o2) {
+    return launch_analysis_p(c, kind, g0, g1, g2, o0, o1, o2, false);
 }
 
 }  // namespace lsd

@@ -33,7 +33,14 @@ static int check_config(const lsd_config* c, int* H, int* Th, int* Ph) {
         return LSD_E_CONFIG;
     }
     if (c->dt <= 0.0) { set_error("dt must be > 0 (fixed step)"); return LSD_E_CONFIG; }
-    if (c->precision != LSD_PLAIN_F64) { set_error("only LSD_PLAIN_F64 is available in this build"); return LSD_E_CONFIG; }
+    if (c->precision != LSD_PLAIN_F64 && c->precision != LSD_MIXED_F32) {
+        set_error("precision must be LSD_PLAIN_F64 or LSD_MIXED_F32");
+        return LSD_E_CONFIG;
+    }
+    if (c->precision != LSD_PLAIN_F64 && ph > 512) {
+        set_error("the mixed forms need Ph <= 512 (fused physical-space pass)");
+        return LSD_E_CONFIG;
+    }
     if (H) *H = c->L * c->M - c->Mp * (c->M - 1) * c->M / 2;
     if (Th) *Th = th;
     if (Ph) *Ph = ph;
@@ -66,13 +73,17 @@ static int dalloc(T** p, size_t n) {
     return LSD_OK;
 }
 
+static void drop_graphs(Ctx& c);
+
 static void release(Ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     void* ptrs[] = {c->d_hl, c->d_hm, c->d_pidx, c->d_apos, c->d_slots, c->d_slot_order, c->d_gp, c->d_gw,
                     c->d_sin, c->d_tab, c->d_atab, c->d_r, c->d_dr1, c->d_dr2, c->d_radlap, c->d_r1dr,
                     c->d_intr2dr, c->d_ll, c->d_lu, c->d_piv, c->d_y, c->d_gre, c->d_invg, c->d_info,
-                    c->d_spec_in, c->d_spec_out, c->d_work, c->d_fourier, c->d_folded, c->d_phys, c->d_nlphys,
+                    c->d_lut, c->d_pivt, c->d_yt, c->d_gret, c->d_lut32, c->d_xband, c->d_xt, c->d_tab32,
+                    c->d_atab32, c->d_tw32,
+                    c->d_spec_in, c->d_spec_out, c->d_fourier, c->d_folded, c->d_phys, c->d_nlphys,
                     c->d_tw, c->d_partial};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -82,6 +93,7 @@ static void release(Ctx* c) {
         if (c->d_hist[k]) (void)hipFree(c->d_hist[k]);
     if (c->h_partial) (void)hipHostFree(c->h_partial);
     for (hipEvent_t e : c->prof_events) (void)hipEventDestroy(e);
+    drop_graphs(*c);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -102,6 +114,7 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
     c->H = H;
     c->Th2 = (Th + 1) / 2;
     c->dt = cfg->dt;
+    c->mixed = cfg->precision != LSD_PLAIN_F64;
     *out = c;
     LSD_HIP(hipSetDevice(device));
     LSD_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -135,7 +148,11 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
         dalloc(&c->d_piv, (size_t)NLU * L * N) || dalloc(&c->d_y, (size_t)NY * L * N * 7) ||
         dalloc(&c->d_gre, (size_t)2 * L * N) || dalloc(&c->d_invg, (size_t)4 * L) ||
         dalloc(&c->d_info, (size_t)NLU * L) || dalloc(&c->d_spec_in, (size_t)SI_COUNT * HN) ||
-        dalloc(&c->d_spec_out, (size_t)SO_COUNT * HN) || dalloc(&c->d_work, (size_t)6 * HN) ||
+        dalloc(&c->d_spec_out, (size_t)SO_COUNT * HN) ||
+        dalloc(&c->d_lut, (size_t)NLU * L * N * LSD_LDAB) || dalloc(&c->d_pivt, (size_t)NLU * L * N) ||
+        dalloc(&c->d_yt, (size_t)NY * L * N * 7) || dalloc(&c->d_gret, (size_t)2 * L * N) ||
+        dalloc(&c->d_lut32, (size_t)NLU * L * N * LSD_LDAB) || dalloc(&c->d_xband, (size_t)NLU * L * N * 7) ||
+        dalloc(&c->d_xt, (size_t)NLU * L * N * 7) ||
         dalloc(&c->d_fourier, (size_t)LSD_NUM_PHYS * c->fourier_elems()) ||
         dalloc(&c->d_folded, (size_t)NO_COUNT * c->folded_elems()) ||
         dalloc(&c->d_phys, (size_t)LSD_NUM_PHYS * c->phys_elems()) ||
@@ -145,7 +162,8 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
         if (dalloc(&c->d_f[k], HN)) return LSD_E_HIP;
     for (int k = 0; k < 6; k++)
         if (dalloc(&c->d_hist[k], HN)) return LSD_E_HIP;
-    c->n_partial_blocks = (H + 63) / 64;
+    c->solve_hb = solve_block_harmonics(N, c->mixed);
+    c->n_partial_blocks = (H + c->solve_hb - 1) / c->solve_hb;
     if (dalloc(&c->d_partial, (size_t)8 * c->n_partial_blocks)) return LSD_E_HIP;
     LSD_HIP(hipHostMalloc((void**)&c->h_partial, sizeof(double) * 8 * c->n_partial_blocks, hipHostMallocDefault));
 
@@ -161,6 +179,7 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
     LSD_HIP(hipMemsetAsync(c->d_lu, 0, sizeof(double) * NLU * L * N * LSD_LDAB, st));
     LSD_HIP(hipMemsetAsync(c->d_piv, 0, sizeof(int) * NLU * L * N, st));
     LSD_HIP(hipMemsetAsync(c->d_y, 0, sizeof(double) * NY * L * N * 7, st));
+    LSD_HIP(hipMemsetAsync(c->d_xband, 0, sizeof(double) * NLU * L * N * 7, st));
     LSD_HIP(hipMemsetAsync(c->d_gre, 0, sizeof(double) * 2 * L * N, st));
     LSD_HIP(hipMemsetAsync(c->d_invg, 0, sizeof(double) * 4 * L, st));
     for (int k = 0; k < LSD_NUM_COLL; k++) LSD_HIP(hipMemsetAsync(c->d_f[k], 0, sizeof(double2) * HN, st));
@@ -175,6 +194,11 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
     if ((rc = fft_setup(*c))) return rc;
     if ((rc = precompute_mesh(*c))) return rc;
     if ((rc = precompute_tables(*c))) return rc;
+    if (c->mixed) {
+        const size_t HT = (size_t)H * c->Th2;
+        if (dalloc(&c->d_tab32, 5 * HT) || dalloc(&c->d_atab32, 3 * HT)) return LSD_E_HIP;
+        if ((rc = tables_to_f32(*c))) return rc;
+    }
     if ((rc = build_operators(*c))) return rc;
     LSD_HIP(hipStreamSynchronize(st));
     return LSD_OK;
@@ -182,26 +206,65 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
 
 // ------------------------------------------------------------ timing
 
-ProfScope::ProfScope(Ctx& ctx, int cls, double work) : c(ctx), on(ctx.prof_class == cls), idx(-1) {
+ProfScope::ProfScope(Ctx& ctx, int cls, double work) : c(ctx), on(ctx.prof_class == cls) {
     if (!on) return;
-    if ((size_t)(c.prof_used + 2) > c.prof_events.size()) {
-        for (int k = 0; k < 64; k++) {
-            hipEvent_t e;
-            if (hipEventCreate(&e) != hipSuccess) {
-                on = false;
-                return;
-            }
-            c.prof_events.push_back(e);
+    hipEvent_t start;
+    if (c.cap >= 0) {
+        // inside a capture: a pair of its own, replayed with the graph
+        if (hipEventCreate(&start) != hipSuccess || hipEventCreate(&stop) != hipSuccess) {
+            on = false;
+            return;
         }
+        c.g_ev[c.cap].push_back(start);
+        c.g_ev[c.cap].push_back(stop);
+        c.g_work[c.cap] += work;
+    } else {
+        if ((size_t)(c.prof_used + 2) > c.prof_events.size()) {
+            for (int k = 0; k < 64; k++) {
+                hipEvent_t e;
+                if (hipEventCreate(&e) != hipSuccess) {
+                    on = false;
+                    return;
+                }
+                c.prof_events.push_back(e);
+            }
+        }
+        start = c.prof_events[c.prof_used];
+        stop = c.prof_events[c.prof_used + 1];
+        c.prof_used += 2;
+        c.prof_work += work;
     }
-    idx = c.prof_used;
-    c.prof_used += 2;
-    c.prof_work += work;
-    (void)hipEventRecord(c.prof_events[idx], c.stream);
+    (void)hipEventRecord(start, c.stream);
 }
 
 ProfScope::~ProfScope() {
-    if (on && idx >= 0) (void)hipEventRecord(c.prof_events[idx + 1], c.stream);
+    if (on && stop) (void)hipEventRecord(stop, c.stream);
+}
+
+// After a replay of graph `which` has completed: add its timed launches.
+static int graph_prof_accumulate(Ctx& c, int which) {
+    if (c.prof_class < 0 || c.g_prof_class != c.prof_class) return LSD_OK;
+    const std::vector<hipEvent_t>& ev = c.g_ev[which];
+    for (size_t k = 0; k + 1 < ev.size(); k += 2) {
+        float ms = 0.f;
+        LSD_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+        c.prof_acc_ms += ms;
+        c.prof_acc_launches++;
+    }
+    c.prof_acc_work += c.g_work[which];
+    return LSD_OK;
+}
+
+static void drop_graphs(Ctx& c) {
+    if (c.g_first) (void)hipGraphExecDestroy(c.g_first);
+    if (c.g_more) (void)hipGraphExecDestroy(c.g_more);
+    c.g_first = c.g_more = nullptr;
+    for (int w = 0; w < 2; w++) {
+        for (hipEvent_t e : c.g_ev[w]) (void)hipEventDestroy(e);
+        c.g_ev[w].clear();
+        c.g_work[w] = 0.0;
+    }
+    c.g_prof_class = -1;
 }
 
 // ------------------------------------------------------------- phases
@@ -219,65 +282,106 @@ static double* nlphys(Ctx& c, int k) { return c.d_nlphys + (size_t)k * c.phys_el
         if (r_) return r_; \
     } while (0)
 
-// vel_transform, cod_transform, comp_transform, mag_transform
-static int transform(Ctx& c) {
-    RC(launch_vec_prep(c, c.d_f[LSD_VEL_TOR], c.d_f[LSD_VEL_POL], spec_in(c, SI_UQ), spec_in(c, SI_US),
-                       spec_in(c, SI_UT), spec_in(c, SI_WQ), spec_in(c, SI_WS), spec_in(c, SI_WT)));
-    RC(launch_synthesis(c, SYN_SCALAR, spec_in(c, SI_UQ), nullptr, fourier(c, LSD_P_VEL_R), nullptr));
-    RC(launch_synthesis(c, SYN_VECTOR, spec_in(c, SI_US), spec_in(c, SI_UT), fourier(c, LSD_P_VEL_T),
-                        fourier(c, LSD_P_VEL_P)));
-    RC(launch_synthesis(c, SYN_SCALAR, spec_in(c, SI_WQ), nullptr, fourier(c, LSD_P_VCURL_R), nullptr));
-    RC(launch_synthesis(c, SYN_VECTOR, spec_in(c, SI_WS), spec_in(c, SI_WT), fourier(c, LSD_P_VCURL_T),
-                        fourier(c, LSD_P_VCURL_P)));
-    RC(launch_fft_c2r(c, fourier(c, 0), phys(c, 0), 6));
+// Buffers in the transform precision: FP32 forms use the first half of the
+// FP64 allocations.
+template <typename B>
+static void* slot_p(B* base, size_t per, int k, bool f32) {
+    if (f32) return reinterpret_cast<float2*>(base) + (size_t)k * per;
+    return reinterpret_cast<double2*>(base) + (size_t)k * per;
+}
+static void* spec_in_p(Ctx& c, int k, bool f) { return slot_p(c.d_spec_in, c.coll_elems(), k, f); }
+static void* spec_out_p(Ctx& c, int k, bool f) { return slot_p(c.d_spec_out, c.coll_elems(), k, f); }
+static void* fourier_p(Ctx& c, int k, bool f) { return slot_p(c.d_fourier, c.fourier_elems(), k, f); }
+static void* folded_p(Ctx& c, int k, bool f) { return slot_p(c.d_folded, c.folded_elems(), k, f); }
+
+// Synthesis of the fields the nonlinear terms need (vel_transform,
+// mag_transform, cod_transform, comp_transform), with the c2r FFTs to the
+// physical buffers when `to_phys` (FP64 only).
+static int synthesise(Ctx& c, bool f, bool to_phys) {
+    const bool fft = to_phys && !f;
+    {
+        void* o[6] = {spec_in_p(c, SI_UQ, f), spec_in_p(c, SI_US, f), spec_in_p(c, SI_UT, f),
+                      spec_in_p(c, SI_WQ, f), spec_in_p(c, SI_WS, f), spec_in_p(c, SI_WT, f)};
+        RC(launch_vec_prep(c, c.d_f[LSD_VEL_TOR], c.d_f[LSD_VEL_POL], o, f));
+        RC(launch_synthesis_p(c, SYN_SCALAR, o[0], nullptr, fourier_p(c, LSD_P_VEL_R, f), nullptr, f));
+        RC(launch_synthesis_p(c, SYN_VECTOR, o[1], o[2], fourier_p(c, LSD_P_VEL_T, f), fourier_p(c, LSD_P_VEL_P, f),
+                              f));
+        RC(launch_synthesis_p(c, SYN_SCALAR, o[3], nullptr, fourier_p(c, LSD_P_VCURL_R, f), nullptr, f));
+        RC(launch_synthesis_p(c, SYN_VECTOR, o[4], o[5], fourier_p(c, LSD_P_VCURL_T, f),
+                              fourier_p(c, LSD_P_VCURL_P, f), f));
+        if (fft) RC(launch_fft_c2r(c, fourier(c, 0), phys(c, 0), 6));
+    }
     if (c.cfg.mag) {
-        RC(launch_vec_prep(c, c.d_f[LSD_MAG_TOR], c.d_f[LSD_MAG_POL], spec_in(c, SI_BQ), spec_in(c, SI_BS),
-                           spec_in(c, SI_BT), spec_in(c, SI_JQ), spec_in(c, SI_JS), spec_in(c, SI_JT)));
-        RC(launch_synthesis(c, SYN_SCALAR, spec_in(c, SI_BQ), nullptr, fourier(c, LSD_P_MAG_R), nullptr));
-        RC(launch_synthesis(c, SYN_VECTOR, spec_in(c, SI_BS), spec_in(c, SI_BT), fourier(c, LSD_P_MAG_T),
-                            fourier(c, LSD_P_MAG_P)));
-        RC(launch_synthesis(c, SYN_SCALAR, spec_in(c, SI_JQ), nullptr, fourier(c, LSD_P_MCURL_R), nullptr));
-        RC(launch_synthesis(c, SYN_VECTOR, spec_in(c, SI_JS), spec_in(c, SI_JT), fourier(c, LSD_P_MCURL_T),
-                            fourier(c, LSD_P_MCURL_P)));
-        RC(launch_fft_c2r(c, fourier(c, 6), phys(c, 6), 6));
+        void* o[6] = {spec_in_p(c, SI_BQ, f), spec_in_p(c, SI_BS, f), spec_in_p(c, SI_BT, f),
+                      spec_in_p(c, SI_JQ, f), spec_in_p(c, SI_JS, f), spec_in_p(c, SI_JT, f)};
+        RC(launch_vec_prep(c, c.d_f[LSD_MAG_TOR], c.d_f[LSD_MAG_POL], o, f));
+        RC(launch_synthesis_p(c, SYN_SCALAR, o[0], nullptr, fourier_p(c, LSD_P_MAG_R, f), nullptr, f));
+        RC(launch_synthesis_p(c, SYN_VECTOR, o[1], o[2], fourier_p(c, LSD_P_MAG_T, f), fourier_p(c, LSD_P_MAG_P, f),
+                              f));
+        RC(launch_synthesis_p(c, SYN_SCALAR, o[3], nullptr, fourier_p(c, LSD_P_MCURL_R, f), nullptr, f));
+        RC(launch_synthesis_p(c, SYN_VECTOR, o[4], o[5], fourier_p(c, LSD_P_MCURL_T, f),
+                              fourier_p(c, LSD_P_MCURL_P, f), f));
+        if (fft) RC(launch_fft_c2r(c, fourier(c, 6), phys(c, 6), 6));
     }
-    if (c.cfg.cod) {
-        RC(launch_scal_prep(c, c.d_f[LSD_COD], spec_in(c, SI_DC)));
-        RC(launch_synthesis(c, SYN_SCALAR, spec_in(c, SI_DC), nullptr, fourier(c, LSD_P_COD_GR), nullptr));
-        RC(launch_synthesis(c, SYN_GRAD, c.d_f[LSD_COD], nullptr, fourier(c, LSD_P_COD_GT), fourier(c, LSD_P_COD_GP)));
-        RC(launch_fft_c2r(c, fourier(c, 12), phys(c, 12), 3));
-    }
-    if (c.cfg.comp) {
-        RC(launch_scal_prep(c, c.d_f[LSD_COMP], spec_in(c, SI_DX)));
-        RC(launch_synthesis(c, SYN_SCALAR, spec_in(c, SI_DX), nullptr, fourier(c, LSD_P_COMP_GR), nullptr));
-        RC(launch_synthesis(c, SYN_GRAD, c.d_f[LSD_COMP], nullptr, fourier(c, LSD_P_COMP_GT),
-                            fourier(c, LSD_P_COMP_GP)));
-        RC(launch_fft_c2r(c, fourier(c, 15), phys(c, 15), 3));
+    // scalars: radial derivative, then (dC/dr) P and the angular gradient
+    const int sf[2] = {LSD_COD, LSD_COMP}, sd[2] = {SI_DC, SI_DX}, sc[2] = {SI_CC, SI_XC};
+    const int pr[2] = {LSD_P_COD_GR, LSD_P_COMP_GR};
+    for (int q = 0; q < 2; q++) {
+        if (!(q == 0 ? c.cfg.cod : c.cfg.comp)) continue;
+        void* dC = spec_in_p(c, sd[q], f);
+        void* Cc = f ? spec_in_p(c, sc[q], f) : nullptr;
+        RC(launch_scal_prep(c, c.d_f[sf[q]], dC, Cc, f));
+        RC(launch_synthesis_p(c, SYN_SCALAR, dC, nullptr, fourier_p(c, pr[q], f), nullptr, f));
+        RC(launch_synthesis_p(c, SYN_GRAD, f ? Cc : (void*)c.d_f[sf[q]], nullptr, fourier_p(c, pr[q] + 1, f),
+                              fourier_p(c, pr[q] + 2, f), f));
+        if (fft) RC(launch_fft_c2r(c, fourier(c, pr[q]), phys(c, pr[q]), 3));
     }
     return LSD_OK;
 }
 
-// non_velocity, non_codensity, non_composition, non_magnetic
+// vel_transform, cod_transform, comp_transform, mag_transform (FP64, to the
+// physical buffers)
+static int transform(Ctx& c) { return synthesise(c, false, true); }
+
+// Analysis of the folded nonlinear products and the coll-side completion of
+// non_velocity, non_codensity, non_composition, non_magnetic.
+static int analyse(Ctx& c, bool f) {
+    RC(launch_analysis_p(c, ANA_VECTOR, folded_p(c, NO_UR, f), folded_p(c, NO_UT, f), folded_p(c, NO_UP, f),
+                         spec_out_p(c, SO_UQ, f), spec_out_p(c, SO_US, f), spec_out_p(c, SO_UT, f), f));
+    if (c.cfg.cod)
+        RC(launch_analysis_p(c, ANA_SCALAR, folded_p(c, NO_C, f), nullptr, nullptr, spec_out_p(c, SO_C, f), nullptr,
+                             nullptr, f));
+    if (c.cfg.comp)
+        RC(launch_analysis_p(c, ANA_SCALAR, folded_p(c, NO_X, f), nullptr, nullptr, spec_out_p(c, SO_X, f), nullptr,
+                             nullptr, f));
+    if (c.cfg.mag)
+        RC(launch_analysis_p(c, ANA_VECTOR, folded_p(c, NO_ER, f), folded_p(c, NO_ET, f), folded_p(c, NO_EP, f),
+                             spec_out_p(c, SO_EQ, f), spec_out_p(c, SO_ES, f), spec_out_p(c, SO_ET, f), f));
+    RC(launch_spec_post(c, f));
+    return LSD_OK;
+}
+
+// non_velocity, non_codensity, non_composition, non_magnetic on the
+// physical buffers left by transform() (FP64)
 static int nonlinear(Ctx& c) {
     RC(launch_nonlinear(c));
     RC(launch_fft_r2c_fold(c, nlphys(c, NO_UR), folded(c, NO_UR), 3));
-    RC(launch_analysis(c, ANA_VECTOR, folded(c, NO_UR), folded(c, NO_UT), folded(c, NO_UP), spec_out(c, SO_UQ),
-                       spec_out(c, SO_US), spec_out(c, SO_UT)));
-    if (c.cfg.cod) {
-        RC(launch_fft_r2c_fold(c, nlphys(c, NO_C), folded(c, NO_C), 1));
-        RC(launch_analysis(c, ANA_SCALAR, folded(c, NO_C), nullptr, nullptr, spec_out(c, SO_C), nullptr, nullptr));
+    if (c.cfg.cod) RC(launch_fft_r2c_fold(c, nlphys(c, NO_C), folded(c, NO_C), 1));
+    if (c.cfg.comp) RC(launch_fft_r2c_fold(c, nlphys(c, NO_X), folded(c, NO_X), 1));
+    if (c.cfg.mag) RC(launch_fft_r2c_fold(c, nlphys(c, NO_ER), folded(c, NO_ER), 3));
+    return analyse(c, false);
+}
+
+// One nonlinear evaluation inside the timestep: synthesis, the fused
+// physical-space pass, analysis.  The physical fields stay on chip.
+static int evaluate(Ctx& c) {
+    if (!physfuse_supported(c)) {
+        RC(transform(c));
+        return nonlinear(c);
     }
-    if (c.cfg.comp) {
-        RC(launch_fft_r2c_fold(c, nlphys(c, NO_X), folded(c, NO_X), 1));
-        RC(launch_analysis(c, ANA_SCALAR, folded(c, NO_X), nullptr, nullptr, spec_out(c, SO_X), nullptr, nullptr));
-    }
-    if (c.cfg.mag) {
-        RC(launch_fft_r2c_fold(c, nlphys(c, NO_ER), folded(c, NO_ER), 3));
-        RC(launch_analysis(c, ANA_VECTOR, folded(c, NO_ER), folded(c, NO_ET), folded(c, NO_EP), spec_out(c, SO_EQ),
-                           spec_out(c, SO_ES), spec_out(c, SO_ET)));
-    }
-    RC(launch_spec_post(c));
-    return LSD_OK;
+    RC(synthesise(c, c.mixed, false));
+    RC(launch_physfuse(c, c.mixed));
+    return analyse(c, c.mixed);
 }
 
 // tim_check_cgce, i_time_step_control = 2 (modules/timestep.F90:791-827).
@@ -307,37 +411,104 @@ static int check_cgce(Ctx& c) {
     return stop;
 }
 
-static int step(Ctx& c, int nsteps, lsd_step_stats* st) {
-    lsd_step_stats z;
-    std::memset(&z, 0, sizeof z);
+// Host side of one corrector pass: accumulate tim_dterr, tim_check_cgce.
+static int after_corrector(Ctx& c, int& its, double& last) {
+    c.dterr += corrector_norms(c);
+    last = c.dterr;
+    its++;
+    if (check_cgce(c)) {
+        set_error("tim_check_cgce: run stopped");
+        return LSD_E_DIVERGED;
+    }
+    return LSD_OK;
+}
+
+static int capture(Ctx& c, bool first, hipGraphExec_t* out) {
+    hipGraph_t g = nullptr;
+    LSD_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    c.cap = first ? 0 : 1;
+    int rc = evaluate(c);
+    if (!rc && first) rc = launch_predictor(c);
+    if (!rc && first) rc = evaluate(c);
+    if (!rc) rc = launch_corrector_async(c);
+    c.cap = -1;
+    hipError_t e = hipStreamEndCapture(c.stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    LSD_HIP(e);
+    hipError_t ie = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    LSD_HIP(ie);
+    return LSD_OK;
+}
+
+// main.F90:100-164 replayed from two captured graphs: the first covers the
+// predictor pass and the first corrector pass of a step, the second any
+// further corrector pass tim_check_cgce asks for.
+static int step_graph(Ctx& c, int nsteps, lsd_step_stats* z) {
+    if (c.g_first && c.g_prof_class != c.prof_class) drop_graphs(c);
+    if (!c.g_first) {
+        RC(capture(c, true, &c.g_first));
+        RC(capture(c, false, &c.g_more));
+        c.g_prof_class = c.prof_class;
+    }
     for (int k = 0; k < nsteps; k++) {
         int its = 0;
         double last = 0.0;
-        do {
-            RC(transform(c));
-            RC(nonlinear(c));
-            z.evaluations++;
-            if (c.it == 0) {
-                RC(launch_predictor(c));
-                c.it = 1;
-            } else {
-                double d = 0.0;
-                RC(launch_corrector(c, &d));
-                c.dterr += d;
-                last = c.dterr;
-                its++;
-                if (check_cgce(c)) {
-                    set_error("tim_check_cgce: run stopped");
-                    return LSD_E_DIVERGED;
-                }
-            }
-        } while (c.it != 0);
+        LSD_HIP(hipGraphLaunch(c.g_first, c.stream));
+        LSD_HIP(hipStreamSynchronize(c.stream));
+        RC(graph_prof_accumulate(c, 0));
+        z->evaluations += 2;
+        c.it = 1;
+        RC(after_corrector(c, its, last));
+        while (c.it != 0) {
+            LSD_HIP(hipGraphLaunch(c.g_more, c.stream));
+            LSD_HIP(hipStreamSynchronize(c.stream));
+            RC(graph_prof_accumulate(c, 1));
+            z->evaluations++;
+            RC(after_corrector(c, its, last));
+        }
         c.t = c.t + c.dt;
         c.step++;
-        z.steps++;
-        z.last_its = its;
-        z.max_its = std::max(z.max_its, its);
-        z.last_dterr = last;
+        z->steps++;
+        z->last_its = its;
+        z->max_its = std::max(z->max_its, its);
+        z->last_dterr = last;
+    }
+    return LSD_OK;
+}
+
+static int step(Ctx& c, int nsteps, lsd_step_stats* st) {
+    lsd_step_stats z;
+    std::memset(&z, 0, sizeof z);
+    // graphs need the loop to start a fresh step
+    if (c.use_graph && c.it == 0) {
+        RC(step_graph(c, nsteps, &z));
+    } else {
+        for (int k = 0; k < nsteps; k++) {
+            int its = 0;
+            double last = 0.0;
+            do {
+                RC(evaluate(c));
+                z.evaluations++;
+                if (c.it == 0) {
+                    RC(launch_predictor(c));
+                    c.it = 1;
+                } else {
+                    RC(launch_corrector_async(c));
+                    LSD_HIP(hipStreamSynchronize(c.stream));
+                    RC(after_corrector(c, its, last));
+                }
+            } while (c.it != 0);
+            c.t = c.t + c.dt;
+            c.step++;
+            z.steps++;
+            z.last_its = its;
+            z.max_its = std::max(z.max_its, its);
+            z.last_dterr = last;
+        }
     }
     z.t = c.t;
     if (st) *st = z;
@@ -519,6 +690,13 @@ int lsd_corrector(lsd_ctx* ctx, double* dterr) {
     return LSD_OK;
 }
 
+int lsd_set_graph(lsd_ctx* ctx, int on) {
+    CTX_OR_FAIL(ctx);
+    LSD_HIP(hipStreamSynchronize(c.stream));
+    c.use_graph = on != 0;
+    return LSD_OK;
+}
+
 int lsd_step(lsd_ctx* ctx, int nsteps, lsd_step_stats* stats) {
     CTX_OR_FAIL(ctx);
     int rc = lsd::step(c, nsteps, stats);
@@ -537,6 +715,8 @@ int lsd_profile_begin(lsd_ctx* ctx, int kernel_class) {
     c.prof_class = kernel_class;
     c.prof_used = 0;
     c.prof_work = 0.0;
+    c.prof_acc_ms = c.prof_acc_work = 0.0;
+    c.prof_acc_launches = 0;
     return LSD_OK;
 }
 
@@ -549,12 +729,14 @@ int lsd_profile_end(lsd_ctx* ctx, double* total_ms, int* launches, double* work)
         LSD_HIP(hipEventElapsedTime(&ms, c.prof_events[k], c.prof_events[k + 1]));
         tot += ms;
     }
-    if (total_ms) *total_ms = tot;
-    if (launches) *launches = c.prof_used / 2;
-    if (work) *work = c.prof_work;
+    if (total_ms) *total_ms = tot + c.prof_acc_ms;
+    if (launches) *launches = c.prof_used / 2 + c.prof_acc_launches;
+    if (work) *work = c.prof_work + c.prof_acc_work;
     c.prof_class = -1;
     c.prof_used = 0;
     c.prof_work = 0.0;
+    c.prof_acc_ms = c.prof_acc_work = 0.0;
+    c.prof_acc_launches = 0;
     return LSD_OK;
 }
 

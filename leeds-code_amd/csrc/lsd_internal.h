@@ -11,7 +11,9 @@
 //            analysis   double [j][h]  (h fastest, so lanes over harmonics
 //                                       read coalesced)
 //   bands    mesh  double [n][7]       M(KL+1+n-j, j) stored per column j
-//            lu    double [l][n][10]   LAPACK band storage + int pivots
+//            lu    double [l][n][10]   LAPACK band storage + int pivots, as
+//                                       built; the solves read a copy laid out
+//                                       [kind][n][row][l] (degree fastest)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -63,7 +65,9 @@ constexpr int NY = 6;
 enum SpecIn {
     SI_UQ = 0, SI_US, SI_UT, SI_WQ, SI_WS, SI_WT,
     SI_BQ, SI_BS, SI_BT, SI_JQ, SI_JS, SI_JT,
-    SI_DC, SI_DX, SI_COUNT
+    SI_DC, SI_DX,
+    SI_CC, SI_XC,  // C and xi themselves in the transform precision (mixed forms)
+    SI_COUNT
 };
 
 // Analysis output (coll-layout) buffer ids.
@@ -77,6 +81,7 @@ struct Ctx {
     int device = 0;
     int L, M, Mp, N, Th, Ph, H, Th2, nslot;
     double dt;
+    bool mixed = false;            // FP32 transforms and solves (LSD_MIXED_*)
     hipStream_t stream = nullptr;
 
     std::vector<int> hl, hm;       // per harmonic l, m_
@@ -97,6 +102,8 @@ struct Ctx {
     double* d_sin = nullptr;       // [Th]
     double* d_tab = nullptr;       // 5 synthesis tables [5][H][Th2]
     double* d_atab = nullptr;      // 3 analysis tables [3][Th2][H]
+    float* d_tab32 = nullptr;      // FP32 copies (mixed forms)
+    float* d_atab32 = nullptr;
     // device: radial mesh
     double* d_r = nullptr;         // [7][N]  r^p, p = -3..3
     double* d_dr1 = nullptr;       // [N][7]
@@ -112,21 +119,40 @@ struct Ctx {
     double* d_gre = nullptr;       // [L][2][N]
     double* d_invg = nullptr;      // [L][4]
     int* d_info = nullptr;         // [NLU][L] dgbtrf info
+    double* d_lut = nullptr;       // [NLU][N][10][L]  solve-side copies,
+    int* d_pivt = nullptr;         // [NLU][N][L]      degree fastest
+    double* d_yt = nullptr;        // [NY][N][7][L]
+    double* d_gret = nullptr;      // [2][N][L]
+    float* d_lut32 = nullptr;      // [NLU][N][10][L]  FP32 factors (mixed forms)
+    double* d_xband = nullptr;     // [NLU][L][N][7]   X before factorisation
+    double* d_xt = nullptr;        // [NLU][N][7][L]   same, degree fastest
 
     // device: fields
     double2* d_f[LSD_NUM_COLL] = {};   // state + current N
     double2* d_hist[6] = {};           // saved N^n
     double2* d_spec_in = nullptr;      // [SI_COUNT][H][N]
     double2* d_spec_out = nullptr;     // [SO_COUNT][H][N]
-    double2* d_work = nullptr;         // solve scratch [6][H][N]
     double2* d_fourier = nullptr;      // [18][N][M][Th]
     double2* d_folded = nullptr;       // [8][N][M][2][Th2]
     double* d_phys = nullptr;          // [18][N][Ph][Th]
     double* d_nlphys = nullptr;        // [8][N][Ph][Th]
     double2* d_tw = nullptr;           // FFT twiddles exp(2 pi i k/Ph) [Ph]
+    float2* d_tw32 = nullptr;          // same, rounded to FP32
     double* d_partial = nullptr;       // corrector norm partials
     double* h_partial = nullptr;       // pinned copy
-    int n_partial_blocks = 0;
+    int n_partial_blocks = 0;          // solve blocks per field
+    int solve_hb = 0;                  // harmonics per solve block
+
+    // captured step graphs: [evaluate, predictor, evaluate, corrector] and
+    // [evaluate, corrector], each ending with the copy of the norm partials
+    bool use_graph = true;
+    hipGraphExec_t g_first = nullptr, g_more = nullptr;
+    int g_prof_class = -1;                 // class timed by events inside the graphs
+    int cap = -1;                          // graph being captured (0 first, 1 more)
+    std::vector<hipEvent_t> g_ev[2];       // (start, stop) pairs recorded by each graph
+    double g_work[2] = {0.0, 0.0};         // work of the timed class per replay
+    double prof_acc_ms = 0.0, prof_acc_work = 0.0;
+    int prof_acc_launches = 0;
 
     // per-launch timing of one kernel class (lsd_profile_begin/end)
     int prof_class = -1;
@@ -151,13 +177,14 @@ struct Ctx {
 struct ProfScope {
     Ctx& c;
     bool on;
-    int idx;
+    hipEvent_t stop = nullptr;
     ProfScope(Ctx& ctx, int cls, double work);
     ~ProfScope();
 };
 
 // precompute.hip
 int precompute_tables(Ctx& c);
+int tables_to_f32(Ctx& c);
 int precompute_mesh(Ctx& c);
 int build_operators(Ctx& c);
 
@@ -167,16 +194,23 @@ int launch_synthesis(Ctx& c, int kind, const double2* in0, const double2* in1, d
 int launch_analysis(Ctx& c, int kind, const double2* g0, const double2* g1, const double2* g2,
                      double2* o0, double2* o1, double2* o2);
 
+int launch_synthesis_p(Ctx& c, int kind, const void* in0, const void* in1, void* out0, void* out1, bool f32);
+int launch_analysis_p(Ctx& c, int kind, const void* g0, const void* g1, const void* g2, void* o0, void* o1,
+                      void* o2, bool f32);
+
+// physfuse.hip
+bool physfuse_supported(const Ctx& c);
+int launch_physfuse(Ctx& c, bool f32);
+
 // fft.hip
 int fft_setup(Ctx& c);
 int launch_fft_c2r(Ctx& c, const double2* fourier, double* phys, int nfield);
 int launch_fft_r2c_fold(Ctx& c, const double* phys, double2* folded, int nfield);
 
 // spectral.hip
-int launch_vec_prep(Ctx& c, const double2* T, const double2* P, double2* q, double2* s, double2* t,
-                    double2* oq, double2* os, double2* ot);
-int launch_scal_prep(Ctx& c, const double2* C, double2* dC);
-int launch_spec_post(Ctx& c);
+int launch_vec_prep(Ctx& c, const double2* T, const double2* P, void* const* out6, bool f32);
+int launch_scal_prep(Ctx& c, const double2* C, void* dC, void* Cc, bool f32);
+int launch_spec_post(Ctx& c, bool f32);
 int launch_layout_spec2coll(Ctx& c, const double* re, const double* im, double2* out);
 int launch_layout_coll2spec(Ctx& c, const double2* in, double* re, double* im);
 
@@ -184,7 +218,11 @@ int launch_layout_coll2spec(Ctx& c, const double2* in, double* re, double* im);
 int launch_nonlinear(Ctx& c);
 
 // solve.hip
+int solve_block_harmonics(int N, bool mixed);
+int launch_ops_layout(Ctx& c);
 int launch_predictor(Ctx& c);
 int launch_corrector(Ctx& c, double* dterr);
+int launch_corrector_async(Ctx& c);
+double corrector_norms(const Ctx& c);
 
 }  // namespace lsd

@@ -1,0 +1,261 @@
+// Physical-space part of one nonlinear evaluation in a single pass:
+// inverse phi FFTs of the 18 synthesised fields (tra_s2p,
+// modules/transform.fftw3.F90:280-364), the pointwise Boussinesq products
+// (modules/nonlinear.F90:71-146, 266-272, 335-341, 420-430), and the
+// forward FFTs with the even/odd fold of tra_p2s / tra_rtp2qst
+// (:379-466, 862-864) -- without the physical fields ever reaching HBM.
+//
+// A workgroup owns one shell and one theta row pair (j, Th-1-j).  The two
+// real rows of a field are packed as one complex sequence (row j real, row
+// Th-1-j imaginary), so the inverse transform, the products and the forward
+// transform of both rows share every FFT, and the forward output separates
+// directly into the folded north+south / north-south buckets the analysis
+// consumes.  Inputs go through LDS in batches of up to six fields (u and
+// curl u; B and curl B; grad C and grad xi); each thread keeps the values
+// of its own phi points in registers between batches.
+//
+// Fourier data are stored [f][n][m][j]: the 16-byte value of row j is one
+// piece of a 128-byte line shared by 8 consecutive row pairs.  Blocks are
+// numbered so that those 8 row pairs are consecutive workgroups on the same
+// XCD (workgroups go round-robin over the 8 XCDs), which makes them share
+// each line through that XCD's L2.
+#include "fft_lds.h"
+#include "lsd_internal.h"
+
+namespace lsd {
+
+template <typename R>
+struct PhysArgs {
+    FftPlan pl;
+    int N, Th, Th2, M, ngroups, gpr;  // gpr: groups of 8 row pairs per shell
+    R cor, cmag, qc, qx, scale;
+    int cod, comp, mag;
+    const double* gp;   // [Th] cos(theta)
+    const double* sn;   // [Th] sin(theta)
+    const typename Cx<R>::t* fourier;  // [18][N][M][Th]
+    typename Cx<R>::t* folded;         // [8][N][M][2][Th2]
+    const typename Cx<R>::t* tw;
+};
+
+// Packs nb fields of row pair (jp, jm) -- f0, f0+1, f0+2, then f1, f1+1,
+// f1+2 -- into Hermitian-extended complex sequences (row jp real part, row
+// jm imaginary part); the DC imaginary parts are dropped as a c2r transform
+// does.
+template <typename V>
+__device__ void pack_inputs(V* Z, const V* __restrict__ fourier, int f0, int f1, int nb, int sh, int N, int M,
+                            int Th, int Ph, int jp, int jm) {
+    for (int idx = threadIdx.x; idx < nb * M; idx += FFT_THREADS) {
+        const int b = idx / M, k = idx - b * M;
+        const int fid = b < 3 ? f0 + b : f1 + b - 3;
+        const V* X = fourier + (((size_t)fid * N + sh) * M + k) * Th;
+        const V A = X[jp], B = X[jm];
+        V* z = Z + (size_t)b * Ph;
+        if (k == 0) {
+            z[0] = cx<V>(A.x, B.x);
+        } else {
+            z[k] = cx<V>(A.x - B.y, A.y + B.x);
+            z[Ph - k] = cx<V>(A.x + B.y, B.x - A.y);
+        }
+    }
+    const int W = Ph - 2 * M + 1;  // zero band k = M .. Ph-M
+    for (int idx = threadIdx.x; idx < nb * W; idx += FFT_THREADS) {
+        const int b = idx / W, k = M + (idx - b * W);
+        Z[(size_t)b * Ph + k] = cx<V>(0, 0);
+    }
+    __syncthreads();
+}
+
+template <typename R, int PP>
+__global__ void __launch_bounds__(FFT_THREADS) k_physfuse(PhysArgs<R> a) {
+    typedef typename Cx<R>::t V;
+    extern __shared__ unsigned char lds_raw[];
+    V* Z = reinterpret_cast<V*>(lds_raw);
+    const int b = blockIdx.x, xcd = b & 7, q = b >> 3;
+    const int G = (q >> 3) * 8 + xcd, member = q & 7;
+    if (G >= a.ngroups) return;
+    const int sh = G / a.gpr, jg = G - sh * a.gpr;
+    const int jp = jg * 8 + member;
+    if (jp >= a.Th2) return;
+    const int jm = a.Th - 1 - jp;
+    const int Ph = a.pl.n, N = a.N, M = a.M, Th = a.Th;
+    const int t = threadIdx.x;
+    const R sn = (R)a.sn[jp];
+    const R gpr[2] = {(R)a.gp[jp], (R)a.gp[jm]};
+
+    R u[PP][2][3], nu[PP][2][3], e[PP][2][3], nc[PP][2], nx[PP][2];
+
+    // u and curl u: advection and Coriolis
+    {
+        pack_inputs(Z, a.fourier, LSD_P_VEL_R, LSD_P_VCURL_R, 6, sh, N, M, Th, Ph, jp, jm);
+        fft_lds<V>(Z, a.pl, 6, a.tw, (R)1);
+#pragma unroll
+        for (int p = 0; p < PP; p++) {
+            const int x = t + p * FFT_THREADS;
+            if (x >= Ph) continue;
+            V z[6];
+#pragma unroll
+            for (int f = 0; f < 6; f++) z[f] = Z[(size_t)f * Ph + x];
+#pragma unroll
+            for (int r = 0; r < 2; r++) {
+                const R ur = r ? z[0].y : z[0].x, ut = r ? z[1].y : z[1].x, up = r ? z[2].y : z[2].x;
+                const R wr = r ? z[3].y : z[3].x, wt = r ? z[4].y : z[4].x, wp = r ? z[5].y : z[5].x;
+                R vr = ut * wp - up * wt;
+                R vt = up * wr - ur * wp;
+                R vp = ur * wt - ut * wr;
+                const R lgp = gpr[r];
+                vr = vr + a.cor * sn * up;
+                vt = vt + a.cor * lgp * up;
+                vp = vp - a.cor * (lgp * ut + sn * ur);
+                u[p][r][0] = ur;
+                u[p][r][1] = ut;
+                u[p][r][2] = up;
+                nu[p][r][0] = vr;
+                nu[p][r][1] = vt;
+                nu[p][r][2] = vp;
+            }
+        }
+        __syncthreads();
+    }
+    // B and curl B: Lorentz force and induction E = u x B
+    if (a.mag) {
+        pack_inputs(Z, a.fourier, LSD_P_MAG_R, LSD_P_MCURL_R, 6, sh, N, M, Th, Ph, jp, jm);
+        fft_lds<V>(Z, a.pl, 6, a.tw, (R)1);
+#pragma unroll
+        for (int p = 0; p < PP; p++) {
+            const int x = t + p * FFT_THREADS;
+            if (x >= Ph) continue;
+            V z[6];
+#pragma unroll
+            for (int f = 0; f < 6; f++) z[f] = Z[(size_t)f * Ph + x];
+#pragma unroll
+            for (int r = 0; r < 2; r++) {
+                const R br = r ? z[0].y : z[0].x, bt = r ? z[1].y : z[1].x, bp = r ? z[2].y : z[2].x;
+                const R jr = r ? z[3].y : z[3].x, jt = r ? z[4].y : z[4].x, jpv = r ? z[5].y : z[5].x;
+                const R ur = u[p][r][0], ut = u[p][r][1], up = u[p][r][2];
+                nu[p][r][0] = nu[p][r][0] + a.cmag * (jt * bp - jpv * bt);
+                nu[p][r][1] = nu[p][r][1] + a.cmag * (jpv * br - jr * bp);
+                nu[p][r][2] = nu[p][r][2] + a.cmag * (jr * bt - jt * br);
+                e[p][r][0] = ut * bp - up * bt;
+                e[p][r][1] = up * br - ur * bp;
+                e[p][r][2] = ur * bt - ut * br;
+            }
+        }
+        __syncthreads();
+    }
+    // grad C and grad xi: advection of codensity and composition
+    if (a.cod || a.comp) {
+        const int nb = 3 * (a.cod + a.comp);
+        pack_inputs(Z, a.fourier, a.cod ? LSD_P_COD_GR : LSD_P_COMP_GR, LSD_P_COMP_GR, nb, sh, N, M, Th, Ph, jp, jm);
+        fft_lds<V>(Z, a.pl, nb, a.tw, (R)1);
+        const int oc = 0, ox = a.cod ? 3 : 0;
+#pragma unroll
+        for (int p = 0; p < PP; p++) {
+            const int x = t + p * FFT_THREADS;
+            if (x >= Ph) continue;
+#pragma unroll
+            for (int r = 0; r < 2; r++) {
+                const R ur = u[p][r][0], ut = u[p][r][1], up = u[p][r][2];
+                if (a.cod) {
+                    const V g0 = Z[(size_t)(oc + 0) * Ph + x], g1 = Z[(size_t)(oc + 1) * Ph + x],
+                            g2 = Z[(size_t)(oc + 2) * Ph + x];
+                    nc[p][r] = -a.qc * ur * (r ? g0.y : g0.x) - a.qc * ut * (r ? g1.y : g1.x) -
+                               a.qc * up * (r ? g2.y : g2.x);
+                }
+                if (a.comp) {
+                    const V g0 = Z[(size_t)(ox + 0) * Ph + x], g1 = Z[(size_t)(ox + 1) * Ph + x],
+                            g2 = Z[(size_t)(ox + 2) * Ph + x];
+                    nx[p][r] = -a.qx * ur * (r ? g0.y : g0.x) - a.qx * ut * (r ? g1.y : g1.x) -
+                               a.qx * up * (r ? g2.y : g2.x);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // outputs, scaled by 1/Ph before the forward transform as tra_p2s does
+    // sequences: N_u r, theta, phi; then codensity, composition, E r, theta, phi as enabled
+    const int ncx = a.cod + a.comp;
+    const int nout = 3 + ncx + (a.mag ? 3 : 0);
+    const R sc = a.scale;
+#pragma unroll
+    for (int p = 0; p < PP; p++) {
+        const int x = t + p * FFT_THREADS;
+        if (x >= Ph) continue;
+        int s = 0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) Z[(size_t)(s++) * Ph + x] = cx<V>(nu[p][0][c] * sc, nu[p][1][c] * sc);
+        if (a.cod) Z[(size_t)(s++) * Ph + x] = cx<V>(nc[p][0] * sc, nc[p][1] * sc);
+        if (a.comp) Z[(size_t)(s++) * Ph + x] = cx<V>(nx[p][0] * sc, nx[p][1] * sc);
+        if (a.mag) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) Z[(size_t)(s++) * Ph + x] = cx<V>(e[p][0][c] * sc, e[p][1][c] * sc);
+        }
+    }
+    __syncthreads();
+    fft_lds<V>(Z, a.pl, nout, a.tw, (R)-1);
+    // separate the two rows and fold: G0 = X_j + X_mirror, G1 = X_j - X_mirror
+    const size_t fstride = (size_t)N * M * 2 * a.Th2;
+    for (int idx = t; idx < nout * M * 2; idx += FFT_THREADS) {
+        const int s = idx / (2 * M), rem = idx - s * 2 * M;
+        const int k = rem >> 1, par = rem & 1;
+        const V zk = Z[(size_t)s * Ph + k];
+        V zc = Z[(size_t)s * Ph + (k ? Ph - k : 0)];
+        zc.y = -zc.y;
+        const V A = cx<V>((R)0.5 * (zk.x + zc.x), (R)0.5 * (zk.y + zc.y));
+        const V D = csubv(zk, zc);
+        const V B = cx<V>((R)0.5 * D.y, (R)-0.5 * D.x);
+        const V o = par ? csubv(A, B) : caddv(A, B);
+        const int oid = s < 3 ? NO_UR + s : (s < 3 + ncx ? ((s == 3 && a.cod) ? NO_C : NO_X) : NO_ER + (s - 3 - ncx));
+        a.folded[oid * fstride + (((size_t)sh * M + k) * 2 + par) * a.Th2 + jp] = o;
+    }
+}
+
+bool physfuse_supported(const Ctx& c) { return c.Ph <= 512 && c.Th % 2 == 0; }
+
+template <typename R>
+static int physfuse_launch(Ctx& c, const void* fourier, void* folded, const void* tw) {
+    typedef typename Cx<R>::t V;
+    const lsd_config& k = c.cfg;
+    PhysArgs<R> a;
+    a.pl = make_fft_plan(c.Ph);
+    a.N = c.N;
+    a.Th = c.Th;
+    a.Th2 = c.Th2;
+    a.M = c.M;
+    a.gpr = (c.Th2 + 7) / 8;
+    a.ngroups = c.N * a.gpr;
+    a.cor = (R)(2.0 * k.Pm / k.E);
+    a.cmag = (R)(2.0 * k.Pm / k.E);
+    a.qc = (R)(k.Pr / k.Pm);
+    a.qx = (R)(k.Sc / k.Pm);
+    a.scale = (R)(1.0 / (double)c.Ph);
+    a.cod = k.cod;
+    a.comp = k.comp;
+    a.mag = k.mag;
+    a.gp = c.d_gp;
+    a.sn = c.d_sin;
+    a.fourier = reinterpret_cast<const V*>(fourier);
+    a.folded = reinterpret_cast<V*>(folded);
+    a.tw = reinterpret_cast<const V*>(tw);
+    const int nblk = ((a.ngroups + 7) / 8) * 64;
+    const size_t lds = sizeof(V) * 8 * c.Ph;
+    const int nin = 6 + (k.mag ? 6 : 0) + (k.cod ? 3 : 0) + (k.comp ? 3 : 0);
+    const int nout = 3 + (k.mag ? 3 : 0) + (k.cod ? 1 : 0) + (k.comp ? 1 : 0);
+    // algorithmic bytes: Fourier fields in, folded fields out
+    ProfScope ps(c, LSD_K_PHYS,
+                 (double)sizeof(V) * c.N * c.M * ((double)nin * c.Th + (double)nout * 2 * c.Th2));
+    if (c.Ph <= FFT_THREADS) hipLaunchKernelGGL((k_physfuse<R, 1>), dim3(nblk), dim3(FFT_THREADS), lds, c.stream, a);
+    else hipLaunchKernelGGL((k_physfuse<R, 2>), dim3(nblk), dim3(FFT_THREADS), lds, c.stream, a);
+    LSD_HIP(hipGetLastError());
+    return LSD_OK;
+}
+
+int launch_physfuse(Ctx& c, bool f32) {
+    if (!physfuse_supported(c)) {
+        set_error("fused physical-space pass needs Ph <= 512");
+        return LSD_E_CONFIG;
+    }
+    if (f32) return physfuse_launch<float>(c, c.d_fourier, c.d_folded, c.d_tw32);
+    return physfuse_launch<double>(c, c.d_fourier, c.d_folded, c.d_tw);
+}
+
+}  // namespace lsd

@@ -474,7 +474,7 @@ struct OpParams {
 __global__ void k_operators(int L, int N, OpParams op, const double* __restrict__ r,
                             const double* __restrict__ radlap, const double* __restrict__ dr1,
                             double* __restrict__ lu, int* __restrict__ piv, double* __restrict__ ym,
-                            int* __restrict__ info) {
+                            int* __restrict__ info, double* __restrict__ xband) {
     int l = blockIdx.x * blockDim.x + threadIdx.x;
     int kind = blockIdx.y;
     if (l >= L) return;
@@ -526,6 +526,10 @@ __global__ void k_operators(int L, int N, OpParams op, const double* __restrict_
     }
 #undef LUA
 #undef DR1
+    // keep X itself (7 bands per column) for residuals of the mixed forms
+    double* xb = xband + ((size_t)kind * L + l) * N * 7;
+    for (int j = 0; j < N; j++)
+        for (int q = 0; q < 7; q++) xb[j * 7 + q] = ab[j * LSD_LDAB + LSD_KL + q];
     info[kind * L + l] = gbtrf_d(N, ab, pv);
     // explicit operator Y
     int yk = kind < 2 ? kind : (kind == 2 ? -1 : kind - 1);
@@ -579,12 +583,29 @@ __global__ void k_green(int L, int N, const double* __restrict__ lu, const int* 
     invg[l * 4 + 3] = g[8];
 }
 
+__global__ void k_to_f32(size_t n, const double* __restrict__ a, float* __restrict__ b) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
+// FP32 copies of the synthesis and analysis tables, rounded from the FP64
+// recursion, for the mixed forms.
+int tables_to_f32(Ctx& c) {
+    const size_t HT = (size_t)c.H * c.Th2;
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((5 * HT + 255) / 256)), dim3(256), 0, c.stream, 5 * HT, c.d_tab,
+                       c.d_tab32);
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((3 * HT + 255) / 256)), dim3(256), 0, c.stream, 3 * HT, c.d_atab,
+                       c.d_atab32);
+    LSD_HIP(hipGetLastError());
+    return LSD_OK;
+}
+
 int build_operators(Ctx& c) {
     OpParams op{c.dt, c.cfg.implicit, c.cfg.Pm, c.cfg.Pr, c.cfg.Sc};
     int L = c.L, N = c.N;
     dim3 grid((L + 63) / 64, NLU);
     hipLaunchKernelGGL(k_operators, grid, dim3(64), 0, c.stream, L, N, op, c.d_r, c.d_radlap, c.d_dr1,
-                       c.d_lu, c.d_piv, c.d_y, c.d_info);
+                       c.d_lu, c.d_piv, c.d_y, c.d_info, c.d_xband);
     LSD_HIP(hipGetLastError());
     double* scratch = nullptr;
     int* ginfo = nullptr;
@@ -611,7 +632,7 @@ int build_operators(Ctx& c) {
             set_error("matrix inversion error 1");
             return LSD_E_SINGULAR;
         }
-    return LSD_OK;
+    return launch_ops_layout(c);
 }
 
 }  // namespace lsd

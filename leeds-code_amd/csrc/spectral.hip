@@ -23,13 +23,40 @@ __device__ __forceinline__ double2 bandrow(const double* __restrict__ band, cons
     return acc;
 }
 
+// Spectral inputs of the transforms are stored in the transform precision
+// (double2, or float2 for the mixed forms); the stencils always run in FP64.
+__device__ __forceinline__ void put(double2* p, double2 v) { *p = v; }
+__device__ __forceinline__ void put(float2* p, double2 v) { *p = make_float2((float)v.x, (float)v.y); }
+__device__ __forceinline__ double2 get(const double2* p) { return *p; }
+__device__ __forceinline__ double2 get(const float2* p) {
+    float2 v = *p;
+    return make_double2((double)v.x, (double)v.y);
+}
+
+// bandrow over a field stored in either precision.
+template <typename TI>
+__device__ __forceinline__ double2 bandrow_t(const double* __restrict__ band, const TI* __restrict__ x, int n,
+                                             int N) {
+    int lo = n - LSD_KL > 0 ? n - LSD_KL : 0;
+    int hi = n + LSD_KL < N - 1 ? n + LSD_KL : N - 1;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int j = lo; j <= hi; j++) {
+        double a = band[j * 7 + (LSD_KL + n - j)];
+        double2 xj = get(x + j);
+        acc.x = fma(a, xj.x, acc.x);
+        acc.y = fma(a, xj.y, acc.y);
+    }
+    return acc;
+}
+
 // var_coll_TorPol2qst_mag (variables.F90:565-605) followed by
 // var_coll_qstcurl (variables.F90:713-754).
+template <typename TO>
 __global__ void k_vec_prep(int H, int N, const int* __restrict__ hl, const double* __restrict__ r,
                            const double* __restrict__ r1dr, const double* __restrict__ ll,
-                           const double2* __restrict__ T, const double2* __restrict__ P, double2* __restrict__ q,
-                           double2* __restrict__ s, double2* __restrict__ t, double2* __restrict__ oq,
-                           double2* __restrict__ os, double2* __restrict__ ot, int L) {
+                           const double2* __restrict__ T, const double2* __restrict__ P, TO* __restrict__ q,
+                           TO* __restrict__ s, TO* __restrict__ t, TO* __restrict__ oq,
+                           TO* __restrict__ os, TO* __restrict__ ot, int L) {
     size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (size_t)H * N) return;
     int h = (int)(idx / N), n = (int)(idx - (size_t)h * N);
@@ -59,22 +86,25 @@ __global__ void k_vec_prep(int H, int N, const int* __restrict__ hl, const doubl
     double2 tn = make_double2(-Tc[n].x * rt, -Tc[n].y * rt);
     double c = rt * rinv[n];
     size_t o = (size_t)h * N + n;
-    q[o] = qn;
-    s[o] = sn;
-    t[o] = tn;
-    ot[o] = make_double2(otn.x - c * qn.x, otn.y - c * qn.y);
-    os[o] = make_double2(-osn.x, -osn.y);
+    put(q + o, qn);
+    put(s + o, sn);
+    put(t + o, tn);
+    put(ot + o, make_double2(otn.x - c * qn.x, otn.y - c * qn.y));
+    put(os + o, make_double2(-osn.x, -osn.y));
     double d = -rt * rinv[n];
-    oq[o] = make_double2(tn.x * d, tn.y * d);
+    put(oq + o, make_double2(tn.x * d, tn.y * d));
 }
 
-// dC/dr by the first-derivative stencil (codensity.F90:207).
+// dC/dr by the first-derivative stencil (codensity.F90:207); Cc, when
+// given, receives C itself in the transform precision.
+template <typename TO>
 __global__ void k_scal_prep(int H, int N, const double* __restrict__ dr1, const double2* __restrict__ C,
-                            double2* __restrict__ dC) {
+                            TO* __restrict__ dC, TO* __restrict__ Cc) {
     size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (size_t)H * N) return;
     int h = (int)(idx / N), n = (int)(idx - (size_t)h * N);
-    dC[idx] = bandrow(dr1, C + (size_t)h * N, n, N);
+    put(dC + idx, bandrow(dr1, C + (size_t)h * N, n, N));
+    if (Cc) put(Cc + idx, C[idx]);
 }
 
 struct PostParams {
@@ -86,9 +116,10 @@ struct PostParams {
 // Nonlinear terms in coll form (nonlinear.F90:148-197, 277-288, 341-348,
 // 432-471): buoyancy on q, r/(l(l+1)) r.curl and r.curl.curl projections,
 // l = 0 zeroed, sources on the l = 0 scalar modes.
+template <typename TI>
 __global__ void k_spec_post(int H, int N, int L, const int* __restrict__ hl, const double* __restrict__ r,
                             const double* __restrict__ r1dr, const double* __restrict__ ll, PostParams pp,
-                            const double2* __restrict__ so, const double2* __restrict__ Cst,
+                            const TI* __restrict__ so, const double2* __restrict__ Cst,
                             const double2* __restrict__ Xst, double2* __restrict__ NuT, double2* __restrict__ NuP,
                             double2* __restrict__ NC, double2* __restrict__ NX, double2* __restrict__ NbT,
                             double2* __restrict__ NbP) {
@@ -100,19 +131,20 @@ __global__ void k_spec_post(int H, int N, int L, const int* __restrict__ hl, con
     const double* rinv = r + 2 * N;
     double rn = r[4 * N + n];
     double rt11 = ll[3 * L + l];
-    const double2* uq = so + SO_UQ * HN;
-    const double2* us = so + SO_US * HN + (size_t)h * N;
-    const double2* ut = so + SO_UT * HN;
+    const TI* uq = so + SO_UQ * HN;
+    const TI* us = so + SO_US * HN + (size_t)h * N;
+    const TI* ut = so + SO_UT * HN;
     // buoyancy (nonlinear.F90:158-185)
-    double2 cq = uq[idx];
+    double2 cq = get(uq + idx);
     double d2 = pp.Ra * rn * pp.Pm2 / pp.Pr;
     cq.x = cq.x + d2 * Cst[idx].x;
     cq.y = cq.y + d2 * Cst[idx].y;
     d2 = pp.Ra_comp * rn * pp.Pm2 / pp.Sc;
     cq.x = cq.x + d2 * Xst[idx].x;
     cq.y = cq.y + d2 * Xst[idx].y;
-    double2 nt = make_double2(-ut[idx].x * rt11, -ut[idx].y * rt11);
-    double2 v = bandrow(r1dr, us, n, N);
+    const double2 utv = get(ut + idx);
+    double2 nt = make_double2(-utv.x * rt11, -utv.y * rt11);
+    double2 v = bandrow_t(r1dr, us, n, N);
     v.x = v.x * rt11;
     v.y = v.y * rt11;
     double2 np = make_double2(cq.x * rinv[n] - v.x, cq.y * rinv[n] - v.y);
@@ -120,24 +152,25 @@ __global__ void k_spec_post(int H, int N, int L, const int* __restrict__ hl, con
     NuT[idx] = nt;
     NuP[idx] = np;
     if (pp.cod) {
-        double2 c = so[SO_C * HN + idx];
+        double2 c = get(so + SO_C * HN + idx);
         if (h == 0) c.x = c.x + pp.src_c;
         NC[idx] = c;
     }
     if (pp.comp) {
-        double2 c = so[SO_X * HN + idx];
+        double2 c = get(so + SO_X * HN + idx);
         if (h == 0) c.x = c.x + pp.src_x;
         NX[idx] = c;
     }
     if (pp.mag) {
-        const double2* eq = so + SO_EQ * HN;
-        const double2* es = so + SO_ES * HN + (size_t)h * N;
-        const double2* et = so + SO_ET * HN;
-        double2 bp = make_double2(-et[idx].x * rt11, -et[idx].y * rt11);
-        double2 w = bandrow(r1dr, es, n, N);
+        const TI* eq = so + SO_EQ * HN;
+        const TI* es = so + SO_ES * HN + (size_t)h * N;
+        const TI* et = so + SO_ET * HN;
+        const double2 etv = get(et + idx), eqv = get(eq + idx);
+        double2 bp = make_double2(-etv.x * rt11, -etv.y * rt11);
+        double2 w = bandrow_t(r1dr, es, n, N);
         w.x = w.x * rt11;
         w.y = w.y * rt11;
-        double2 bt = make_double2(eq[idx].x * rinv[n] - w.x, eq[idx].y * rinv[n] - w.y);
+        double2 bt = make_double2(eqv.x * rinv[n] - w.x, eqv.y * rinv[n] - w.y);
         if (h == 0) bp = bt = make_double2(0.0, 0.0);
         NbP[idx] = bp;
         NbT[idx] = bt;
@@ -165,29 +198,48 @@ __global__ void k_coll2spec(int H, int N, const double2* __restrict__ in, double
 
 static dim3 grid1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
-int launch_vec_prep(Ctx& c, const double2* T, const double2* P, double2* q, double2* s, double2* t, double2* oq,
-                    double2* os, double2* ot) {
-    ProfScope ps(c, LSD_K_SPEC, 8.0 * 16.0 * c.coll_elems());
-    hipLaunchKernelGGL(k_vec_prep, grid1(c.coll_elems()), dim3(256), 0, c.stream, c.H, c.N, c.d_hl, c.d_r,
-                       c.d_r1dr, c.d_ll, T, P, q, s, t, oq, os, ot, c.L);
+template <typename TO>
+static int vec_prep_t(Ctx& c, const double2* T, const double2* P, TO* const* o) {
+    ProfScope ps(c, LSD_K_SPEC, (2.0 * 16.0 + 6.0 * sizeof(TO)) * c.coll_elems());
+    hipLaunchKernelGGL(k_vec_prep<TO>, grid1(c.coll_elems()), dim3(256), 0, c.stream, c.H, c.N, c.d_hl, c.d_r,
+                       c.d_r1dr, c.d_ll, T, P, o[0], o[1], o[2], o[3], o[4], o[5], c.L);
     LSD_HIP(hipGetLastError());
     return LSD_OK;
 }
 
-int launch_scal_prep(Ctx& c, const double2* C, double2* dC) {
-    ProfScope ps(c, LSD_K_SPEC, 2.0 * 16.0 * c.coll_elems());
-    hipLaunchKernelGGL(k_scal_prep, grid1(c.coll_elems()), dim3(256), 0, c.stream, c.H, c.N, c.d_dr1, C, dC);
+// o[0..5] = q, s, t, curl q, curl s, curl t in the layout of spec_in slots
+int launch_vec_prep(Ctx& c, const double2* T, const double2* P, void* const* o, bool f32) {
+    if (f32) {
+        float2* p[6];
+        for (int k = 0; k < 6; k++) p[k] = reinterpret_cast<float2*>(o[k]);
+        return vec_prep_t<float2>(c, T, P, p);
+    }
+    double2* p[6];
+    for (int k = 0; k < 6; k++) p[k] = reinterpret_cast<double2*>(o[k]);
+    return vec_prep_t<double2>(c, T, P, p);
+}
+
+int launch_scal_prep(Ctx& c, const double2* C, void* dC, void* Cc, bool f32) {
+    const size_t out = f32 ? sizeof(float2) : sizeof(double2);
+    ProfScope ps(c, LSD_K_SPEC, (16.0 + out * (Cc ? 2.0 : 1.0)) * c.coll_elems());
+    if (f32)
+        hipLaunchKernelGGL(k_scal_prep<float2>, grid1(c.coll_elems()), dim3(256), 0, c.stream, c.H, c.N, c.d_dr1, C,
+                           reinterpret_cast<float2*>(dC), reinterpret_cast<float2*>(Cc));
+    else
+        hipLaunchKernelGGL(k_scal_prep<double2>, grid1(c.coll_elems()), dim3(256), 0, c.stream, c.H, c.N, c.d_dr1,
+                           C, reinterpret_cast<double2*>(dC), reinterpret_cast<double2*>(Cc));
     LSD_HIP(hipGetLastError());
     return LSD_OK;
 }
 
-int launch_spec_post(Ctx& c) {
+int launch_spec_post(Ctx& c, bool f32) {
     PostParams pp;
     const lsd_config& k = c.cfg;
-    // reads: 3 u terms + C + X (+ C, X, E terms), writes the N fields
-    int nrd = 5 + (k.cod ? 1 : 0) + (k.comp ? 1 : 0) + (k.mag ? 3 : 0);
-    int nwr = 2 + (k.cod ? 1 : 0) + (k.comp ? 1 : 0) + (k.mag ? 2 : 0);
-    ProfScope ps(c, LSD_K_SPEC, (double)(nrd + nwr) * 16.0 * c.coll_elems());
+    // reads: the analysis outputs and the C, xi state; writes the N fields
+    const int nso = 3 + (k.cod ? 1 : 0) + (k.comp ? 1 : 0) + (k.mag ? 3 : 0);
+    const int nwr = 2 + (k.cod ? 1 : 0) + (k.comp ? 1 : 0) + (k.mag ? 2 : 0);
+    const double rb = f32 ? 8.0 : 16.0;
+    ProfScope ps(c, LSD_K_SPEC, (nso * rb + 2 * 16.0 + nwr * 16.0) * c.coll_elems());
     pp.Ra = k.Ra;
     pp.Ra_comp = k.Ra_comp;
     pp.Pm2 = k.Pm * k.Pm;
@@ -198,10 +250,16 @@ int launch_spec_post(Ctx& c) {
     pp.cod = k.cod;
     pp.comp = k.comp;
     pp.mag = k.mag;
-    hipLaunchKernelGGL(k_spec_post, grid1(c.coll_elems()), dim3(256), 0, c.stream, c.H, c.N, c.L, c.d_hl, c.d_r,
-                       c.d_r1dr, c.d_ll, pp, c.d_spec_out, c.d_f[LSD_COD], c.d_f[LSD_COMP], c.d_f[LSD_N_VEL_TOR],
-                       c.d_f[LSD_N_VEL_POL], c.d_f[LSD_N_COD], c.d_f[LSD_N_COMP], c.d_f[LSD_N_MAG_TOR],
-                       c.d_f[LSD_N_MAG_POL]);
+    if (f32)
+        hipLaunchKernelGGL(k_spec_post<float2>, grid1(c.coll_elems()), dim3(256), 0, c.stream, c.H, c.N, c.L, c.d_hl,
+                           c.d_r, c.d_r1dr, c.d_ll, pp, reinterpret_cast<const float2*>(c.d_spec_out),
+                           c.d_f[LSD_COD], c.d_f[LSD_COMP], c.d_f[LSD_N_VEL_TOR], c.d_f[LSD_N_VEL_POL],
+                           c.d_f[LSD_N_COD], c.d_f[LSD_N_COMP], c.d_f[LSD_N_MAG_TOR], c.d_f[LSD_N_MAG_POL]);
+    else
+        hipLaunchKernelGGL(k_spec_post<double2>, grid1(c.coll_elems()), dim3(256), 0, c.stream, c.H, c.N, c.L,
+                           c.d_hl, c.d_r, c.d_r1dr, c.d_ll, pp, c.d_spec_out, c.d_f[LSD_COD], c.d_f[LSD_COMP],
+                           c.d_f[LSD_N_VEL_TOR], c.d_f[LSD_N_VEL_POL], c.d_f[LSD_N_COD], c.d_f[LSD_N_COMP],
+                           c.d_f[LSD_N_MAG_TOR], c.d_f[LSD_N_MAG_POL]);
     LSD_HIP(hipGetLastError());
     return LSD_OK;
 }

@@ -1,0 +1,26 @@
+#!/bin/bash
+# Compile check of the Fortran drop-in module transform.hip.F90 against the
+# reference's own parameters / dyn_mpi / meshs / variables modules (serial
+# build, _Nth = _Nr = 1), compiled where they lie under /root/reference.
+# Nothing of the reference is copied: its templates are reached through
+# symlinks in the build folder, and all outputs (.mod, .o) stay in
+# leeds-code_amd/_build/dropin.  Needs the reference sources and amdflang,
+# so it runs in the build container only.
+set -euo pipefail
+REF=${REF:-/root/reference}
+FC=${FC_DROPIN:-/opt/rocm/bin/amdflang}
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT=$HERE/../_build/dropin
+rm -rf "$OUT"
+mkdir -p "$OUT/inc/sub"
+ln -s "$REF/Scripts/parallel.h_template" "$OUT/inc/parallel.h"
+ln -s "$REF/Scripts/parameters.F90_template" "$OUT/parameters.F90"
+cd "$OUT"
+FFLAGS="-cpp -c -O0 -I$OUT/inc/sub -I$OUT/inc"
+$FC $FFLAGS parameters.F90
+$FC $FFLAGS "$REF/modules/dyn_mpi.F90"
+$FC $FFLAGS "$REF/modules/meshs.F90"
+$FC $FFLAGS "$REF/modules/variables.F90"
+$FC $FFLAGS "$HERE/lsd_c.F90"
+$FC $FFLAGS "$HERE/transform.hip.F90"
+echo "transform.hip.F90 compiles against the reference's parameters and variables modules"

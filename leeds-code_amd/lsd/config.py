@@ -71,10 +71,11 @@ def ra_crit_estimate(E, rratio=0.35):
     return unified_ra(RA_MOD_CRIT_E1EM3 * (1e-3 / E) ** (1.0 / 3.0), E, rratio)
 
 
-def config1():
+def config1(precision=PLAIN_F64):
     """Plain method, Boussinesq MHD, no composition: E=1e-3, Ra_mod=100
     (d_Ra = 6.5e4), Pr=1, Pm=5, N=32, L=M=32, 48x96 grid, dt=1e-4."""
-    return make_config(L=32, M=32, N=32, Th=48, Ph=96, comp=0, Ra=6.5e4, Pm=5.0, E=1e-3, dt=1e-4)
+    return make_config(L=32, M=32, N=32, Th=48, Ph=96, comp=0, Ra=6.5e4, Pm=5.0, E=1e-3, dt=1e-4,
+                       precision=precision)
 
 
 def config3(precision=PLAIN_F64):
@@ -101,4 +102,3 @@ def ra_table():
 if __name__ == "__main__":
     for name, c in (("config1", config1()), ("config3", config3())):
         print(name, c.sizes(), "Ra=%.6g" % c.Ra)
-    print(math.pi)

@@ -27,12 +27,12 @@ EXPORTS = (
     "lsd_last_error", "lsd_config_sizes", "lsd_create", "lsd_destroy", "lsd_set_coll", "lsd_get_coll",
     "lsd_get_phys", "lsd_get_table", "lsd_get_mesh", "lsd_transform", "lsd_nonlinear", "lsd_predictor",
     "lsd_corrector", "lsd_step", "lsd_set_dt", "lsd_tra_spec2phys", "lsd_tra_phys2spec", "lsd_tra_grad",
-    "lsd_tra_qst2rtp", "lsd_tra_rtp2qst", "lsd_sync", "lsd_profile_begin", "lsd_profile_end",
+    "lsd_tra_qst2rtp", "lsd_tra_rtp2qst", "lsd_sync", "lsd_profile_begin", "lsd_profile_end", "lsd_set_graph",
 )
 
 # kernel classes for lsd_profile_begin
-K_SYNTH, K_ANAL, K_FFT_INV, K_FFT_FWD, K_NONLIN, K_SPEC, K_SOLVE = range(7)
-K_NAMES = ("synth", "anal", "fft_inv", "fft_fwd", "nonlin", "spec", "solve")
+K_SYNTH, K_ANAL, K_FFT_INV, K_FFT_FWD, K_NONLIN, K_SPEC, K_SOLVE, K_PHYS = range(8)
+K_NAMES = ("synth", "anal", "fft_inv", "fft_fwd", "nonlin", "spec", "solve", "phys")
 
 
 class LsdError(RuntimeError):
@@ -69,6 +69,7 @@ def load(path=LIB_PATH):
     lib.lsd_corrector.argtypes = [ctypes.c_void_p, _dp]
     lib.lsd_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(StepStats)]
     lib.lsd_set_dt.argtypes = [ctypes.c_void_p, ctypes.c_double]
+    lib.lsd_set_graph.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.lsd_tra_spec2phys.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
     lib.lsd_tra_phys2spec.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
     lib.lsd_tra_grad.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
@@ -199,6 +200,10 @@ class Context:
 
     def set_dt(self, dt):
         self._chk(self.lib.lsd_set_dt(self.h, dt), "lsd_set_dt")
+
+    def set_graph(self, on):
+        """lsd_step from captured HIP graphs (True, default) or eager launches."""
+        self._chk(self.lib.lsd_set_graph(self.h, 1 if on else 0), "lsd_set_graph")
 
     # --- drop-in transforms (spec re[n, h]; phys [n, k, j])
     def spec2phys(self, sre, sim):

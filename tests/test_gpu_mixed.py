@@ -1,0 +1,106 @@
+"""Mixed-precision timestep (LSD_MIXED_F32) against the plain FP64 path.
+
+The mixed form runs the Legendre contractions on FP32 MFMA, the phi FFTs
+and nonlinear products in FP32, and the banded solves on FP32 factors with
+one FP64-residual refinement sweep; the state, the radial stencils and the
+time-stepping stay FP64.  Its error is measured per field as
+sqrt(sum |coeff|^2) of the difference over all (l, m, r), relative to the
+same norm of the FP64 field, against the plain FP64 GPU path, which the
+parity tests pin to the CPU oracle (1 step at L = M = 170, N = 96; 100 steps
+at L = M = 32, N = 32).  Bound: 1e-6 after 1 and after 1000 steps at
+L = M = 170, N = 96.
+"""
+import numpy as np
+import pytest
+
+from lsd import config1, config3, config_small, seeded_state, MIXED_F32, PLAIN_F64, SEEDS, AMPS
+from lsd.gpu import Context
+from lsd.initial import FIELD_ID
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("uT", "uP", "C", "X", "BT", "BP")
+
+
+def field_errors(gm, gp, cfg):
+    out = {}
+    for name in FIELDS:
+        if name == "C" and not cfg.cod or name == "X" and not cfg.comp:
+            continue
+        a = gm.get_coll(FIELD_ID[name])
+        b = gp.get_coll(FIELD_ID[name])
+        num = np.sqrt(sum(((x - y) ** 2).sum() for x, y in zip(a, b)))
+        den = np.sqrt(sum((y ** 2).sum() for y in b))
+        out[name] = num / den if den > 0 else num
+    return out
+
+
+def pair(cfg_fn, seed, amp):
+    cp = cfg_fn(precision=PLAIN_F64)
+    cm = cfg_fn(precision=MIXED_F32)
+    st = seeded_state(cp, seed, amp)
+    gp, gm = Context(cp), Context(cm)
+    gp.set_state(st)
+    gm.set_state(st)
+    return cp, gp, gm
+
+
+@pytest.mark.parametrize("which", ["small", "config1"])
+def test_mixed_small_cases(which):
+    fn = config_small if which == "small" else config1
+    cfg, gp, gm = pair(fn, 5, 1e-3)
+    print("case", which, cfg.sizes(), flush=True)
+    sp, sm = gp.step(10), gm.step(10)
+    assert sp.evaluations == sm.evaluations
+    errs = field_errors(gm, gp, cfg)
+    print(which, {k: "%.2e" % v for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v < 1e-6, (k, v)
+    gp.close()
+    gm.close()
+
+
+def test_mixed_config3_1_and_1000_steps():
+    cfg, gp, gm = pair(config3, SEEDS[3], AMPS[3])
+    print("case L=M=170 N=96 mixed vs plain", flush=True)
+    gp.step(1)
+    gm.step(1)
+    e1 = field_errors(gm, gp, cfg)
+    print("after 1 step", {k: "%.2e" % v for k, v in e1.items()}, flush=True)
+    for k, v in e1.items():
+        assert v < 1e-6, (k, v)
+    for block in range(9):
+        sp = gp.step(111)
+        sm = gm.step(111)
+        assert sp.evaluations == sm.evaluations
+        print("after %d steps" % (2 + 111 * (block + 1) - 1), flush=True)
+    e = field_errors(gm, gp, cfg)
+    print("after 1000 steps", {k: "%.2e" % v for k, v in e.items()}, flush=True)
+    for k, v in e.items():
+        assert v < 1e-6, (k, v)
+    gp.close()
+    gm.close()
+
+
+@pytest.mark.parametrize("prec", [PLAIN_F64, MIXED_F32])
+def test_graph_replays_match_eager(prec):
+    """The captured-graph step against host-launched steps: every replay,
+    with eager work and synchronisations of the program's own between
+    replays, must give the eager result bit for bit."""
+    cfg = config1(precision=prec)
+    st = seeded_state(cfg, 3, 1e-3)
+    ge, gg = Context(cfg), Context(cfg)
+    ge.set_graph(False)
+    for g in (ge, gg):
+        g.set_state(st)
+    for rep in range(4):
+        se, sg = ge.step(2), gg.step(2)
+        assert se.evaluations == sg.evaluations and se.last_dterr == sg.last_dterr
+        for k in range(6):
+            a, b = gg.get_coll(k), ge.get_coll(k)
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), (rep, k)
+        # eager work on the graph context between replays
+        gg.transform()
+        gg.sync()
+    ge.close()
+    gg.close()
