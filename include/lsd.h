@@ -175,6 +175,23 @@ int lsd_corrector(lsd_ctx* ctx, double* dterr);
  * i_time_step_control = 2) accepts. */
 int lsd_step(lsd_ctx* ctx, int nsteps, lsd_step_stats* stats);
 
+/* Ensembles: nsteps timesteps of each of n independent contexts (any mix of
+ * truncations and parameters), with every member's step graph launched on
+ * its own stream before any is waited for, so the runs execute
+ * concurrently.  Each member follows the same control as lsd_step and
+ * gives the same results as stepping it alone.  stats[n] may be NULL.  No
+ * counterpart in the reference (one run per program there). */
+int lsd_step_many(lsd_ctx** ctxs, int n, int nsteps, lsd_step_stats* stats);
+
+/* Kinetic (which = 0) or magnetic (which = 1) energy of the current state
+ * and its l- and m-spectra, as io_write_vel_energy / io_write_mag_energy
+ * compute them (program/io.F90:1799-1884; var_coll_norm, var_coll_normlm,
+ * modules/variables.F90:901-957, 1267-1313): E = Efac * sum over the q, s,
+ * t parts of int |a|^2 dV, Efac = 1/2 (velocity) or Pm/E (magnetic,
+ * program/io.F90:238-247).  El[L] and Em[M] (may be NULL) receive the
+ * per-degree and per-order parts, times Efac. */
+int lsd_energy(lsd_ctx* ctx, int which, double* E, double* El, double* Em);
+
 /* Whether lsd_step replays the timestep from captured HIP graphs (default 1)
  * or launches every kernel from the host (0).  No counterpart in the
  * reference; results are identical either way. */

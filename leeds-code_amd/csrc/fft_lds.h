@@ -98,23 +98,32 @@ __device__ __forceinline__ void dft(V* v, decltype(V().x) sg) {
     }
 }
 
-// One Stockham stage over nf sequences of length n held in Z[p*n + i].
-template <int RAD, typename V>
+// Index of element i of the sequences in LDS: dense, or with one pad slot
+// per 8 complex values, which spreads the stride-8 writes of the first
+// Stockham stages over the banks.
+template <bool PAD>
+__device__ __forceinline__ int lidx(int i) {
+    return PAD ? i + (i >> 3) : i;
+}
+
+// One Stockham stage over nf sequences of length n held in Z[lidx(p*n + i)]
+// by a workgroup of THREADS threads, each taking up to EPT elements.
+template <int RAD, int THREADS, int EPT, bool PAD, typename V>
 __device__ void fft_stage(V* Z, int n, int nf, int Ns, const V* __restrict__ tw, decltype(V().x) sg) {
-    constexpr int BPT = (FFT_EPT + RAD - 1) / RAD;
+    constexpr int BPT = (EPT + RAD - 1) / RAD;
     const int nb = n / RAD;
     const int total = nf * nb;
     const int step = n / (Ns * RAD);
     V v[BPT][RAD];
 #pragma unroll
     for (int q = 0; q < BPT; q++) {
-        int b = threadIdx.x + q * FFT_THREADS;
+        int b = threadIdx.x + q * THREADS;
         if (b < total) {
             int p = b / nb, j = b - p * nb;
             int jm = j % Ns;
-            const V* src = Z + (size_t)p * n + j;
+            const int base = p * n + j;
 #pragma unroll
-            for (int r = 0; r < RAD; r++) v[q][r] = src[r * nb];
+            for (int r = 0; r < RAD; r++) v[q][r] = Z[lidx<PAD>(base + r * nb)];
             if (jm) {
 #pragma unroll
                 for (int r = 1; r < RAD; r++) {
@@ -129,29 +138,29 @@ __device__ void fft_stage(V* Z, int n, int nf, int Ns, const V* __restrict__ tw,
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < BPT; q++) {
-        int b = threadIdx.x + q * FFT_THREADS;
+        int b = threadIdx.x + q * THREADS;
         if (b < total) {
             int p = b / nb, j = b - p * nb;
             int jm = j % Ns;
-            V* dst = Z + (size_t)p * n + (j / Ns) * Ns * RAD + jm;
+            const int base = p * n + (j / Ns) * Ns * RAD + jm;
 #pragma unroll
-            for (int r = 0; r < RAD; r++) dst[r * Ns] = v[q][r];
+            for (int r = 0; r < RAD; r++) Z[lidx<PAD>(base + r * Ns)] = v[q][r];
         }
     }
     __syncthreads();
 }
 
-// Requires nf * n <= FFT_EPT * FFT_THREADS; ends synchronised.
-template <typename V>
+// Requires nf * n <= EPT * THREADS; ends synchronised.
+template <typename V, int THREADS = FFT_THREADS, int EPT = FFT_EPT, bool PAD = false>
 __device__ void fft_lds(V* Z, const FftPlan& pl, int nf, const V* __restrict__ tw, decltype(V().x) sg) {
     int Ns = 1;
     for (int s = 0; s < pl.nstage; s++) {
         int R = pl.radix[s];
         switch (R) {
-        case 8: fft_stage<8>(Z, pl.n, nf, Ns, tw, sg); break;
-        case 4: fft_stage<4>(Z, pl.n, nf, Ns, tw, sg); break;
-        case 2: fft_stage<2>(Z, pl.n, nf, Ns, tw, sg); break;
-        default: fft_stage<3>(Z, pl.n, nf, Ns, tw, sg); break;
+        case 8: fft_stage<8, THREADS, EPT, PAD>(Z, pl.n, nf, Ns, tw, sg); break;
+        case 4: fft_stage<4, THREADS, EPT, PAD>(Z, pl.n, nf, Ns, tw, sg); break;
+        case 2: fft_stage<2, THREADS, EPT, PAD>(Z, pl.n, nf, Ns, tw, sg); break;
+        default: fft_stage<3, THREADS, EPT, PAD>(Z, pl.n, nf, Ns, tw, sg); break;
         }
         Ns *= R;
     }

@@ -50,6 +50,26 @@ struct Mfma<float> {
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Block numbering shared by both contractions.  Work is grouped by table
+// block (slot, theta chunk); the ncc column chunks of a group read the same
+// table block, so they are given consecutive places in one XCD's dispatch
+// sequence (workgroups go round-robin over the 8 XCDs) and share it through
+// that XCD's L2.  Groups keep the longest-slot-first order within each XCD.
+struct BlockId {
+    int group, cc;
+    bool valid;
+};
+__device__ __forceinline__ BlockId decode_block(int ngroups, int ncc) {
+    const int b = blockIdx.x, xcd = b & 7, q = b >> 3;
+    const int gl = q / ncc;
+    BlockId r;
+    r.cc = q - gl * ncc;
+    r.group = gl * 8 + xcd;
+    r.valid = r.group < ngroups;
+    return r;
+}
+static unsigned grid_blocks(int ngroups, int ncc) { return (unsigned)(((ngroups + 7) / 8) * 8 * ncc); }
+
 template <typename T>
 __device__ __forceinline__ T partner(T v) {
     return __shfl_xor(v, 1);
@@ -58,6 +78,7 @@ __device__ __forceinline__ T partner(T v) {
 template <typename T>
 struct SynArgs {
     int N, Th, Th2, M, H;
+    int nth, ncc, ngroups;  // theta chunks, column chunks, nslot * nth
     const Slot* slots;
     const int* order;
     const T* tabA;   // [h][Th2]
@@ -76,11 +97,13 @@ __global__ void __launch_bounds__(64) k_syn(SynArgs<T> a) {
     typedef typename Mfma<T>::acc acc_t;
     constexpr int NOUT = KIND == 0 ? 1 : 2;
     constexpr int CB = KIND == 0 ? 2 : 1;
-    const int sl = uniform(a.order[blockIdx.x]);
+    const BlockId bid = decode_block(a.ngroups, a.ncc);
+    if (!bid.valid) return;
+    const int sl = uniform(a.order[bid.group / a.nth]);
     const Slot s = a.slots[sl];
     const int lane = threadIdx.x, l16 = lane & 15, q4 = lane >> 4;
-    const int j0 = blockIdx.y * 64;
-    const int c0 = blockIdx.z * 16 * CB;
+    const int j0 = (bid.group % a.nth) * 64;
+    const int c0 = bid.cc * 16 * CB;
     const int C2 = 2 * a.N;
     acc_t Y[NOUT][2][CB][4];
 #pragma unroll
@@ -224,7 +247,10 @@ static int syn_launch(Ctx& c, int kind, const T* tA, const T* tB, const T* in0, 
     a.out0 = out0;
     a.out1 = out1;
     const int cb = kind == SYN_SCALAR ? 2 : 1;
-    dim3 grid(c.nslot, (c.Th2 + 63) / 64, (2 * c.N + 16 * cb - 1) / (16 * cb));
+    a.nth = (c.Th2 + 63) / 64;
+    a.ncc = (2 * c.N + 16 * cb - 1) / (16 * cb);
+    a.ngroups = c.nslot * a.nth;
+    dim3 grid(grid_blocks(a.ngroups, a.ncc));
     if (kind == SYN_SCALAR) hipLaunchKernelGGL((k_syn<T, 0>), grid, dim3(64), 0, c.stream, a);
     else if (kind == SYN_VECTOR) hipLaunchKernelGGL((k_syn<T, 1>), grid, dim3(64), 0, c.stream, a);
     else hipLaunchKernelGGL((k_syn<T, 2>), grid, dim3(64), 0, c.stream, a);
@@ -260,6 +286,7 @@ int launch_synthesis(Ctx& c, int kind, const double2* in0, const double2* in1, d
 template <typename T>
 struct AnaArgs {
     int N, Th2, M, H;
+    int ncc, ngroups;  // column chunks, slots
     const Slot* slots;
     const int* order;
     const T* tP;   // gwP        [Th2][H] (parity-grouped h)
@@ -286,11 +313,13 @@ __global__ void __launch_bounds__(256) k_ana(AnaArgs<T> a) {
     constexpr int NO = VEC ? 3 : 1;
     // LDS: [field][parity][theta][column]
     __shared__ T G[NF][2][ANA_JC][ANA_COLS + 1];
-    const int sl = uniform(a.order[blockIdx.x]);
+    const BlockId bid = decode_block(a.ngroups, a.ncc);
+    if (!bid.valid) return;
+    const int sl = uniform(a.order[bid.group]);
     const Slot s = a.slots[sl];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int l16 = lane & 15, q4 = lane >> 4;
-    const int c0 = blockIdx.y * ANA_COLS;
+    const int c0 = bid.cc * ANA_COLS;
     const int C2 = 2 * a.N;
     const int n_even = (s.len + 1) / 2, n_odd = s.len / 2;
     const int t_even = (n_even + 15) / 16, t_odd = (n_odd + 15) / 16;
@@ -422,7 +451,9 @@ static int ana_launch(Ctx& c, int kind, const T* tP, const T* tL, const T* tS, c
     a.o1 = o0;
     a.o2 = o1;
     a.o3 = o2;
-    dim3 grid(c.nslot, (2 * c.N + ANA_COLS - 1) / ANA_COLS);
+    a.ncc = (2 * c.N + ANA_COLS - 1) / ANA_COLS;
+    a.ngroups = c.nslot;
+    dim3 grid(grid_blocks(a.ngroups, a.ncc));
     if (kind == ANA_SCALAR) hipLaunchKernelGGL((k_ana<T, false>), grid, dim3(256), 0, c.stream, a);
     else hipLaunchKernelGGL((k_ana<T, true>), grid, dim3(256), 0, c.stream, a);
     LSD_HIP(hipGetLastError());

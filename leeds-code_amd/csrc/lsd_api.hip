@@ -81,7 +81,7 @@ static void release(Ctx* c) {
     void* ptrs[] = {c->d_hl, c->d_hm, c->d_pidx, c->d_apos, c->d_slots, c->d_slot_order, c->d_gp, c->d_gw,
                     c->d_sin, c->d_tab, c->d_atab, c->d_r, c->d_dr1, c->d_dr2, c->d_radlap, c->d_r1dr,
                     c->d_intr2dr, c->d_ll, c->d_lu, c->d_piv, c->d_y, c->d_gre, c->d_invg, c->d_info,
-                    c->d_lut, c->d_pivt, c->d_yt, c->d_gret, c->d_lut32, c->d_xband, c->d_xt, c->d_tab32,
+                    c->d_lut32, c->d_xband, c->d_sblocks, c->d_sorder, c->d_tab32,
                     c->d_atab32, c->d_tw32,
                     c->d_spec_in, c->d_spec_out, c->d_fourier, c->d_folded, c->d_phys, c->d_nlphys,
                     c->d_tw, c->d_partial};
@@ -149,10 +149,8 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
         dalloc(&c->d_gre, (size_t)2 * L * N) || dalloc(&c->d_invg, (size_t)4 * L) ||
         dalloc(&c->d_info, (size_t)NLU * L) || dalloc(&c->d_spec_in, (size_t)SI_COUNT * HN) ||
         dalloc(&c->d_spec_out, (size_t)SO_COUNT * HN) ||
-        dalloc(&c->d_lut, (size_t)NLU * L * N * LSD_LDAB) || dalloc(&c->d_pivt, (size_t)NLU * L * N) ||
-        dalloc(&c->d_yt, (size_t)NY * L * N * 7) || dalloc(&c->d_gret, (size_t)2 * L * N) ||
         dalloc(&c->d_lut32, (size_t)NLU * L * N * LSD_LDAB) || dalloc(&c->d_xband, (size_t)NLU * L * N * 7) ||
-        dalloc(&c->d_xt, (size_t)NLU * L * N * 7) ||
+        dalloc(&c->d_sorder, H) ||
         dalloc(&c->d_fourier, (size_t)LSD_NUM_PHYS * c->fourier_elems()) ||
         dalloc(&c->d_folded, (size_t)NO_COUNT * c->folded_elems()) ||
         dalloc(&c->d_phys, (size_t)LSD_NUM_PHYS * c->phys_elems()) ||
@@ -162,8 +160,26 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
         if (dalloc(&c->d_f[k], HN)) return LSD_E_HIP;
     for (int k = 0; k < 6; k++)
         if (dalloc(&c->d_hist[k], HN)) return LSD_E_HIP;
-    c->solve_hb = solve_block_harmonics(N, c->mixed);
-    c->n_partial_blocks = (H + c->solve_hb - 1) / c->solve_hb;
+    // solve blocks: harmonics grouped by degree, at most solve_hb per block
+    c->solve_hb = solve_reg_supported(N) ? 32 : solve_block_harmonics(N, c->mixed);
+    if (c->solve_hb < 1) {
+        set_error("N too large for the banded-solve kernels");
+        return LSD_E_CONFIG;
+    }
+    std::vector<int> sorder;
+    std::vector<int4> sblocks;
+    {
+        std::vector<std::vector<int>> byl(L);
+        for (int h = 0; h < H; h++) byl[c->hl[h]].push_back(h);
+        for (int l = 0; l < L; l++)
+            for (size_t k = 0; k < byl[l].size(); k += c->solve_hb) {
+                int cnt = (int)std::min(byl[l].size() - k, (size_t)c->solve_hb);
+                sblocks.push_back(make_int4(l, (int)sorder.size(), cnt, 0));
+                for (int q = 0; q < cnt; q++) sorder.push_back(byl[l][k + q]);
+            }
+    }
+    if (dalloc(&c->d_sblocks, sblocks.size())) return LSD_E_HIP;
+    c->n_partial_blocks = (int)sblocks.size();
     if (dalloc(&c->d_partial, (size_t)8 * c->n_partial_blocks)) return LSD_E_HIP;
     LSD_HIP(hipHostMalloc((void**)&c->h_partial, sizeof(double) * 8 * c->n_partial_blocks, hipHostMallocDefault));
 
@@ -172,6 +188,9 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
     LSD_HIP(hipMemcpyAsync(c->d_hm, c->hm.data(), sizeof(int) * H, hipMemcpyHostToDevice, st));
     LSD_HIP(hipMemcpyAsync(c->d_pidx, pidx.data(), sizeof(int) * pidx.size(), hipMemcpyHostToDevice, st));
     LSD_HIP(hipMemcpyAsync(c->d_apos, apos.data(), sizeof(int) * H, hipMemcpyHostToDevice, st));
+    LSD_HIP(hipMemcpyAsync(c->d_sorder, sorder.data(), sizeof(int) * H, hipMemcpyHostToDevice, st));
+    LSD_HIP(hipMemcpyAsync(c->d_sblocks, sblocks.data(), sizeof(int4) * sblocks.size(), hipMemcpyHostToDevice,
+                           st));
     LSD_HIP(hipMemcpyAsync(c->d_slots, c->slots.data(), sizeof(Slot) * c->nslot, hipMemcpyHostToDevice, st));
     LSD_HIP(hipMemcpyAsync(c->d_slot_order, c->slot_order.data(), sizeof(int) * c->nslot, hipMemcpyHostToDevice, st));
     LSD_HIP(hipMemsetAsync(c->d_tab, 0, sizeof(double) * 5 * H * c->Th2, st));
@@ -483,8 +502,9 @@ static int step_graph(Ctx& c, int nsteps, lsd_step_stats* z) {
 static int step(Ctx& c, int nsteps, lsd_step_stats* st) {
     lsd_step_stats z;
     std::memset(&z, 0, sizeof z);
-    // graphs need the loop to start a fresh step
-    if (c.use_graph && c.it == 0) {
+    // graphs need the loop to start a fresh step; HIP events recorded inside
+    // a graph cannot be timed, so per-class timing runs the eager loop
+    if (c.use_graph && c.it == 0 && c.prof_class < 0) {
         RC(step_graph(c, nsteps, &z));
     } else {
         for (int k = 0; k < nsteps; k++) {
@@ -512,6 +532,74 @@ static int step(Ctx& c, int nsteps, lsd_step_stats* st) {
     }
     z.t = c.t;
     if (st) *st = z;
+    return LSD_OK;
+}
+
+// Steps independent contexts together: each pass launches every member's
+// graph on its own stream before waiting for any, so the members run
+// concurrently; each member's tim_check_cgce then runs on the host.
+static int step_many(std::vector<Ctx*>& cs, int nsteps, lsd_step_stats* st) {
+    const size_t n = cs.size();
+    std::vector<lsd_step_stats> z(n);
+    for (auto& q : z) std::memset(&q, 0, sizeof q);
+    for (Ctx* c : cs) {
+        if (!c->use_graph || c->it != 0 || c->prof_class >= 0) {
+            set_error("lsd_step_many: every member must be between steps, with graphs on and no profiling");
+            return LSD_E_ARG;
+        }
+        if (c->g_first && c->g_prof_class != c->prof_class) drop_graphs(*c);
+        if (!c->g_first) {
+            LSD_HIP(hipSetDevice(c->device));
+            RC(capture(*c, true, &c->g_first));
+            RC(capture(*c, false, &c->g_more));
+            c->g_prof_class = c->prof_class;
+        }
+    }
+    std::vector<int> its(n), more(n);
+    std::vector<double> last(n);
+    for (int k = 0; k < nsteps; k++) {
+        for (size_t i = 0; i < n; i++) {
+            LSD_HIP(hipSetDevice(cs[i]->device));
+            LSD_HIP(hipGraphLaunch(cs[i]->g_first, cs[i]->stream));
+        }
+        bool pending = true;
+        for (size_t i = 0; i < n; i++) {
+            its[i] = 0;
+            more[i] = 1;
+        }
+        while (pending) {
+            pending = false;
+            for (size_t i = 0; i < n; i++) {
+                if (!more[i]) continue;
+                Ctx& c = *cs[i];
+                LSD_HIP(hipStreamSynchronize(c.stream));
+                if (its[i] == 0) {
+                    z[i].evaluations += 2;
+                    c.it = 1;
+                } else {
+                    z[i].evaluations++;
+                }
+                RC(after_corrector(c, its[i], last[i]));
+                if (c.it != 0) {
+                    LSD_HIP(hipSetDevice(c.device));
+                    LSD_HIP(hipGraphLaunch(c.g_more, c.stream));
+                    pending = true;
+                } else {
+                    more[i] = 0;
+                    c.t = c.t + c.dt;
+                    c.step++;
+                    z[i].steps++;
+                    z[i].last_its = its[i];
+                    z[i].max_its = std::max(z[i].max_its, its[i]);
+                    z[i].last_dterr = last[i];
+                }
+            }
+        }
+    }
+    for (size_t i = 0; i < n; i++) {
+        z[i].t = cs[i]->t;
+        if (st) st[i] = z[i];
+    }
     return LSD_OK;
 }
 
@@ -688,6 +776,40 @@ int lsd_corrector(lsd_ctx* ctx, double* dterr) {
     if (rc) return rc;
     if (dterr) *dterr = d;
     return LSD_OK;
+}
+
+int lsd_energy(lsd_ctx* ctx, int which, double* E, double* El, double* Em) {
+    CTX_OR_FAIL(ctx);
+    if ((which != 0 && which != 1) || !E) {
+        lsd::set_error("lsd_energy: which must be 0 (velocity) or 1 (magnetic), E non-null");
+        return LSD_E_ARG;
+    }
+    if (which == 1 && !c.cfg.mag) {
+        lsd::set_error("lsd_energy: the magnetic field is not time-stepped");
+        return LSD_E_ARG;
+    }
+    return lsd::energy(c, which, E, El, Em);
+}
+
+int lsd_step_many(lsd_ctx** ctxs, int n, int nsteps, lsd_step_stats* stats) {
+    if (!ctxs || n < 1) {
+        lsd::set_error("lsd_step_many: no contexts");
+        return LSD_E_ARG;
+    }
+    std::vector<Ctx*> cs;
+    for (int i = 0; i < n; i++) {
+        if (!ctxs[i] || !ctxs[i]->c) {
+            lsd::set_error("lsd_step_many: null context");
+            return LSD_E_ARG;
+        }
+        for (Ctx* o : cs)
+            if (o == ctxs[i]->c) {
+                lsd::set_error("lsd_step_many: a context appears twice");
+                return LSD_E_ARG;
+            }
+        cs.push_back(ctxs[i]->c);
+    }
+    return lsd::step_many(cs, nsteps, stats);
 }
 
 int lsd_set_graph(lsd_ctx* ctx, int on) {

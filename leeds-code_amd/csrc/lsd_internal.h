@@ -11,9 +11,7 @@
 //            analysis   double [j][h]  (h fastest, so lanes over harmonics
 //                                       read coalesced)
 //   bands    mesh  double [n][7]       M(KL+1+n-j, j) stored per column j
-//            lu    double [l][n][10]   LAPACK band storage + int pivots, as
-//                                       built; the solves read a copy laid out
-//                                       [kind][n][row][l] (degree fastest)
+//            lu    double [l][n][10]   LAPACK band storage + int pivots
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -119,13 +117,10 @@ struct Ctx {
     double* d_gre = nullptr;       // [L][2][N]
     double* d_invg = nullptr;      // [L][4]
     int* d_info = nullptr;         // [NLU][L] dgbtrf info
-    double* d_lut = nullptr;       // [NLU][N][10][L]  solve-side copies,
-    int* d_pivt = nullptr;         // [NLU][N][L]      degree fastest
-    double* d_yt = nullptr;        // [NY][N][7][L]
-    double* d_gret = nullptr;      // [2][N][L]
-    float* d_lut32 = nullptr;      // [NLU][N][10][L]  FP32 factors (mixed forms)
+    float* d_lut32 = nullptr;      // [NLU][L][N][10]  FP32 factors (mixed forms)
     double* d_xband = nullptr;     // [NLU][L][N][7]   X before factorisation
-    double* d_xt = nullptr;        // [NLU][N][7][L]   same, degree fastest
+    int4* d_sblocks = nullptr;     // solve blocks: degree, first index into d_sorder, count
+    int* d_sorder = nullptr;       // [H] harmonics grouped by degree
 
     // device: fields
     double2* d_f[LSD_NUM_COLL] = {};   // state + current N
@@ -217,8 +212,13 @@ int launch_layout_coll2spec(Ctx& c, const double2* in, double* re, double* im);
 // nonlinear.hip
 int launch_nonlinear(Ctx& c);
 
+// energy.hip
+int energy(Ctx& c, int which, double* E, double* El, double* Em);
+
 // solve.hip
 int solve_block_harmonics(int N, bool mixed);
+bool solve_reg_supported(int N);
+size_t solve_lds_bytes(const Ctx& c);
 int launch_ops_layout(Ctx& c);
 int launch_predictor(Ctx& c);
 int launch_corrector(Ctx& c, double* dterr);

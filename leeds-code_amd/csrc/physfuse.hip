@@ -11,8 +11,11 @@
 // transform of both rows share every FFT, and the forward output separates
 // directly into the folded north+south / north-south buckets the analysis
 // consumes.  Inputs go through LDS in batches of up to six fields (u and
-// curl u; B and curl B; grad C and grad xi); each thread keeps the values
-// of its own phi points in registers between batches.
+// curl u; B and curl B; grad C and grad xi); each of the 512 threads keeps
+// the values of its own phi point in registers between batches, and does
+// one radix-8 butterfly per FFT stage.  The sequences are stored with one
+// pad slot per 8 complex values, against bank conflicts of the stride-8
+// Stockham writes.
 //
 // Fourier data are stored [f][n][m][j]: the 16-byte value of row j is one
 // piece of a 128-byte line shared by 8 consecutive row pairs.  Blocks are
@@ -23,6 +26,10 @@
 #include "lsd_internal.h"
 
 namespace lsd {
+
+constexpr int PF_THREADS = 512;
+constexpr int PF_EPT = 8;      // 8 sequences x 512 points = one radix-8 butterfly per thread
+constexpr int PF_MAXSEQ = 8;
 
 template <typename R>
 struct PhysArgs {
@@ -44,29 +51,34 @@ struct PhysArgs {
 template <typename V>
 __device__ void pack_inputs(V* Z, const V* __restrict__ fourier, int f0, int f1, int nb, int sh, int N, int M,
                             int Th, int Ph, int jp, int jm) {
-    for (int idx = threadIdx.x; idx < nb * M; idx += FFT_THREADS) {
+    for (int idx = threadIdx.x; idx < nb * M; idx += PF_THREADS) {
         const int b = idx / M, k = idx - b * M;
         const int fid = b < 3 ? f0 + b : f1 + b - 3;
         const V* X = fourier + (((size_t)fid * N + sh) * M + k) * Th;
         const V A = X[jp], B = X[jm];
-        V* z = Z + (size_t)b * Ph;
+        const int z = b * Ph;
         if (k == 0) {
-            z[0] = cx<V>(A.x, B.x);
+            Z[lidx<true>(z)] = cx<V>(A.x, B.x);
         } else {
-            z[k] = cx<V>(A.x - B.y, A.y + B.x);
-            z[Ph - k] = cx<V>(A.x + B.y, B.x - A.y);
+            Z[lidx<true>(z + k)] = cx<V>(A.x - B.y, A.y + B.x);
+            Z[lidx<true>(z + Ph - k)] = cx<V>(A.x + B.y, B.x - A.y);
         }
     }
     const int W = Ph - 2 * M + 1;  // zero band k = M .. Ph-M
-    for (int idx = threadIdx.x; idx < nb * W; idx += FFT_THREADS) {
+    for (int idx = threadIdx.x; idx < nb * W; idx += PF_THREADS) {
         const int b = idx / W, k = M + (idx - b * W);
-        Z[(size_t)b * Ph + k] = cx<V>(0, 0);
+        Z[lidx<true>(b * Ph + k)] = cx<V>(0, 0);
     }
     __syncthreads();
 }
 
+template <typename V>
+__device__ __forceinline__ void fft_pf(V* Z, const FftPlan& pl, int nf, const V* tw, decltype(V().x) sg) {
+    fft_lds<V, PF_THREADS, PF_EPT, true>(Z, pl, nf, tw, sg);
+}
+
 template <typename R, int PP>
-__global__ void __launch_bounds__(FFT_THREADS) k_physfuse(PhysArgs<R> a) {
+__global__ void __launch_bounds__(PF_THREADS) k_physfuse(PhysArgs<R> a) {
     typedef typename Cx<R>::t V;
     extern __shared__ unsigned char lds_raw[];
     V* Z = reinterpret_cast<V*>(lds_raw);
@@ -87,14 +99,14 @@ __global__ void __launch_bounds__(FFT_THREADS) k_physfuse(PhysArgs<R> a) {
     // u and curl u: advection and Coriolis
     {
         pack_inputs(Z, a.fourier, LSD_P_VEL_R, LSD_P_VCURL_R, 6, sh, N, M, Th, Ph, jp, jm);
-        fft_lds<V>(Z, a.pl, 6, a.tw, (R)1);
+        fft_pf<V>(Z, a.pl, 6, a.tw, (R)1);
 #pragma unroll
         for (int p = 0; p < PP; p++) {
-            const int x = t + p * FFT_THREADS;
+            const int x = t + p * PF_THREADS;
             if (x >= Ph) continue;
             V z[6];
 #pragma unroll
-            for (int f = 0; f < 6; f++) z[f] = Z[(size_t)f * Ph + x];
+            for (int f = 0; f < 6; f++) z[f] = Z[lidx<true>(f * Ph + x)];
 #pragma unroll
             for (int r = 0; r < 2; r++) {
                 const R ur = r ? z[0].y : z[0].x, ut = r ? z[1].y : z[1].x, up = r ? z[2].y : z[2].x;
@@ -119,14 +131,14 @@ __global__ void __launch_bounds__(FFT_THREADS) k_physfuse(PhysArgs<R> a) {
     // B and curl B: Lorentz force and induction E = u x B
     if (a.mag) {
         pack_inputs(Z, a.fourier, LSD_P_MAG_R, LSD_P_MCURL_R, 6, sh, N, M, Th, Ph, jp, jm);
-        fft_lds<V>(Z, a.pl, 6, a.tw, (R)1);
+        fft_pf<V>(Z, a.pl, 6, a.tw, (R)1);
 #pragma unroll
         for (int p = 0; p < PP; p++) {
-            const int x = t + p * FFT_THREADS;
+            const int x = t + p * PF_THREADS;
             if (x >= Ph) continue;
             V z[6];
 #pragma unroll
-            for (int f = 0; f < 6; f++) z[f] = Z[(size_t)f * Ph + x];
+            for (int f = 0; f < 6; f++) z[f] = Z[lidx<true>(f * Ph + x)];
 #pragma unroll
             for (int r = 0; r < 2; r++) {
                 const R br = r ? z[0].y : z[0].x, bt = r ? z[1].y : z[1].x, bp = r ? z[2].y : z[2].x;
@@ -146,24 +158,24 @@ __global__ void __launch_bounds__(FFT_THREADS) k_physfuse(PhysArgs<R> a) {
     if (a.cod || a.comp) {
         const int nb = 3 * (a.cod + a.comp);
         pack_inputs(Z, a.fourier, a.cod ? LSD_P_COD_GR : LSD_P_COMP_GR, LSD_P_COMP_GR, nb, sh, N, M, Th, Ph, jp, jm);
-        fft_lds<V>(Z, a.pl, nb, a.tw, (R)1);
+        fft_pf<V>(Z, a.pl, nb, a.tw, (R)1);
         const int oc = 0, ox = a.cod ? 3 : 0;
 #pragma unroll
         for (int p = 0; p < PP; p++) {
-            const int x = t + p * FFT_THREADS;
+            const int x = t + p * PF_THREADS;
             if (x >= Ph) continue;
 #pragma unroll
             for (int r = 0; r < 2; r++) {
                 const R ur = u[p][r][0], ut = u[p][r][1], up = u[p][r][2];
                 if (a.cod) {
-                    const V g0 = Z[(size_t)(oc + 0) * Ph + x], g1 = Z[(size_t)(oc + 1) * Ph + x],
-                            g2 = Z[(size_t)(oc + 2) * Ph + x];
+                    const V g0 = Z[lidx<true>((oc + 0) * Ph + x)], g1 = Z[lidx<true>((oc + 1) * Ph + x)],
+                            g2 = Z[lidx<true>((oc + 2) * Ph + x)];
                     nc[p][r] = -a.qc * ur * (r ? g0.y : g0.x) - a.qc * ut * (r ? g1.y : g1.x) -
                                a.qc * up * (r ? g2.y : g2.x);
                 }
                 if (a.comp) {
-                    const V g0 = Z[(size_t)(ox + 0) * Ph + x], g1 = Z[(size_t)(ox + 1) * Ph + x],
-                            g2 = Z[(size_t)(ox + 2) * Ph + x];
+                    const V g0 = Z[lidx<true>((ox + 0) * Ph + x)], g1 = Z[lidx<true>((ox + 1) * Ph + x)],
+                            g2 = Z[lidx<true>((ox + 2) * Ph + x)];
                     nx[p][r] = -a.qx * ur * (r ? g0.y : g0.x) - a.qx * ut * (r ? g1.y : g1.x) -
                                a.qx * up * (r ? g2.y : g2.x);
                 }
@@ -178,27 +190,27 @@ __global__ void __launch_bounds__(FFT_THREADS) k_physfuse(PhysArgs<R> a) {
     const R sc = a.scale;
 #pragma unroll
     for (int p = 0; p < PP; p++) {
-        const int x = t + p * FFT_THREADS;
+        const int x = t + p * PF_THREADS;
         if (x >= Ph) continue;
         int s = 0;
 #pragma unroll
-        for (int c = 0; c < 3; c++) Z[(size_t)(s++) * Ph + x] = cx<V>(nu[p][0][c] * sc, nu[p][1][c] * sc);
-        if (a.cod) Z[(size_t)(s++) * Ph + x] = cx<V>(nc[p][0] * sc, nc[p][1] * sc);
-        if (a.comp) Z[(size_t)(s++) * Ph + x] = cx<V>(nx[p][0] * sc, nx[p][1] * sc);
+        for (int c = 0; c < 3; c++) Z[lidx<true>((s++) * Ph + x)] = cx<V>(nu[p][0][c] * sc, nu[p][1][c] * sc);
+        if (a.cod) Z[lidx<true>((s++) * Ph + x)] = cx<V>(nc[p][0] * sc, nc[p][1] * sc);
+        if (a.comp) Z[lidx<true>((s++) * Ph + x)] = cx<V>(nx[p][0] * sc, nx[p][1] * sc);
         if (a.mag) {
 #pragma unroll
-            for (int c = 0; c < 3; c++) Z[(size_t)(s++) * Ph + x] = cx<V>(e[p][0][c] * sc, e[p][1][c] * sc);
+            for (int c = 0; c < 3; c++) Z[lidx<true>((s++) * Ph + x)] = cx<V>(e[p][0][c] * sc, e[p][1][c] * sc);
         }
     }
     __syncthreads();
-    fft_lds<V>(Z, a.pl, nout, a.tw, (R)-1);
+    fft_pf<V>(Z, a.pl, nout, a.tw, (R)-1);
     // separate the two rows and fold: G0 = X_j + X_mirror, G1 = X_j - X_mirror
     const size_t fstride = (size_t)N * M * 2 * a.Th2;
-    for (int idx = t; idx < nout * M * 2; idx += FFT_THREADS) {
+    for (int idx = t; idx < nout * M * 2; idx += PF_THREADS) {
         const int s = idx / (2 * M), rem = idx - s * 2 * M;
         const int k = rem >> 1, par = rem & 1;
-        const V zk = Z[(size_t)s * Ph + k];
-        V zc = Z[(size_t)s * Ph + (k ? Ph - k : 0)];
+        const V zk = Z[lidx<true>(s * Ph + k)];
+        V zc = Z[lidx<true>(s * Ph + (k ? Ph - k : 0))];
         zc.y = -zc.y;
         const V A = cx<V>((R)0.5 * (zk.x + zc.x), (R)0.5 * (zk.y + zc.y));
         const V D = csubv(zk, zc);
@@ -237,14 +249,13 @@ static int physfuse_launch(Ctx& c, const void* fourier, void* folded, const void
     a.folded = reinterpret_cast<V*>(folded);
     a.tw = reinterpret_cast<const V*>(tw);
     const int nblk = ((a.ngroups + 7) / 8) * 64;
-    const size_t lds = sizeof(V) * 8 * c.Ph;
+    const size_t lds = sizeof(V) * (PF_MAXSEQ * c.Ph + PF_MAXSEQ * c.Ph / 8 + 1);
     const int nin = 6 + (k.mag ? 6 : 0) + (k.cod ? 3 : 0) + (k.comp ? 3 : 0);
     const int nout = 3 + (k.mag ? 3 : 0) + (k.cod ? 1 : 0) + (k.comp ? 1 : 0);
     // algorithmic bytes: Fourier fields in, folded fields out
     ProfScope ps(c, LSD_K_PHYS,
                  (double)sizeof(V) * c.N * c.M * ((double)nin * c.Th + (double)nout * 2 * c.Th2));
-    if (c.Ph <= FFT_THREADS) hipLaunchKernelGGL((k_physfuse<R, 1>), dim3(nblk), dim3(FFT_THREADS), lds, c.stream, a);
-    else hipLaunchKernelGGL((k_physfuse<R, 2>), dim3(nblk), dim3(FFT_THREADS), lds, c.stream, a);
+    hipLaunchKernelGGL((k_physfuse<R, 1>), dim3(nblk), dim3(PF_THREADS), lds, c.stream, a);
     LSD_HIP(hipGetLastError());
     return LSD_OK;
 }

@@ -87,6 +87,23 @@ def config3(precision=PLAIN_F64):
                        Pr=1.0, Sc=1.0, Pm=2.0, E=1e-4, dt=2e-6, precision=precision)
 
 
+def grid_for(L, M):
+    """Th x Ph for a truncation when the reference rule (Th = 3L/2,
+    Ph = 3M) does not suit the transforms here: the smallest even
+    Th >= 3L/2 and the smallest Ph = 2^a 3^b >= 3M."""
+    th = (3 * L + 1) // 2
+    th += th % 2
+    ph = 3 * M
+    while True:
+        n = ph
+        for f in (2, 3):
+            while n % f == 0:
+                n //= f
+        if n == 1 and ph % 2 == 0:
+            return th, ph
+        ph += 1
+
+
 def config_small(**kw):
     """A reduced case for quick parity checks (odd sizes on purpose)."""
     base = dict(L=12, M=12, N=16, Th=18, Ph=36, comp=1, Ra=6.5e4, Ra_comp=2.0e4, Pm=5.0,

@@ -27,7 +27,7 @@ EXPORTS = (
     "lsd_last_error", "lsd_config_sizes", "lsd_create", "lsd_destroy", "lsd_set_coll", "lsd_get_coll",
     "lsd_get_phys", "lsd_get_table", "lsd_get_mesh", "lsd_transform", "lsd_nonlinear", "lsd_predictor",
     "lsd_corrector", "lsd_step", "lsd_set_dt", "lsd_tra_spec2phys", "lsd_tra_phys2spec", "lsd_tra_grad",
-    "lsd_tra_qst2rtp", "lsd_tra_rtp2qst", "lsd_sync", "lsd_profile_begin", "lsd_profile_end", "lsd_set_graph",
+    "lsd_tra_qst2rtp", "lsd_tra_rtp2qst", "lsd_sync", "lsd_profile_begin", "lsd_profile_end", "lsd_set_graph", "lsd_energy", "lsd_step_many",
 )
 
 # kernel classes for lsd_profile_begin
@@ -70,6 +70,9 @@ def load(path=LIB_PATH):
     lib.lsd_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(StepStats)]
     lib.lsd_set_dt.argtypes = [ctypes.c_void_p, ctypes.c_double]
     lib.lsd_set_graph.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.lsd_energy.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp]
+    lib.lsd_step_many.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int,
+                                  ctypes.POINTER(StepStats)]
     lib.lsd_tra_spec2phys.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
     lib.lsd_tra_phys2spec.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
     lib.lsd_tra_grad.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
@@ -201,6 +204,14 @@ class Context:
     def set_dt(self, dt):
         self._chk(self.lib.lsd_set_dt(self.h, dt), "lsd_set_dt")
 
+    def energy(self, which):
+        """(E, El[L], Em[M]) of the velocity (0) or magnetic field (1)."""
+        E = ctypes.c_double()
+        El = np.zeros(self.cfg.L)
+        Em = np.zeros(self.cfg.M)
+        self._chk(self.lib.lsd_energy(self.h, which, ctypes.byref(E), _p(El), _p(Em)), "lsd_energy")
+        return E.value, El, Em
+
     def set_graph(self, on):
         """lsd_step from captured HIP graphs (True, default) or eager launches."""
         self._chk(self.lib.lsd_set_graph(self.h, 1 if on else 0), "lsd_set_graph")
@@ -235,3 +246,15 @@ class Context:
         args = [_p(np.ascontiguousarray(a)) for a in (r, th, ph)] + [_p(o) for o in out]
         self._chk(self.lib.lsd_tra_rtp2qst(self.h, *args), "lsd_tra_rtp2qst")
         return (out[0], out[1]), (out[2], out[3]), (out[4], out[5])
+
+
+def step_many(contexts, nsteps):
+    """lsd_step_many: steps the contexts together; returns their StepStats."""
+    lib = load()
+    n = len(contexts)
+    hs = (ctypes.c_void_p * n)(*[c.h.value for c in contexts])
+    st = (StepStats * n)()
+    rc = lib.lsd_step_many(hs, n, nsteps, st)
+    if rc != 0:
+        raise LsdError("lsd_step_many: %s (code %d)" % (lib.lsd_last_error().decode(), rc))
+    return list(st)

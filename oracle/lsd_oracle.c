@@ -1782,3 +1782,50 @@ int orc_coll_norm(orc_sim* s, int field, double* out) {
     free(Fod);
     return 0;
 }
+
+/* Bulk energies and their l- and m-spectra, as io_write_vel_energy /
+ * io_write_mag_energy assemble them (program/io.F90:1799-1884): the
+ * field in qst form (var_coll_TorPol2qst, Boussinesq: compr1dr = r1dr,
+ * modules/variables.F90:511-556), E = Efac * sum over q, s, t of the
+ * var_coll_norm integrals (variables.F90:901-957), El/Em the
+ * var_coll_normlm parts (variables.F90:1267-1313) times Efac.
+ * Efac = 1/2 for the velocity, Pm/E for the magnetic field in the
+ * Boussinesq scaling (program/io.F90:238-247).  which: 0 velocity, 1 magnetic. */
+int orc_energy(orc_sim* s, int which, double* E, double* El, double* Em) {
+    int N = s->N, H = s->H;
+    const coll_t* T = &s->f[which ? LSD_MAG_TOR : LSD_VEL_TOR];
+    const coll_t* P = &s->f[which ? LSD_MAG_POL : LSD_VEL_POL];
+    double efac = which ? s->c.Pm / s->c.E : 0.5;
+    coll_t q = coll_new(H * N), sf = coll_new(H * N), t = coll_new(H * N);
+    torpol2qst(s, T, P, &q, &sf, &t);
+    const coll_t* parts[3] = {&q, &sf, &t};
+    double* F = (double*)calloc((size_t)N, sizeof(double));
+    for (int l = 0; l < s->c.L; l++) El[l] = 0.0;
+    for (int m = 0; m < s->c.M; m++) Em[m] = 0.0;
+    double tot = 0.0;
+    for (int k = 0; k < 3; k++) {
+        const coll_t* a = parts[k];
+        for (int n = 0; n < N; n++) F[n] = 0.0;
+        for (int nh = 0; nh < H; nh++) {
+            int l = s->hl[nh], m_ = s->hm[nh];
+            double w = 4.0 * D_PI / (double)(2 * l + 1);
+            if (m_ != 0) w = 4.0 * w;
+            double e = 0.0;
+            for (int n = 0; n < N; n++) {
+                size_t o = (size_t)nh * N + n;
+                double f = w * (a->re[o] * a->re[o] + a->im[o] * a->im[o]);
+                F[n] += f;
+                e += f * s->intr2dr[n];
+            }
+            El[l] += efac * e;
+            Em[m_ / s->c.Mp] += efac * e;
+        }
+        double ek = 0.0;
+        for (int n = 0; n < N; n++) ek += F[n] * s->intr2dr[n];
+        tot += ek;
+    }
+    *E = efac * tot;
+    free(F);
+    coll_free(&q); coll_free(&sf); coll_free(&t);
+    return 0;
+}

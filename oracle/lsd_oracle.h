@@ -66,6 +66,7 @@ int  orc_tra_rtp2qst(orc_sim* s, const double* r, const double* th, const double
 /* Energy diagnostics, var_coll_norm (modules/variables.F90:903-957):
  * out[0..3] = Eodd, Eeven, Em0le, Em0 of one coll field. */
 int  orc_coll_norm(orc_sim* s, int field, double* out);
+int  orc_energy(orc_sim* s, int which, double* E, double* El, double* Em);
 
 #ifdef __cplusplus
 }

@@ -57,6 +57,7 @@ def load():
     lib.orc_tra_qst2rtp.argtypes = [ctypes.c_void_p] + [_dp] * 9
     lib.orc_tra_rtp2qst.argtypes = [ctypes.c_void_p] + [_dp] * 9
     lib.orc_coll_norm.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+    lib.orc_energy.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp]
     return lib
 
 
@@ -146,6 +147,13 @@ class Oracle:
         out = np.zeros(4)
         self._chk(self.lib.orc_coll_norm(self.h, k, _p(out)), "norm")
         return out
+
+    def energy(self, which):
+        E = np.zeros(1)
+        El = np.zeros(self.cfg.L)
+        Em = np.zeros(self.cfg.M)
+        self._chk(self.lib.orc_energy(self.h, which, _p(E), _p(El), _p(Em)), "energy")
+        return float(E[0]), El, Em
 
     # drop-in transforms (spec layout re[n, h]; phys [n, k, j])
     def spec2phys(self, sre, sim):

@@ -149,3 +149,17 @@ def test_one_step_config3():
         assert rel_l2(a, b, not_l0) < 1e-11, k
         assert rel_l2(a, b) < 5e-11, k
     g.close()
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_energy_spectra(pair, which):
+    # lsd_energy against the oracle's Parseval-pinned energies
+    cfg, g, o = pair
+    st = seeded_state(cfg, 23, 1e-2)
+    g.set_state(st)
+    o.set_state(st)
+    Eg, Elg, Emg = g.energy(which)
+    Eo, Elo, Emo = o.energy(which)
+    assert abs(Eg - Eo) <= 1e-12 * abs(Eo)
+    assert np.abs(Elg - Elo).max() <= 1e-12 * abs(Eo)
+    assert np.abs(Emg - Emo).max() <= 1e-12 * abs(Eo)
