@@ -34,7 +34,9 @@ struct Smem {
     double* xb;    // [N][7] (mixed)
     double* y;     // [N][7]
     double* g;     // [2][N]
+    double* rd;    // [N] 1 / U(j,j) (plain)
     float* lu32;   // [N][LDAB] (mixed)
+    float* rd32;   // [N] 1 / U(j,j) of the FP32 factors (mixed)
     int* piv;      // [N]
 };
 
@@ -44,7 +46,7 @@ __host__ __device__ inline size_t solve_smem_bytes(int N, int S, bool mix) {
     size_t b = 8 * (size_t)N * S;
     if (mix) b += align8(4 * (size_t)N * S) + 56 * (size_t)N + 40 * (size_t)N;
     else b += 8 * LSD_LDAB * (size_t)N;
-    return b + 56 * (size_t)N + 16 * (size_t)N + 4 * (size_t)N;
+    return b + 56 * (size_t)N + 16 * (size_t)N + 8 * (size_t)N + (mix ? 4 * (size_t)N : 0) + 4 * (size_t)N;
 }
 
 __device__ inline Smem carve(unsigned char* p, int N, int S, bool mix) {
@@ -64,9 +66,13 @@ __device__ inline Smem carve(unsigned char* p, int N, int S, bool mix) {
     p += 56 * (size_t)N;
     m.g = reinterpret_cast<double*>(p);
     p += 16 * (size_t)N;
+    m.rd = reinterpret_cast<double*>(p);
+    p += 8 * (size_t)N;
     if (mix) {
         m.lu32 = reinterpret_cast<float*>(p);
         p += 40 * (size_t)N;
+        m.rd32 = reinterpret_cast<float*>(p);
+        p += 4 * (size_t)N;
     }
     m.piv = reinterpret_cast<int*>(p);
     return m;
@@ -81,8 +87,10 @@ __device__ void stage_lu(const SolveArgs& a, const Smem& m, int kind, int l) {
     if constexpr (MIX) {
         for (int i = threadIdx.x; i < N * LSD_LDAB; i += 64) m.lu32[i] = a.lu32[off * LSD_LDAB + i];
         for (int i = threadIdx.x; i < N * 7; i += 64) m.xb[i] = a.xb[off * 7 + i];
+        for (int i = threadIdx.x; i < N; i += 64) m.rd32[i] = 1.0f / a.lu32[(off + i) * LSD_LDAB + 2 * LSD_KL];
     } else {
         for (int i = threadIdx.x; i < N * LSD_LDAB; i += 64) m.lu[i] = a.lu[off * LSD_LDAB + i];
+        for (int i = threadIdx.x; i < N; i += 64) m.rd[i] = 1.0 / a.lu[(off + i) * LSD_LDAB + 2 * LSD_KL];
     }
     for (int i = threadIdx.x; i < N; i += 64) m.piv[i] = a.piv[off + i];
 }

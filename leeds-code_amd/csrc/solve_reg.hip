@@ -33,8 +33,12 @@ constexpr int TILE_LD = 65;  // tile row stride (doubles), padded against bank c
 // dgbtrs('N') (timestep.F90:566-588 -> LAPACK dgbtrs, KL = KU = 3,
 // LDAB = 10) on a register column; ab/ip are the LDS factors of this
 // (kind, l), AB(i,j) (1-based) = ab[(j-1)*10 + i-1].
+// The row interchange is a select on the pivot offset, which keeps the
+// pivot load off the serial chain; the backward sweep multiplies by the
+// staged reciprocal of U(j,j) (rd) instead of dividing, which keeps the
+// long-latency division off it too.
 template <int N, typename T>
-__device__ __forceinline__ void gbtrs_reg(T (&b)[N], const T* ab, const int* ip) {
+__device__ __forceinline__ void gbtrs_reg(T (&b)[N], const T* ab, const T* rd, const int* ip) {
 #pragma unroll
     for (int j = 0; j < N - 1; j++) {
         const int p = ip[j] - 1 - j;  // 0..3 (pivot inside the band)
@@ -54,7 +58,7 @@ __device__ __forceinline__ void gbtrs_reg(T (&b)[N], const T* ab, const int* ip)
 #pragma unroll
     for (int j = N - 1; j >= 0; j--) {
         const T* a = ab + j * LSD_LDAB;
-        const T v = b[j] / a[2 * LSD_KL];
+        const T v = b[j] * rd[j];
         b[j] = v;
 #pragma unroll
         for (int d = 1; d <= 2 * LSD_KL; d++)
@@ -66,12 +70,12 @@ __device__ __forceinline__ void gbtrs_reg(T (&b)[N], const T* ab, const int* ip)
 // Mixed forms: b <- X^-1 b with FP32 factors and one FP64-residual sweep
 // against X in band storage, X(n,j) = xb[j*7 + 3+n-j].
 template <int N>
-__device__ __forceinline__ void solve_refined_reg(double (&b)[N], const float* ab32, const int* ip,
-                                                  const double* xb) {
+__device__ __forceinline__ void solve_refined_reg(double (&b)[N], const float* ab32, const float* rd32,
+                                                  const int* ip, const double* xb) {
     float w[N];
 #pragma unroll
     for (int n = 0; n < N; n++) w[n] = (float)b[n];
-    gbtrs_reg<N, float>(w, ab32, ip);
+    gbtrs_reg<N, float>(w, ab32, rd32, ip);
     // residual r = b - X x0, kept in b (row n reads b[n] only once)
 #pragma unroll
     for (int n = 0; n < N; n++) {
@@ -91,15 +95,15 @@ __device__ __forceinline__ void solve_refined_reg(double (&b)[N], const float* a
         pin(r[n]);
         pin(b[n]);
     }
-    gbtrs_reg<N, float>(r, ab32, ip);
+    gbtrs_reg<N, float>(r, ab32, rd32, ip);
 #pragma unroll
     for (int n = 0; n < N; n++) b[n] = b[n] + (double)r[n];
 }
 
 template <int N, bool MIX>
 __device__ __forceinline__ void solve_reg(double (&b)[N], const Smem& m) {
-    if constexpr (MIX) solve_refined_reg<N>(b, m.lu32, m.piv, m.xb);
-    else gbtrs_reg<N, double>(b, m.lu, m.piv);
+    if constexpr (MIX) solve_refined_reg<N>(b, m.lu32, m.rd32, m.piv, m.xb);
+    else gbtrs_reg<N, double>(b, m.lu, m.rd, m.piv);
 }
 
 // Reads the 8-row chunk starting at n0 of every harmonic column of the block
