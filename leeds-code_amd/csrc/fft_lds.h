@@ -109,7 +109,7 @@ __device__ __forceinline__ int lidx(int i) {
 // One Stockham stage over nf sequences of length n held in Z[lidx(p*n + i)]
 // by a workgroup of THREADS threads, each taking up to EPT elements.
 template <int RAD, int THREADS, int EPT, bool PAD, typename V>
-__device__ void fft_stage(V* Z, int n, int nf, int Ns, const V* __restrict__ tw, decltype(V().x) sg) {
+__device__ void fft_stage(V* Z, int n, int nf, int Ns, const V* tw, decltype(V().x) sg) {
     constexpr int BPT = (EPT + RAD - 1) / RAD;
     const int nb = n / RAD;
     const int total = nf * nb;
@@ -152,7 +152,7 @@ __device__ void fft_stage(V* Z, int n, int nf, int Ns, const V* __restrict__ tw,
 
 // Requires nf * n <= EPT * THREADS; ends synchronised.
 template <typename V, int THREADS = FFT_THREADS, int EPT = FFT_EPT, bool PAD = false>
-__device__ void fft_lds(V* Z, const FftPlan& pl, int nf, const V* __restrict__ tw, decltype(V().x) sg) {
+__device__ void fft_lds(V* Z, const FftPlan& pl, int nf, const V* tw, decltype(V().x) sg) {
     int Ns = 1;
     for (int s = 0; s < pl.nstage; s++) {
         int R = pl.radix[s];

@@ -72,16 +72,58 @@ __device__ void pack_inputs(V* Z, const V* __restrict__ fourier, int f0, int f1,
     __syncthreads();
 }
 
-template <typename V>
-__device__ __forceinline__ void fft_pf(V* Z, const FftPlan& pl, int nf, const V* tw, decltype(V().x) sg) {
-    fft_lds<V, PF_THREADS, PF_EPT, true>(Z, pl, nf, tw, sg);
+// Radix-8 stage of a length-512 transform (512 = 8^3) with every index a
+// compile-time shift: NS = 1, 8, 64 is the Stockham sub-transform size.
+// One butterfly per thread (nf <= 8 sequences x 64 butterflies).
+template <int NS, int SG, typename V>
+__device__ __forceinline__ void stage512(V* Z, int nf, const V* tw) {
+    typedef decltype(V().x) S;
+    constexpr S sg = (S)SG;
+    const int b = threadIdx.x;
+    const bool on = b < nf * 64;
+    const int p = b >> 6, j = b & 63, jm = j & (NS - 1);
+    V v[8];
+    if (on) {
+        const int base = p * 512 + j;
+#pragma unroll
+        for (int r = 0; r < 8; r++) v[r] = Z[lidx<true>(base + 64 * r)];
+        if (NS > 1 && jm) {
+#pragma unroll
+            for (int r = 1; r < 8; r++) {
+                V w = tw[jm * r * (64 / NS)];
+                w.y *= sg;
+                v[r] = cmulv(v[r], w);
+            }
+        }
+        dft<8>(v, sg);
+    }
+    __syncthreads();
+    if (on) {
+        const int base = p * 512 + (j / NS) * NS * 8 + jm;
+#pragma unroll
+        for (int r = 0; r < 8; r++) Z[lidx<true>(base + r * NS)] = v[r];
+    }
+    __syncthreads();
+}
+
+template <int SG, typename V>
+__device__ __forceinline__ void fft_pf(V* Z, const FftPlan& pl, int nf, const V* tw) {
+    if (pl.n == 512 && pl.nstage == 3) {  // make_fft_plan gives 8, 8, 8 for 512
+        stage512<1, SG>(Z, nf, tw);
+        stage512<8, SG>(Z, nf, tw);
+        stage512<64, SG>(Z, nf, tw);
+    } else {
+        fft_lds<V, PF_THREADS, PF_EPT, true>(Z, pl, nf, tw, (decltype(V().x))SG);
+    }
 }
 
 template <typename R, int PP>
 __global__ void __launch_bounds__(PF_THREADS) k_physfuse(PhysArgs<R> a) {
     typedef typename Cx<R>::t V;
     extern __shared__ unsigned char lds_raw[];
-    V* Z = reinterpret_cast<V*>(lds_raw);
+    // twiddles first (read by every FFT stage), then the padded sequences
+    V* twl = reinterpret_cast<V*>(lds_raw);
+    V* Z = twl + a.pl.n;
     const int b = blockIdx.x, xcd = b & 7, q = b >> 3;
     const int G = (q >> 3) * 8 + xcd, member = q & 7;
     if (G >= a.ngroups) return;
@@ -91,6 +133,7 @@ __global__ void __launch_bounds__(PF_THREADS) k_physfuse(PhysArgs<R> a) {
     const int jm = a.Th - 1 - jp;
     const int Ph = a.pl.n, N = a.N, M = a.M, Th = a.Th;
     const int t = threadIdx.x;
+    for (int i = t; i < Ph; i += PF_THREADS) twl[i] = a.tw[i];
     const R sn = (R)a.sn[jp];
     const R gpr[2] = {(R)a.gp[jp], (R)a.gp[jm]};
 
@@ -99,7 +142,7 @@ __global__ void __launch_bounds__(PF_THREADS) k_physfuse(PhysArgs<R> a) {
     // u and curl u: advection and Coriolis
     {
         pack_inputs(Z, a.fourier, LSD_P_VEL_R, LSD_P_VCURL_R, 6, sh, N, M, Th, Ph, jp, jm);
-        fft_pf<V>(Z, a.pl, 6, a.tw, (R)1);
+        fft_pf<1>(Z, a.pl, 6, twl);
 #pragma unroll
         for (int p = 0; p < PP; p++) {
             const int x = t + p * PF_THREADS;
@@ -131,7 +174,7 @@ __global__ void __launch_bounds__(PF_THREADS) k_physfuse(PhysArgs<R> a) {
     // B and curl B: Lorentz force and induction E = u x B
     if (a.mag) {
         pack_inputs(Z, a.fourier, LSD_P_MAG_R, LSD_P_MCURL_R, 6, sh, N, M, Th, Ph, jp, jm);
-        fft_pf<V>(Z, a.pl, 6, a.tw, (R)1);
+        fft_pf<1>(Z, a.pl, 6, twl);
 #pragma unroll
         for (int p = 0; p < PP; p++) {
             const int x = t + p * PF_THREADS;
@@ -158,7 +201,7 @@ __global__ void __launch_bounds__(PF_THREADS) k_physfuse(PhysArgs<R> a) {
     if (a.cod || a.comp) {
         const int nb = 3 * (a.cod + a.comp);
         pack_inputs(Z, a.fourier, a.cod ? LSD_P_COD_GR : LSD_P_COMP_GR, LSD_P_COMP_GR, nb, sh, N, M, Th, Ph, jp, jm);
-        fft_pf<V>(Z, a.pl, nb, a.tw, (R)1);
+        fft_pf<1>(Z, a.pl, nb, twl);
         const int oc = 0, ox = a.cod ? 3 : 0;
 #pragma unroll
         for (int p = 0; p < PP; p++) {
@@ -203,7 +246,7 @@ __global__ void __launch_bounds__(PF_THREADS) k_physfuse(PhysArgs<R> a) {
         }
     }
     __syncthreads();
-    fft_pf<V>(Z, a.pl, nout, a.tw, (R)-1);
+    fft_pf<-1>(Z, a.pl, nout, twl);
     // separate the two rows and fold: G0 = X_j + X_mirror, G1 = X_j - X_mirror
     const size_t fstride = (size_t)N * M * 2 * a.Th2;
     for (int idx = t; idx < nout * M * 2; idx += PF_THREADS) {
@@ -249,7 +292,7 @@ static int physfuse_launch(Ctx& c, const void* fourier, void* folded, const void
     a.folded = reinterpret_cast<V*>(folded);
     a.tw = reinterpret_cast<const V*>(tw);
     const int nblk = ((a.ngroups + 7) / 8) * 64;
-    const size_t lds = sizeof(V) * (PF_MAXSEQ * c.Ph + PF_MAXSEQ * c.Ph / 8 + 1);
+    const size_t lds = sizeof(V) * (c.Ph + PF_MAXSEQ * c.Ph + PF_MAXSEQ * c.Ph / 8 + 1);
     const int nin = 6 + (k.mag ? 6 : 0) + (k.cod ? 3 : 0) + (k.comp ? 3 : 0);
     const int nout = 3 + (k.mag ? 3 : 0) + (k.cod ? 1 : 0) + (k.comp ? 1 : 0);
     // algorithmic bytes: Fourier fields in, folded fields out
