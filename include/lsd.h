@@ -238,6 +238,30 @@ int lsd_sync(lsd_ctx* ctx);
 int lsd_profile_begin(lsd_ctx* ctx, int kernel_class);
 int lsd_profile_end(lsd_ctx* ctx, double* total_ms, int* launches, double* work);
 
+/* ------------------------------------------------------------------------
+ * Learned closure network (no counterpart in the reference): a 3-layer MLP
+ * n_in -> width -> width -> n_out with GELU, BF16 weights and activations
+ * on the matrix cores with FP32 accumulation, FP32 master weights trained
+ * by Adam on a mean-squared-error loss.  Parameters, FP32 and contiguous:
+ * W1[width][n_in] b1[width] W2[width][width] b2[width] W3[n_out][width]
+ * b3[n_out].  Rows of x / y / target are contiguous, host memory.
+ * ---------------------------------------------------------------------- */
+typedef struct lsd_mlp lsd_mlp;
+int  lsd_mlp_create(int n_in, int width, int n_out, unsigned long long seed, lsd_mlp** out);
+void lsd_mlp_destroy(lsd_mlp* m);
+long lsd_mlp_num_params(const lsd_mlp* m);
+int  lsd_mlp_get_params(lsd_mlp* m, float* params);
+int  lsd_mlp_set_params(lsd_mlp* m, const float* params);
+/* y[rows][n_out] = MLP(x[rows][n_in]) */
+int  lsd_mlp_forward(lsd_mlp* m, long rows, const float* x, float* y);
+/* loss = mean (MLP(x) - target)^2 and its gradient w.r.t. the parameters
+ * (grad may be NULL) */
+int  lsd_mlp_grad(lsd_mlp* m, long rows, const float* x, const float* target, float* loss, float* grad);
+/* `steps` Adam steps (lr, beta 0.9 / 0.999) on consecutive minibatches of
+ * `batch` rows; loss[steps] (may be NULL) receives the minibatch losses */
+int  lsd_mlp_train(lsd_mlp* m, long rows, const float* x, const float* target, long batch, int steps, float lr,
+                   float* loss);
+
 #ifdef __cplusplus
 }
 #endif

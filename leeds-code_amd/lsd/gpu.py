@@ -28,6 +28,8 @@ EXPORTS = (
     "lsd_get_phys", "lsd_get_table", "lsd_get_mesh", "lsd_transform", "lsd_nonlinear", "lsd_predictor",
     "lsd_corrector", "lsd_step", "lsd_set_dt", "lsd_tra_spec2phys", "lsd_tra_phys2spec", "lsd_tra_grad",
     "lsd_tra_qst2rtp", "lsd_tra_rtp2qst", "lsd_sync", "lsd_profile_begin", "lsd_profile_end", "lsd_set_graph", "lsd_energy", "lsd_step_many",
+    "lsd_mlp_create", "lsd_mlp_destroy", "lsd_mlp_num_params", "lsd_mlp_get_params", "lsd_mlp_set_params",
+    "lsd_mlp_forward", "lsd_mlp_grad", "lsd_mlp_train",
 )
 
 # kernel classes for lsd_profile_begin
@@ -73,6 +75,19 @@ def load(path=LIB_PATH):
     lib.lsd_energy.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp]
     lib.lsd_step_many.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int,
                                   ctypes.POINTER(StepStats)]
+    _fp = ctypes.POINTER(ctypes.c_float)
+    lib.lsd_mlp_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong,
+                                   ctypes.POINTER(ctypes.c_void_p)]
+    lib.lsd_mlp_destroy.argtypes = [ctypes.c_void_p]
+    lib.lsd_mlp_destroy.restype = None
+    lib.lsd_mlp_num_params.argtypes = [ctypes.c_void_p]
+    lib.lsd_mlp_num_params.restype = ctypes.c_long
+    lib.lsd_mlp_get_params.argtypes = [ctypes.c_void_p, _fp]
+    lib.lsd_mlp_set_params.argtypes = [ctypes.c_void_p, _fp]
+    lib.lsd_mlp_forward.argtypes = [ctypes.c_void_p, ctypes.c_long, _fp, _fp]
+    lib.lsd_mlp_grad.argtypes = [ctypes.c_void_p, ctypes.c_long, _fp, _fp, _fp, _fp]
+    lib.lsd_mlp_train.argtypes = [ctypes.c_void_p, ctypes.c_long, _fp, _fp, ctypes.c_long, ctypes.c_int,
+                                  ctypes.c_float, _fp]
     lib.lsd_tra_spec2phys.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
     lib.lsd_tra_phys2spec.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
     lib.lsd_tra_grad.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
@@ -258,3 +273,68 @@ def step_many(contexts, nsteps):
     if rc != 0:
         raise LsdError("lsd_step_many: %s (code %d)" % (lib.lsd_last_error().decode(), rc))
     return list(st)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _pf(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class Mlp:
+    """The closure network on the GPU (lsd_mlp_*): 3 layers with GELU,
+    BF16 weights on the matrix cores, FP32 accumulation, Adam training."""
+
+    def __init__(self, n_in, width, n_out, seed=0):
+        self.lib = load()
+        self.n_in, self.width, self.n_out = n_in, width, n_out
+        h = ctypes.c_void_p()
+        rc = self.lib.lsd_mlp_create(n_in, width, n_out, seed, ctypes.byref(h))
+        if rc != 0:
+            raise LsdError("lsd_mlp_create: %s" % self.lib.lsd_last_error().decode())
+        self.h = h
+        self.n_params = self.lib.lsd_mlp_num_params(self.h)
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise LsdError("%s: %s (code %d)" % (what, self.lib.lsd_last_error().decode(), rc))
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.lib.lsd_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def params(self):
+        p = np.zeros(self.n_params, dtype=np.float32)
+        self._chk(self.lib.lsd_mlp_get_params(self.h, _pf(p)), "lsd_mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p = _f32(p)
+        assert p.size == self.n_params
+        self._chk(self.lib.lsd_mlp_set_params(self.h, _pf(p)), "lsd_mlp_set_params")
+
+    def forward(self, x):
+        x = _f32(x)
+        y = np.zeros((x.shape[0], self.n_out), dtype=np.float32)
+        self._chk(self.lib.lsd_mlp_forward(self.h, x.shape[0], _pf(x), _pf(y)), "lsd_mlp_forward")
+        return y
+
+    def grad(self, x, target):
+        x, target = _f32(x), _f32(target)
+        loss = np.zeros(1, dtype=np.float32)
+        g = np.zeros(self.n_params, dtype=np.float32)
+        self._chk(self.lib.lsd_mlp_grad(self.h, x.shape[0], _pf(x), _pf(target), _pf(loss), _pf(g)), "lsd_mlp_grad")
+        return float(loss[0]), g
+
+    def train(self, x, target, batch, steps, lr):
+        x, target = _f32(x), _f32(target)
+        loss = np.zeros(max(steps, 1), dtype=np.float32)
+        self._chk(self.lib.lsd_mlp_train(self.h, x.shape[0], _pf(x), _pf(target), batch, steps, lr, _pf(loss)),
+                  "lsd_mlp_train")
+        return loss[:steps]

@@ -15,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared():
     src = open(os.path.join(ROOT, "include", "lsd.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(lsd_\w+)\s*\(", src, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|long|void|const char\*)\s+(lsd_\w+)\s*\(", src, re.M)))
 
 
 def test_every_declared_symbol_is_exported():
