@@ -99,6 +99,23 @@ def field_errors(a, b, cfg):
     return out
 
 
+def copy_calibration():
+    """Five copies of 1 GiB between two device buffers: each dispatch of the
+    copy kernel reads and writes exactly 2^30 bytes, which scales what the
+    hardware counters report for the other kernels."""
+    import torch
+    n = 1 << 30
+    a = torch.empty(n, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n, "copies": 5}), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,7 +124,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison run")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 5 device-to-device copies of 1 GiB (the reference kernel for the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the per-timestep path runs on a single MI355X "
               "(multi-GPU sharding is design-only); rerun with --gpus 1", file=sys.stderr)

@@ -80,7 +80,7 @@ static void release(Ctx* c) {
     (void)hipSetDevice(c->device);
     void* ptrs[] = {c->d_hl, c->d_hm, c->d_pidx, c->d_apos, c->d_slots, c->d_slot_order, c->d_gp, c->d_gw,
                     c->d_sin, c->d_tab, c->d_atab, c->d_r, c->d_dr1, c->d_dr2, c->d_radlap, c->d_r1dr,
-                    c->d_intr2dr, c->d_ll, c->d_lu, c->d_piv, c->d_y, c->d_gre, c->d_invg, c->d_info,
+                    c->d_intr2dr, c->d_ll, c->d_lu, c->d_piv, c->d_pivpk, c->d_y, c->d_gre, c->d_invg, c->d_info,
                     c->d_lut32, c->d_xband, c->d_sblocks, c->d_sorder, c->d_tab32,
                     c->d_atab32, c->d_tw32,
                     c->d_spec_in, c->d_spec_out, c->d_fourier, c->d_folded, c->d_phys, c->d_nlphys,
@@ -89,8 +89,11 @@ static void release(Ctx* c) {
         if (p) (void)hipFree(p);
     for (int k = 0; k < LSD_NUM_COLL; k++)
         if (c->d_f[k]) (void)hipFree(c->d_f[k]);
-    for (int k = 0; k < 6; k++)
+    for (int k = 0; k < 6; k++) {
         if (c->d_hist[k]) (void)hipFree(c->d_hist[k]);
+        if (c->d_rhs[k]) (void)hipFree(c->d_rhs[k]);
+    }
+    if (c->d_rhs2) (void)hipFree(c->d_rhs2);
     if (c->h_partial) (void)hipHostFree(c->h_partial);
     for (hipEvent_t e : c->prof_events) (void)hipEventDestroy(e);
     drop_graphs(*c);
@@ -150,6 +153,7 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
         dalloc(&c->d_info, (size_t)NLU * L) || dalloc(&c->d_spec_in, (size_t)SI_COUNT * HN) ||
         dalloc(&c->d_spec_out, (size_t)SO_COUNT * HN) ||
         dalloc(&c->d_lut32, (size_t)NLU * L * N * LSD_LDAB) || dalloc(&c->d_xband, (size_t)NLU * L * N * 7) ||
+        dalloc(&c->d_pivpk, (size_t)NLU * L * ((N + 15) / 16)) ||
         dalloc(&c->d_sorder, H) ||
         dalloc(&c->d_fourier, (size_t)LSD_NUM_PHYS * c->fourier_elems()) ||
         dalloc(&c->d_folded, (size_t)NO_COUNT * c->folded_elems()) ||
@@ -160,8 +164,14 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
         if (dalloc(&c->d_f[k], HN)) return LSD_E_HIP;
     for (int k = 0; k < 6; k++)
         if (dalloc(&c->d_hist[k], HN)) return LSD_E_HIP;
+    const bool direct = solve_direct_supported(N);
+    if (direct) {
+        for (int k = 0; k < 6; k++)
+            if (dalloc(&c->d_rhs[k], HN)) return LSD_E_HIP;
+        if (dalloc(&c->d_rhs2, HN)) return LSD_E_HIP;
+    }
     // solve blocks: harmonics grouped by degree, at most solve_hb per block
-    c->solve_hb = solve_reg_supported(N) ? 32 : solve_block_harmonics(N, c->mixed);
+    c->solve_hb = direct ? 32 : solve_block_harmonics(N, c->mixed);
     if (c->solve_hb < 1) {
         set_error("N too large for the banded-solve kernels");
         return LSD_E_CONFIG;

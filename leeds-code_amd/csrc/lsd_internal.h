@@ -113,6 +113,7 @@ struct Ctx {
     // device: operators
     double* d_lu = nullptr;        // [NLU][L][N][10]
     int* d_piv = nullptr;          // [NLU][L][N]
+    unsigned* d_pivpk = nullptr;   // [NLU][L][(N+15)/16] pivot offsets, 2 bits per row
     double* d_y = nullptr;         // [NY][L][N][7]
     double* d_gre = nullptr;       // [L][2][N]
     double* d_invg = nullptr;      // [L][4]
@@ -125,6 +126,8 @@ struct Ctx {
     // device: fields
     double2* d_f[LSD_NUM_COLL] = {};   // state + current N
     double2* d_hist[6] = {};           // saved N^n
+    double2* d_rhs[6] = {};            // solve right-hand sides [H][N] (two-pass solves)
+    double2* d_rhs2 = nullptr;         // poloidal velocity's second right-hand side
     double2* d_spec_in = nullptr;      // [SI_COUNT][H][N]
     double2* d_spec_out = nullptr;     // [SO_COUNT][H][N]
     double2* d_fourier = nullptr;      // [18][N][M][Th]
@@ -217,12 +220,15 @@ int energy(Ctx& c, int which, double* E, double* El, double* Em);
 
 // solve.hip
 int solve_block_harmonics(int N, bool mixed);
-bool solve_reg_supported(int N);
 size_t solve_lds_bytes(const Ctx& c);
 int launch_ops_layout(Ctx& c);
 int launch_predictor(Ctx& c);
 int launch_corrector(Ctx& c, double* dterr);
 int launch_corrector_async(Ctx& c);
 double corrector_norms(const Ctx& c);
+
+// solve_direct.hip
+bool solve_direct_supported(int N);
+int launch_solve_direct(Ctx& c, bool pred);
 
 }  // namespace lsd

@@ -326,9 +326,26 @@ __global__ void k_lu_f32(size_t n, const double* __restrict__ lu, float* __restr
     if (i < n) lu32[i] = (float)lu[i];
 }
 
+// Pivot offsets ip(j)-1-j (0..3: dgbtrf pivots within KL rows) packed 2 bits
+// per row, 16 rows per word: pk[(kind*L + l)*npk + j/16].
+__global__ void k_pack_piv(size_t nw, int N, int npk, const int* __restrict__ piv, unsigned* __restrict__ pk) {
+    const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nw) return;
+    const size_t kl = w / npk;
+    const int j0 = (int)(w - kl * npk) * 16;
+    unsigned v = 0;
+    for (int j = j0; j < j0 + 16 && j < N; j++) v |= (unsigned)((piv[kl * N + j] - 1 - j) & 3) << (2 * (j - j0));
+    pk[w] = v;
+}
+
 int launch_ops_layout(Ctx& c) {
     const size_t n = (size_t)NLU * c.L * c.N * LSD_LDAB;
     hipLaunchKernelGGL(k_lu_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, n, c.d_lu, c.d_lut32);
+    LSD_HIP(hipGetLastError());
+    const int npk = (c.N + 15) / 16;
+    const size_t nw = (size_t)NLU * c.L * npk;
+    hipLaunchKernelGGL(k_pack_piv, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, c.stream, nw, c.N, npk, c.d_piv,
+                       c.d_pivpk);
     LSD_HIP(hipGetLastError());
     return LSD_OK;
 }
@@ -385,11 +402,11 @@ static int active_fields(const Ctx& c) {
 static size_t solve_lds(const Ctx& c) { return solve_lds_bytes(c); }
 
 int launch_predictor(Ctx& c) {
+    if (solve_direct_supported(c.N)) return launch_solve_direct(c, true);
     SolveArgs a = make_args(c);
     // per field: read state and N, write state and saved N
     ProfScope ps(c, LSD_K_SOLVE, 4.0 * 16.0 * c.coll_elems() * active_fields(c));
     dim3 grid(c.n_partial_blocks, 6);
-    if (solve_reg_supported(c.N)) return launch_solve_reg(c, false, a, grid);
     if (c.mixed) hipLaunchKernelGGL(k_predict<true>, grid, dim3(64), solve_lds(c), c.stream, a);
     else hipLaunchKernelGGL(k_predict<false>, grid, dim3(64), solve_lds(c), c.stream, a);
     LSD_HIP(hipGetLastError());
@@ -403,13 +420,13 @@ int launch_corrector_async(Ctx& c) {
     SolveArgs a = make_args(c);
     const int nb = c.n_partial_blocks;
     dim3 grid(nb, 6);
-    {
+    if (solve_direct_supported(c.N)) {
+        int rc = launch_solve_direct(c, false);
+        if (rc) return rc;
+    } else {
         // per field: read state, N and saved N, write state and saved N
         ProfScope ps(c, LSD_K_SOLVE, 5.0 * 16.0 * c.coll_elems() * active_fields(c));
-        if (solve_reg_supported(c.N)) {
-            int rc = launch_solve_reg(c, true, a, grid);
-            if (rc) return rc;
-        } else if (c.mixed) {
+        if (c.mixed) {
             hipLaunchKernelGGL(k_correct<true>, grid, dim3(64), solve_lds(c), c.stream, a);
         } else {
             hipLaunchKernelGGL(k_correct<false>, grid, dim3(64), solve_lds(c), c.stream, a);

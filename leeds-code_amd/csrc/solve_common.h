@@ -1,5 +1,5 @@
 // Shared declarations of the banded-solve kernels (solve.hip: columns in
-// LDS, any N; solve_reg.hip: columns in registers, N fixed at compile time).
+// LDS, any N; solve_direct.hip: two passes, N fixed at compile time).
 #pragma once
 
 #include "lsd_internal.h"
@@ -103,9 +103,5 @@ __device__ inline void stage_g(const SolveArgs& a, const Smem& m, int l) {
     for (int i = threadIdx.x; i < 2 * a.N; i += 64) m.g[i] = a.gre[off + i];
 }
 
-
-// solve_reg.hip: register-column kernels for the N they are built for
-bool solve_reg_supported(int N);
-int launch_solve_reg(Ctx& c, bool corr, const SolveArgs& a, dim3 grid);
 
 }  // namespace lsd

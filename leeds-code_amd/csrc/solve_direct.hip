@@ -1,0 +1,451 @@
+// Predictor / corrector solves in two passes, for the radial resolutions
+// the library is built for (N fixed at compile time).
+//
+// Pass 1, k_rhs: one work-item per (harmonic, radial point), a wave over
+// 64 consecutive points of one coll column, so every read and write is
+// coalesced.  It forms each field's right-hand side in FP64 -- Y f + N
+// with the boundary rows of the *_setbc routines (predictor), c (N* - N)
+// (corrector) -- saves N for the next corrector, and writes the right-hand
+// side in the solve precision.
+//
+// Pass 2, k_sdir: one 64-lane wave per block of up to 32 harmonics of one
+// degree l (lane = harmonic, Re/Im part), so every lane needs the same
+// factors, which are staged once in LDS and read at lane-uniform
+// addresses.  Each lane loads its whole right-hand-side column straight
+// into registers -- every row's load in flight at once -- runs dgbtrs on
+// it, and writes the result back; no LDS transposition and no barrier
+// between the loads and the sweeps, so the forward sweep starts on the
+// first rows while the later ones are still arriving.
+//
+// The reference's predictor x* = X^-1 (Y x + N) and corrector
+// x += X^-1 c (N* - N), with the poloidal velocity's XGre / XPol /
+// vel_gre_adjust sequence (modules/velocity.F90:369-465), in FP64 for both
+// the plain and the mixed forms: the sweeps are latency- and
+// bandwidth-bound, and FP32 arithmetic issues at the FP64 rate here, so
+// FP32 factors would only add the refinement sweep they need when X is
+// poorly conditioned (large dt or fine radial grids).
+#include <type_traits>
+
+#include "solve_common.h"
+
+namespace lsd {
+
+struct DirArgs {
+    int H, N, L;
+    int on[6];           // field enabled: vT, vP, C, X, bT, bP
+    double cimp;
+    const int* hl;
+    const int* hm;
+    const int4* blocks;  // per block: degree l, first index into order, harmonic count
+    const int* order;    // harmonics grouped by degree
+    const double* lu;    // [NLU][L][N][LDAB]  FP64 factors
+    const unsigned* pk;  // [NLU][L][npk] pivot offsets ip(j)-1-j, 2 bits per row, 16 rows per word
+    int npk;
+    const double* y;     // [NY][L][N][7]
+    const double* gre;   // [L][2][N]
+    const double* invg;  // [L][4]
+    double2* x[6];       // state
+    const double2* nn[6];  // current N
+    double2* hist[6];      // saved N
+    double2* rhs[6];     // [H][N] right-hand sides
+    double2* rhs2;       // poloidal velocity: the Y x part of the XPol right-hand side
+    double* partial;     // [8][gridDim.x]
+};
+
+// v[f] for a field index that is the same across the workgroup, as scalar
+// selects (indexing the kernel-argument array directly would copy it to
+// scratch memory).
+template <typename P>
+__device__ __forceinline__ P pick(const P (&v)[6], int f) {
+    P r = v[0];
+#pragma unroll
+    for (int k = 1; k < 6; k++)
+        if (f == k) r = v[k];
+    return r;
+}
+
+// LU kind of the (main) solve of field f; the poloidal velocity's XGre solve.
+__host__ __device__ inline int lu_kind(int f) { return f == 0 ? 0 : (f == 1 ? 2 : f + 1); }
+
+// Row n of (band . x) accumulated onto acc in ascending column order
+// (tim_multY); band stored per column, A(n,j) = band[j*7 + 3+n-j].
+__device__ __forceinline__ double2 band_acc(double2 acc, const double* __restrict__ band,
+                                            const double2* __restrict__ xc, int n, int N) {
+    const int lo = n - LSD_KL > 0 ? n - LSD_KL : 0;
+    const int hi = n + LSD_KL < N - 1 ? n + LSD_KL : N - 1;
+    for (int j = lo; j <= hi; j++) {
+        const double a = band[j * 7 + LSD_KL + n - j];
+        const double2 xj = xc[j];
+        acc.x = fma(a, xj.x, acc.x);
+        acc.y = fma(a, xj.y, acc.y);
+    }
+    return acc;
+}
+
+template <bool PRED>
+__global__ void __launch_bounds__(256) k_rhs(DirArgs a) {
+    const int f = blockIdx.y;
+    if (!a.on[f]) return;
+    const int N = a.N;
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (size_t)a.H * N) return;
+    const int h = (int)(idx / N), n = (int)(idx - (size_t)h * N);
+    const int l = a.hl[h];
+    // boundary rows as a factor, not a select between double2 values
+    // (which the compiler would route through scratch memory)
+    const double keep = (n == 0 || n == N - 1) ? 0.0 : 1.0;
+    const double2 nv = pick(a.nn, f)[idx];
+    const double2* xc = pick(a.x, f) + (size_t)h * N;
+    double2* hist = pick(a.hist, f);
+    double2* rhs = pick(a.rhs, f);
+    if constexpr (PRED) {
+        hist[idx] = nv;
+        if (f == 1 && l != 0) {
+            // XGre acts on N with zero boundary rows; the Y x part of the
+            // XPol right-hand side (its boundary rows are zero too)
+            rhs[idx] = make_double2(keep * nv.x, keep * nv.y);
+            const double2 r = band_acc(make_double2(0.0, 0.0), a.y + ((size_t)1 * a.L + l) * N * 7, xc, n, N);
+            a.rhs2[idx] = make_double2(keep * r.x, keep * r.y);
+        } else {
+            const bool has_y = f == 2 || f == 3 || l != 0;
+            double2 b = nv;
+            if (has_y) b = band_acc(b, a.y + ((size_t)f * a.L + l) * N * 7, xc, n, N);
+            b.x = keep * b.x;
+            b.y = keep * b.y;
+            if ((f == 2 || f == 3) && h == 0 && n == 0) b.x = 1.0;
+            rhs[idx] = b;
+        }
+    } else {
+        const double2 hv = hist[idx];
+        hist[idx] = nv;
+        double2 d = make_double2(keep * (a.cimp * (nv.x - hv.x)), keep * (a.cimp * (nv.y - hv.y)));
+        // vel_setbc_Tor for (1,0) once tim_it /= 0
+        if (f == 0 && l == 1 && a.hm[h] == 0 && n == 0) d.x = 0.0 - xc[0].x;
+        rhs[idx] = d;
+    }
+}
+
+// Ends one row of a sweep: the scheduler moves nothing across, so the
+// factor reads issued for later rows stay where they are placed.
+#define SWEEP_FENCE() __builtin_amdgcn_sched_barrier(0)
+// Keeps the memory operations of one streamed batch from being merged with
+// those of the next (each batch's loads would otherwise all be hoisted).
+#define BATCH_FENCE() asm volatile("" ::: "memory")
+
+// Materialises a value in its registers at this point of the program, so
+// the arithmetic that produces it is not sunk below a later fence.
+template <typename T>
+__device__ __forceinline__ void pin(T& v) {
+    asm volatile("" : "+v"(v));
+}
+
+constexpr int SWEEP_AHEAD = 2;  // rows of factors read from LDS ahead of the sweep
+
+// dgbtrs('N') (timestep.F90:566-588 -> LAPACK dgbtrs, KL = KU = 3,
+// LDAB = 10) on a register column; ab holds the factors of this (kind, l)
+// in LDS, AB(i,j) (1-based) = ab[(j-1)*10 + i-1], rd[j] = 1 / U(j,j), and
+// pk the pivot offsets ip(j)-1-j (0..3) packed 2 bits per row, read with
+// scalar loads.  The row interchange is a select on the (scalar) offset,
+// the backward sweep multiplies by the reciprocal of U(j,j) instead of
+// dividing, and each row's factors are read SWEEP_AHEAD rows early, so no
+// LDS or division latency sits on the serial chain.
+template <int N, typename T>
+__device__ __forceinline__ void gbtrs_reg(T (&b)[N], const T* ab, const T* rd, const unsigned* __restrict__ pk) {
+    constexpr int NPK = (N + 15) / 16;
+    unsigned pw[NPK];
+#pragma unroll
+    for (int k = 0; k < NPK; k++) pw[k] = pk[k];
+    T fl[N][3];
+#pragma unroll
+    for (int j = 0; j < SWEEP_AHEAD; j++)
+#pragma unroll
+        for (int e = 0; e < 3; e++) fl[j][e] = ab[j * LSD_LDAB + 2 * LSD_KL + 1 + e];
+#pragma unroll
+    for (int j = 0; j < N - 1; j++) {
+        if (j + SWEEP_AHEAD < N - 1) {
+#pragma unroll
+            for (int e = 0; e < 3; e++) fl[j + SWEEP_AHEAD][e] = ab[(j + SWEEP_AHEAD) * LSD_LDAB + 2 * LSD_KL + 1 + e];
+        }
+        const unsigned p = (pw[j >> 4] >> (2 * (j & 15))) & 3u;
+        const T bj = b[j];
+        T c1 = T(0), c2 = T(0), c3 = T(0);
+        if (j + 1 < N) c1 = b[j + 1];
+        if (j + 2 < N) c2 = b[j + 2];
+        if (j + 3 < N) c3 = b[j + 3];
+        const T y = p == 0 ? bj : (p == 1 ? c1 : (p == 2 ? c2 : c3));
+        b[j] = y;
+        if (j + 1 < N) b[j + 1] = (p == 1 ? bj : c1) - fl[j][0] * y;
+        if (j + 2 < N) b[j + 2] = (p == 2 ? bj : c2) - fl[j][1] * y;
+        if (j + 3 < N) b[j + 3] = (p == 3 ? bj : c3) - fl[j][2] * y;
+        SWEEP_FENCE();
+    }
+    T fu[N][2 * LSD_KL + 1];  // [0] = 1/U(j,j), [d] = U(j-d, j)
+#pragma unroll
+    for (int j = N - 1; j > N - 1 - SWEEP_AHEAD; j--) {
+        fu[j][0] = rd[j];
+#pragma unroll
+        for (int d = 1; d <= 2 * LSD_KL; d++) fu[j][d] = ab[j * LSD_LDAB + 2 * LSD_KL - d];
+    }
+#pragma unroll
+    for (int j = N - 1; j >= 0; j--) {
+        const int q = j - SWEEP_AHEAD;
+        if (q >= 0) {
+            fu[q][0] = rd[q];
+#pragma unroll
+            for (int d = 1; d <= 2 * LSD_KL; d++) fu[q][d] = ab[q * LSD_LDAB + 2 * LSD_KL - d];
+        }
+        const T v = b[j] * fu[j][0];
+        b[j] = v;
+#pragma unroll
+        for (int d = 1; d <= 2 * LSD_KL; d++)
+            if (j - d >= 0) b[j - d] = b[j - d] - v * fu[j][d];
+        SWEEP_FENCE();
+    }
+}
+
+// LDS of one block: the factors of one (kind, l) and the Green's functions.
+template <int N>
+struct DirSmem {
+    double ab[N * LSD_LDAB];
+    double rd[N];
+    double g[2 * N];
+};
+
+// Copies the factors of (kind, l) into LDS, with the reciprocal diagonal.
+// Callers synchronise before the data are read and before they are replaced.
+template <int N>
+__device__ __forceinline__ void stage_factors(const DirArgs& a, DirSmem<N>& s, int kind, int l) {
+    const double* src = a.lu + ((size_t)kind * a.L + l) * N * LSD_LDAB;
+    constexpr int NA = (N * LSD_LDAB + 63) / 64;
+    double v[NA];
+#pragma unroll
+    for (int k = 0; k < NA; k++) {
+        const int i = threadIdx.x + 64 * k;
+        if (i < N * LSD_LDAB) v[k] = src[i];
+    }
+#pragma unroll
+    for (int k = 0; k < NA; k++) {
+        const int i = threadIdx.x + 64 * k;
+        if (i < N * LSD_LDAB) {
+            s.ab[i] = v[k];
+            if (i % LSD_LDAB == 2 * LSD_KL) s.rd[i / LSD_LDAB] = 1.0 / v[k];
+        }
+    }
+}
+
+__device__ __forceinline__ const unsigned* pivots(const DirArgs& a, int kind, int l) {
+    return a.pk + ((size_t)kind * a.L + l) * a.npk;
+}
+
+// Rows per batch when a second column (the state, or the poloidal
+// velocity's second right-hand side) is streamed against the register
+// column: 16 rows keep the column (2N VGPRs) and the batch within the 256
+// VGPRs of two waves per SIMD at N = 96.
+constexpr int DIR_BATCH = 16;
+
+// At most 256 VGPRs, so two waves share each SIMD.
+template <int N, bool PRED>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_sdir(DirArgs a) {
+    static_assert(N % DIR_BATCH == 0, "N must be a multiple of the batch");
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ double red[2][64];
+    DirSmem<N>& s = *reinterpret_cast<DirSmem<N>*>(smem_raw);
+    constexpr int BT = DIR_BATCH;
+    const int f = blockIdx.y;
+    const int t = threadIdx.x;
+    double e2 = 0.0, eic = 0.0;
+    if (a.on[f]) {
+        const int4 blk = a.blocks[blockIdx.x];
+        const int l = blk.x, first = blk.y, nh = blk.z;
+        const int hh = t >> 1, part = t & 1;
+        const bool act = hh < nh;
+        const int h = a.order[first + (act ? hh : 0)];
+        const bool solve = f == 2 || f == 3 || l != 0;
+        // this lane's real column of the right-hand side, every row at once,
+        // issued ahead of the factor staging so both are in flight together
+        const double* src = reinterpret_cast<const double*>(pick(a.rhs, f) + (size_t)h * N) + part;
+        double b[N];
+#pragma unroll
+        for (int n = 0; n < N; n++) b[n] = src[2 * n];
+        if (solve) {
+            stage_factors<N>(a, s, lu_kind(f), l);
+            if (f == 1)
+                for (int i = t; i < 2 * N; i += 64) s.g[i] = a.gre[(size_t)l * 2 * N + i];
+        }
+        __syncthreads();
+        if (solve) gbtrs_reg<N, double>(b, s.ab, s.rd, pivots(a, lu_kind(f), l));
+        double* xs = reinterpret_cast<double*>(pick(a.x, f) + (size_t)h * N) + part;
+        if (f == 1 && solve) {
+            if constexpr (PRED) {
+                // XPol right-hand side: XGre^-1 N + Y x inside, zero on the
+                // boundary rows (where the second part is zero)
+                const double* src2 = reinterpret_cast<const double*>(a.rhs2 + (size_t)h * N) + part;
+#pragma unroll
+                for (int c = 0; c < N / BT; c++) {
+                    double r2[BT];
+#pragma unroll
+                    for (int r = 0; r < BT; r++) r2[r] = src2[2 * (c * BT + r)];
+#pragma unroll
+                    for (int r = 0; r < BT; r++) {
+                        const int n = c * BT + r;
+                        b[n] = (n == 0 || n == N - 1) ? r2[r] : b[n] + r2[r];
+                        pin(b[n]);
+                    }
+                    BATCH_FENCE();
+                }
+            } else {
+                b[0] = 0.0;
+                b[N - 1] = 0.0;
+            }
+            __syncthreads();
+            stage_factors<N>(a, s, 1, l);
+            __syncthreads();
+            gbtrs_reg<N, double>(b, s.ab, s.rd, pivots(a, 1, l));
+            // vel_gre_adjust (velocity.F90:445-465), on x* (predictor) or on
+            // the correction (corrector)
+            const double* ig = a.invg + l * 4;
+            const double bi = b[0], bo = b[N - 1];
+            const double ca = ig[0] * bi + ig[2] * bo;
+            const double cb = ig[1] * bi + ig[3] * bo;
+            if constexpr (PRED) {
+#pragma unroll
+                for (int c = 0; c < N / BT; c++) {
+#pragma unroll
+                    for (int r = 0; r < BT; r++) {
+                        const int n = c * BT + r;
+                        if (act) xs[2 * n] = b[n] - ca * s.g[n] - cb * s.g[N + n];
+                    }
+                    BATCH_FENCE();
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < N / BT; c++) {
+                    double xo[BT];
+#pragma unroll
+                    for (int r = 0; r < BT; r++) xo[r] = xs[2 * (c * BT + r)];
+#pragma unroll
+                    for (int r = 0; r < BT; r++) {
+                        const int n = c * BT + r;
+                        const double d = b[n] - ca * s.g[n] - cb * s.g[N + n];
+                        if (act) {
+                            xs[2 * n] = xo[r] + d;
+                            e2 = e2 + d * d;
+                        }
+                    }
+                    BATCH_FENCE();
+                }
+            }
+        } else if constexpr (PRED) {
+#pragma unroll
+            for (int n = 0; n < N; n++)
+                if (act) xs[2 * n] = b[n];
+        } else {
+            // x += d, with the correction norms of tim_addcorr
+#pragma unroll
+            for (int c = 0; c < N / BT; c++) {
+                double xo[BT];
+#pragma unroll
+                for (int r = 0; r < BT; r++) xo[r] = xs[2 * (c * BT + r)];
+#pragma unroll
+                for (int r = 0; r < BT; r++) {
+                    const int n = c * BT + r;
+                    const double d = b[n];
+                    if (act) {
+                        xs[2 * n] = xo[r] + d;
+                        e2 = e2 + d * d;
+                        if (f >= 4 && n == 0 && l != 0) eic = eic + d * d;
+                    }
+                }
+                BATCH_FENCE();
+            }
+        }
+    }
+    if constexpr (!PRED) {
+        red[0][t] = e2;
+        red[1][t] = eic;
+        __syncthreads();
+        if (t == 0) {
+            double sum = 0.0, si = 0.0;
+            for (int k = 0; k < 64; k++) {
+                sum += red[0][k];
+                si += red[1][k];
+            }
+            const int slot = f < 4 ? f : (f == 4 ? 4 : 6);
+            a.partial[(size_t)slot * gridDim.x + blockIdx.x] = sum;
+            if (f >= 4) a.partial[(size_t)(slot + 1) * gridDim.x + blockIdx.x] = si;
+        }
+    }
+}
+
+bool solve_direct_supported(int N) { return N == 32 || N == 48 || N == 64 || N == 96; }
+
+static DirArgs dir_args(Ctx& c) {
+    DirArgs a;
+    a.H = c.H;
+    a.N = c.N;
+    a.L = c.L;
+    a.on[0] = 1;
+    a.on[1] = 1;
+    a.on[2] = c.cfg.cod;
+    a.on[3] = c.cfg.comp;
+    a.on[4] = c.cfg.mag;
+    a.on[5] = c.cfg.mag;
+    a.cimp = c.cfg.implicit;
+    a.hl = c.d_hl;
+    a.hm = c.d_hm;
+    a.blocks = c.d_sblocks;
+    a.order = c.d_sorder;
+    a.lu = c.d_lu;
+    a.pk = c.d_pivpk;
+    a.npk = (c.N + 15) / 16;
+    a.y = c.d_y;
+    a.gre = c.d_gre;
+    a.invg = c.d_invg;
+    const int fx[6] = {LSD_VEL_TOR, LSD_VEL_POL, LSD_COD, LSD_COMP, LSD_MAG_TOR, LSD_MAG_POL};
+    const int fn[6] = {LSD_N_VEL_TOR, LSD_N_VEL_POL, LSD_N_COD, LSD_N_COMP, LSD_N_MAG_TOR, LSD_N_MAG_POL};
+    for (int k = 0; k < 6; k++) {
+        a.x[k] = c.d_f[fx[k]];
+        a.nn[k] = c.d_f[fn[k]];
+        a.hist[k] = c.d_hist[k];
+        a.rhs[k] = c.d_rhs[k];
+    }
+    a.rhs2 = c.d_rhs2;
+    a.partial = c.d_partial;
+    return a;
+}
+
+template <int N, bool PRED>
+static void launch_sdir(Ctx& c, const DirArgs& a) {
+    hipLaunchKernelGGL((k_sdir<N, PRED>), dim3(c.n_partial_blocks, 6), dim3(64), sizeof(DirSmem<N>), c.stream, a);
+}
+
+template <bool PRED>
+static int launch_sdir_n(Ctx& c, const DirArgs& a) {
+    switch (c.N) {
+    case 32: launch_sdir<32, PRED>(c, a); break;
+    case 48: launch_sdir<48, PRED>(c, a); break;
+    case 64: launch_sdir<64, PRED>(c, a); break;
+    case 96: launch_sdir<96, PRED>(c, a); break;
+    default: set_error("no two-pass solve kernel for this N"); return LSD_E_CONFIG;
+    }
+    LSD_HIP(hipGetLastError());
+    return LSD_OK;
+}
+
+// Both passes of the predictor (pred) or the corrector on the context's
+// stream.  Work (bytes) per field and coll element: pass 1 reads N and the
+// state (predictor) or N and the saved N (corrector) and writes the saved
+// N and the right-hand side; pass 2 reads the right-hand side and writes
+// the state, reading it too in the corrector.
+int launch_solve_direct(Ctx& c, bool pred) {
+    const DirArgs a = dir_args(c);
+    const int nf = 2 + (c.cfg.cod ? 1 : 0) + (c.cfg.comp ? 1 : 0) + (c.cfg.mag ? 2 : 0);
+    ProfScope ps(c, LSD_K_SOLVE, (4.0 * 16.0 + (pred ? 2.0 : 3.0) * 16.0) * c.coll_elems() * nf);
+    const dim3 g1((unsigned)((c.coll_elems() + 255) / 256), 6);
+    if (pred) hipLaunchKernelGGL((k_rhs<true>), g1, dim3(256), 0, c.stream, a);
+    else hipLaunchKernelGGL((k_rhs<false>), g1, dim3(256), 0, c.stream, a);
+    LSD_HIP(hipGetLastError());
+    return pred ? launch_sdir_n<true>(c, a) : launch_sdir_n<false>(c, a);
+}
+
+}  // namespace lsd

@@ -10,18 +10,19 @@ from a seeded state band-limited to l < 24 and spins up 6000 steps; 200
 snapshots 20 steps apart give (truncated state, truncated resolved N,
 truncated-model N) triples; a random 70/30 split trains and tests the
 network (3000 Adam steps, batch 8192).  The truncated model (L = M = 24)
-then runs 300 steps from the last truncated state, without and with the
-closure, beside the resolved run.
+then runs without and with the closure beside the resolved run, over three
+consecutive windows of 300 steps, each started from the truncated resolved
+state.
 
-Bounds (the observed values, on this workload, in brackets):
-  held-out skill over all five terms > 0.15 (0.24), toroidal velocity > 0.4
-  (0.65);
-  after 300 steps, the closure run's relative error against the truncated
-  resolved state in the toroidal velocity and codensity below the
-  fallback's (0.480 vs 0.508, 0.320 vs 0.342), its kinetic-energy error
-  below the fallback's (1.39e4 vs 2.11e4), and its kinetic-energy spectrum
-  nearer the resolved one: mean |log(E_l / E_l resolved)| over l = 1..23
-  below 0.95 x the fallback's (0.399 vs 0.480).
+Bounds: held-out skill over all five terms > 0.15, toroidal velocity > 0.4
+(0.24 and 0.65 on this workload); averaged over the three windows, the
+closure run's relative error against the truncated resolved state in the
+toroidal velocity and codensity, its kinetic-energy error, and the mean
+|log(E_l / E_l resolved)| of its kinetic-energy spectrum over l = 1..23 are
+all below the fallback's.  (The resolved run is chaotic, so the individual
+values depend on its trajectory; on one 300-step window the closure
+lowered them from 0.508, 0.342, 2.11e4, 0.480 to 0.480, 0.320, 1.39e4,
+0.399.)
 """
 import time
 
@@ -75,28 +76,33 @@ def test_closure_skill_and_truncated_run():
     assert skill > 0.15
     assert skill_ut > 0.4
 
+    # three consecutive 300-step windows of the resolved run, each started
+    # from its truncated state
     tmap = C.truncation_map(ch, cl)
-    st_lo = C.truncate({k: gh.get_coll(k) for k in C.STATE}, tmap)
-    runs = []
-    for use in (False, True):
-        g = Context(cl)
-        for k, (re, im) in st_lo.items():
-            g.set_coll(k, re, im)
-        C.step_with_closure(g, cl, 300, clo if use else None)
-        runs.append(g)
-    gh.step(300)
-    ref = C.truncate({k: gh.get_coll(k) for k in C.STATE}, tmap)
-    Eh, Elh, _ = gh.energy(0)
-    res = []
-    for g in runs:
-        E, El, _ = g.energy(0)
-        res.append({"uT": _rel(g.get_coll(0), ref[0]), "C": _rel(g.get_coll(2), ref[2]),
-                    "dKE": abs(E - Eh), "spec": float(np.abs(np.log(El[1:cl.L] / Elh[1:cl.L])).mean())})
-    print("closure: fallback", res[0], "closure", res[1], "(%.1f s)" % (time.time() - t0), flush=True)
-    fb, cr = res
+    res = {False: [], True: []}
+    for w in range(3):
+        st_lo = C.truncate({k: gh.get_coll(k) for k in C.STATE}, tmap)
+        runs = {}
+        for use in (False, True):
+            g = Context(cl)
+            for k, (re, im) in st_lo.items():
+                g.set_coll(k, re, im)
+            C.step_with_closure(g, cl, 300, clo if use else None)
+            runs[use] = g
+        gh.step(300)
+        ref = C.truncate({k: gh.get_coll(k) for k in C.STATE}, tmap)
+        Eh, Elh, _ = gh.energy(0)
+        for use, g in runs.items():
+            E, El, _ = g.energy(0)
+            res[use].append({"uT": _rel(g.get_coll(0), ref[0]), "C": _rel(g.get_coll(2), ref[2]),
+                             "dKE": abs(E - Eh), "spec": float(np.abs(np.log(El[1:cl.L] / Elh[1:cl.L])).mean())})
+            g.close()
+    fb = {k: float(np.mean([r[k] for r in res[False]])) for k in res[False][0]}
+    cr = {k: float(np.mean([r[k] for r in res[True]])) for k in res[True][0]}
+    print("closure: fallback", fb, "closure", cr, "(%.1f s)" % (time.time() - t0), flush=True)
     assert cr["uT"] < fb["uT"]
     assert cr["C"] < fb["C"]
     assert cr["dKE"] < fb["dKE"]
-    assert cr["spec"] < 0.95 * fb["spec"]
-    for g in runs + [gh, gl]:
+    assert cr["spec"] < fb["spec"]
+    for g in (gh, gl):
         g.close()
