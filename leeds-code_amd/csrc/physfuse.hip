@@ -17,9 +17,10 @@
 // pad slot per 8 complex values, against bank conflicts of the stride-8
 // Stockham writes.
 //
-// Fourier data are stored [f][n][m][j]: the 16-byte value of row j is one
-// piece of a 128-byte line shared by 8 consecutive row pairs.  Blocks are
-// numbered so that those 8 row pairs are consecutive workgroups on the same
+// Fourier data are stored [f][n][m][j]: the complex value of row j is one
+// piece of a 128-byte line shared by 16 (FP32) or 8 (FP64) consecutive
+// row pairs, and so is each folded output value.  Blocks are numbered so
+// that the row pairs of one line are consecutive workgroups on the same
 // XCD (workgroups go round-robin over the 8 XCDs), which makes them share
 // each line through that XCD's L2.
 #include "fft_lds.h"
@@ -31,10 +32,16 @@ constexpr int PF_THREADS = 512;
 constexpr int PF_EPT = 8;      // 8 sequences x 512 points = one radix-8 butterfly per thread
 constexpr int PF_MAXSEQ = 8;
 
+// Row pairs per group: as many as fill one 128-byte line of a Fourier or
+// folded row with their complex values (16 in FP32, 8 in FP64).
+template <typename R>
+constexpr int PF_GROUP = 128 / (int)sizeof(typename Cx<R>::t);
+
 template <typename R>
 struct PhysArgs {
     FftPlan pl;
-    int N, Th, Th2, M, ngroups, gpr;  // gpr: groups of 8 row pairs per shell
+    int N, Th, Th2, M, ngroups, gpr;  // gpr: groups of PF_GROUP<R> row pairs per shell
+    float inv_m, inv_w, inv_2m;       // 1/M, 1/(Ph - 2M + 1), 1/(2M) for quot()
     R cor, cmag, qc, qx, scale;
     int cod, comp, mag;
     const double* gp;   // [Th] cos(theta)
@@ -44,15 +51,21 @@ struct PhysArgs {
     const typename Cx<R>::t* tw;
 };
 
+// i / d for 0 <= i < 2^21, d > 0, given inv = 1/d rounded to FP32: the true
+// quotient q has q + 1/(2d) <= (i + 1/2)/d <= q + 1 - 1/(2d), and the
+// computed product is within 2^-23 (i + 1/2)/d < 1/(2d) of (i + 1/2)/d, so
+// its floor is q.
+__device__ __forceinline__ int quot(int i, float inv) { return (int)(((float)i + 0.5f) * inv); }
+
 // Packs nb fields of row pair (jp, jm) -- f0, f0+1, f0+2, then f1, f1+1,
 // f1+2 -- into Hermitian-extended complex sequences (row jp real part, row
 // jm imaginary part); the DC imaginary parts are dropped as a c2r transform
 // does.
 template <typename V>
 __device__ void pack_inputs(V* Z, const V* __restrict__ fourier, int f0, int f1, int nb, int sh, int N, int M,
-                            int Th, int Ph, int jp, int jm) {
+                            int Th, int Ph, int jp, int jm, float inv_m, float inv_w) {
     for (int idx = threadIdx.x; idx < nb * M; idx += PF_THREADS) {
-        const int b = idx / M, k = idx - b * M;
+        const int b = quot(idx, inv_m), k = idx - b * M;
         const int fid = b < 3 ? f0 + b : f1 + b - 3;
         const V* X = fourier + (((size_t)fid * N + sh) * M + k) * Th;
         const V A = X[jp], B = X[jm];
@@ -66,7 +79,7 @@ __device__ void pack_inputs(V* Z, const V* __restrict__ fourier, int f0, int f1,
     }
     const int W = Ph - 2 * M + 1;  // zero band k = M .. Ph-M
     for (int idx = threadIdx.x; idx < nb * W; idx += PF_THREADS) {
-        const int b = idx / W, k = M + (idx - b * W);
+        const int b = quot(idx, inv_w), k = M + (idx - b * W);
         Z[lidx<true>(b * Ph + k)] = cx<V>(0, 0);
     }
     __syncthreads();
@@ -74,7 +87,12 @@ __device__ void pack_inputs(V* Z, const V* __restrict__ fourier, int f0, int f1,
 
 // Radix-8 stage of a length-512 transform (512 = 8^3) with every index a
 // compile-time shift: NS = 1, 8, 64 is the Stockham sub-transform size.
-// One butterfly per thread (nf <= 8 sequences x 64 butterflies).
+// One butterfly per thread (nf <= 8 sequences x 64 butterflies).  With one
+// pad slot per 8 values, lidx(x + 8c) = lidx(x) + 9c, so all eight reads
+// and all eight writes of a butterfly are one base address plus
+// compile-time offsets.  Twiddles come from the stage's own table
+// tw[(r-1)*NS + jm] = exp(2 pi i jm r / (8 NS)) (see stage_twiddles), read
+// by consecutive lanes at consecutive entries.
 template <int NS, int SG, typename V>
 __device__ __forceinline__ void stage512(V* Z, int nf, const V* tw) {
     typedef decltype(V().x) S;
@@ -84,13 +102,14 @@ __device__ __forceinline__ void stage512(V* Z, int nf, const V* tw) {
     const int p = b >> 6, j = b & 63, jm = j & (NS - 1);
     V v[8];
     if (on) {
-        const int base = p * 512 + j;
+        const int rb = lidx<true>(p * 512 + j);
 #pragma unroll
-        for (int r = 0; r < 8; r++) v[r] = Z[lidx<true>(base + 64 * r)];
-        if (NS > 1 && jm) {
+        for (int r = 0; r < 8; r++) v[r] = Z[rb + 72 * r];
+        if (NS > 1) {
+            const V* t = tw + (NS == 8 ? 0 : 56) + jm;
 #pragma unroll
             for (int r = 1; r < 8; r++) {
-                V w = tw[jm * r * (64 / NS)];
+                V w = t[(r - 1) * NS];
                 w.y *= sg;
                 v[r] = cmulv(v[r], w);
             }
@@ -99,16 +118,36 @@ __device__ __forceinline__ void stage512(V* Z, int nf, const V* tw) {
     }
     __syncthreads();
     if (on) {
-        const int base = p * 512 + (j / NS) * NS * 8 + jm;
+        const int wb = lidx<true>(p * 512 + (j / NS) * NS * 8 + jm);
+        constexpr int step = NS == 1 ? 1 : (NS == 8 ? 9 : 72);
 #pragma unroll
-        for (int r = 0; r < 8; r++) Z[lidx<true>(base + r * NS)] = v[r];
+        for (int r = 0; r < 8; r++) Z[wb + step * r] = v[r];
     }
     __syncthreads();
 }
 
-template <int SG, typename V>
+// Twiddle tables of the two twiddled stages of a length-512 transform,
+// from the full table exp(2 pi i k / 512): stage NS = 8 at [0, 56),
+// entry (r-1)*8 + jm = w^(8 jm r); stage NS = 64 at [56, 504), entry
+// 56 + (r-1)*64 + jm = w^(jm r).
+template <typename V>
+__device__ void stage_twiddles(V* twl, const V* __restrict__ tw) {
+    for (int i = threadIdx.x; i < 504; i += PF_THREADS) {
+        int k;
+        if (i < 56) {
+            const int r = i / 8 + 1, jm = i & 7;
+            k = 8 * jm * r;
+        } else {
+            const int q = i - 56, r = q / 64 + 1, jm = q & 63;
+            k = jm * r;
+        }
+        twl[i] = tw[k];
+    }
+}
+
+template <int SG, bool R512, typename V>
 __device__ __forceinline__ void fft_pf(V* Z, const FftPlan& pl, int nf, const V* tw) {
-    if (pl.n == 512 && pl.nstage == 3) {  // make_fft_plan gives 8, 8, 8 for 512
+    if constexpr (R512) {
         stage512<1, SG>(Z, nf, tw);
         stage512<8, SG>(Z, nf, tw);
         stage512<64, SG>(Z, nf, tw);
@@ -117,23 +156,43 @@ __device__ __forceinline__ void fft_pf(V* Z, const FftPlan& pl, int nf, const V*
     }
 }
 
-template <typename R, int PP>
+// Length-512 transforms (radices 8, 8, 8) have their own kernel, which
+// needs far fewer registers than the general-length path and a smaller
+// twiddle area (504 entries), so more workgroups fit on a CU.
+constexpr int TW512 = 504;
+
+bool physfuse_r512(int Ph) {
+    const FftPlan pl = make_fft_plan(Ph);
+    return Ph == 512 && pl.nstage == 3 && pl.radix[0] == 8 && pl.radix[1] == 8 && pl.radix[2] == 8;
+}
+
+// LDS of one workgroup: the twiddles, then PF_MAXSEQ padded sequences
+// (one pad slot per 8 values).
+template <typename V>
+size_t physfuse_lds(int Ph, bool r512) {
+    return sizeof(V) * ((r512 ? TW512 : Ph) + (size_t)PF_MAXSEQ * Ph / 8 * 9);
+}
+
+template <typename R, bool R512, int PP>
 __global__ void __launch_bounds__(PF_THREADS) k_physfuse(PhysArgs<R> a) {
     typedef typename Cx<R>::t V;
     extern __shared__ unsigned char lds_raw[];
     // twiddles first (read by every FFT stage), then the padded sequences
     V* twl = reinterpret_cast<V*>(lds_raw);
-    V* Z = twl + a.pl.n;
+    V* Z = twl + (R512 ? TW512 : a.pl.n);
+    constexpr int GS = PF_GROUP<R>;
     const int b = blockIdx.x, xcd = b & 7, q = b >> 3;
-    const int G = (q >> 3) * 8 + xcd, member = q & 7;
+    const int G = (q / GS) * 8 + xcd, member = q % GS;
     if (G >= a.ngroups) return;
     const int sh = G / a.gpr, jg = G - sh * a.gpr;
-    const int jp = jg * 8 + member;
+    const int jp = jg * GS + member;
     if (jp >= a.Th2) return;
     const int jm = a.Th - 1 - jp;
     const int Ph = a.pl.n, N = a.N, M = a.M, Th = a.Th;
     const int t = threadIdx.x;
-    for (int i = t; i < Ph; i += PF_THREADS) twl[i] = a.tw[i];
+    if constexpr (R512) stage_twiddles(twl, a.tw);
+    else
+        for (int i = t; i < Ph; i += PF_THREADS) twl[i] = a.tw[i];
     const R sn = (R)a.sn[jp];
     const R gpr[2] = {(R)a.gp[jp], (R)a.gp[jm]};
 
@@ -141,8 +200,8 @@ __global__ void __launch_bounds__(PF_THREADS) k_physfuse(PhysArgs<R> a) {
 
     // u and curl u: advection and Coriolis
     {
-        pack_inputs(Z, a.fourier, LSD_P_VEL_R, LSD_P_VCURL_R, 6, sh, N, M, Th, Ph, jp, jm);
-        fft_pf<1>(Z, a.pl, 6, twl);
+        pack_inputs(Z, a.fourier, LSD_P_VEL_R, LSD_P_VCURL_R, 6, sh, N, M, Th, Ph, jp, jm, a.inv_m, a.inv_w);
+        fft_pf<1, R512>(Z, a.pl, 6, twl);
 #pragma unroll
         for (int p = 0; p < PP; p++) {
             const int x = t + p * PF_THREADS;
@@ -173,8 +232,8 @@ __global__ void __launch_bounds__(PF_THREADS) k_physfuse(PhysArgs<R> a) {
     }
     // B and curl B: Lorentz force and induction E = u x B
     if (a.mag) {
-        pack_inputs(Z, a.fourier, LSD_P_MAG_R, LSD_P_MCURL_R, 6, sh, N, M, Th, Ph, jp, jm);
-        fft_pf<1>(Z, a.pl, 6, twl);
+        pack_inputs(Z, a.fourier, LSD_P_MAG_R, LSD_P_MCURL_R, 6, sh, N, M, Th, Ph, jp, jm, a.inv_m, a.inv_w);
+        fft_pf<1, R512>(Z, a.pl, 6, twl);
 #pragma unroll
         for (int p = 0; p < PP; p++) {
             const int x = t + p * PF_THREADS;
@@ -200,8 +259,9 @@ __global__ void __launch_bounds__(PF_THREADS) k_physfuse(PhysArgs<R> a) {
     // grad C and grad xi: advection of codensity and composition
     if (a.cod || a.comp) {
         const int nb = 3 * (a.cod + a.comp);
-        pack_inputs(Z, a.fourier, a.cod ? LSD_P_COD_GR : LSD_P_COMP_GR, LSD_P_COMP_GR, nb, sh, N, M, Th, Ph, jp, jm);
-        fft_pf<1>(Z, a.pl, nb, twl);
+        pack_inputs(Z, a.fourier, a.cod ? LSD_P_COD_GR : LSD_P_COMP_GR, LSD_P_COMP_GR, nb, sh, N, M, Th, Ph, jp, jm,
+                    a.inv_m, a.inv_w);
+        fft_pf<1, R512>(Z, a.pl, nb, twl);
         const int oc = 0, ox = a.cod ? 3 : 0;
 #pragma unroll
         for (int p = 0; p < PP; p++) {
@@ -246,11 +306,11 @@ __global__ void __launch_bounds__(PF_THREADS) k_physfuse(PhysArgs<R> a) {
         }
     }
     __syncthreads();
-    fft_pf<-1>(Z, a.pl, nout, twl);
+    fft_pf<-1, R512>(Z, a.pl, nout, twl);
     // separate the two rows and fold: G0 = X_j + X_mirror, G1 = X_j - X_mirror
     const size_t fstride = (size_t)N * M * 2 * a.Th2;
     for (int idx = t; idx < nout * M * 2; idx += PF_THREADS) {
-        const int s = idx / (2 * M), rem = idx - s * 2 * M;
+        const int s = quot(idx, a.inv_2m), rem = idx - s * 2 * M;
         const int k = rem >> 1, par = rem & 1;
         const V zk = Z[lidx<true>(s * Ph + k)];
         V zc = Z[lidx<true>(s * Ph + (k ? Ph - k : 0))];
@@ -276,7 +336,10 @@ static int physfuse_launch(Ctx& c, const void* fourier, void* folded, const void
     a.Th = c.Th;
     a.Th2 = c.Th2;
     a.M = c.M;
-    a.gpr = (c.Th2 + 7) / 8;
+    a.gpr = (c.Th2 + PF_GROUP<R> - 1) / PF_GROUP<R>;
+    a.inv_m = 1.0f / (float)c.M;
+    a.inv_w = 1.0f / (float)(c.Ph - 2 * c.M + 1);
+    a.inv_2m = 1.0f / (float)(2 * c.M);
     a.ngroups = c.N * a.gpr;
     a.cor = (R)(2.0 * k.Pm / k.E);
     a.cmag = (R)(2.0 * k.Pm / k.E);
@@ -291,14 +354,16 @@ static int physfuse_launch(Ctx& c, const void* fourier, void* folded, const void
     a.fourier = reinterpret_cast<const V*>(fourier);
     a.folded = reinterpret_cast<V*>(folded);
     a.tw = reinterpret_cast<const V*>(tw);
-    const int nblk = ((a.ngroups + 7) / 8) * 64;
-    const size_t lds = sizeof(V) * (c.Ph + PF_MAXSEQ * c.Ph + PF_MAXSEQ * c.Ph / 8 + 1);
+    const int nblk = ((a.ngroups + 7) / 8) * 8 * PF_GROUP<R>;
+    const bool r512 = physfuse_r512(c.Ph);
+    const size_t lds = physfuse_lds<V>(c.Ph, r512);
     const int nin = 6 + (k.mag ? 6 : 0) + (k.cod ? 3 : 0) + (k.comp ? 3 : 0);
     const int nout = 3 + (k.mag ? 3 : 0) + (k.cod ? 1 : 0) + (k.comp ? 1 : 0);
     // algorithmic bytes: Fourier fields in, folded fields out
     ProfScope ps(c, LSD_K_PHYS,
                  (double)sizeof(V) * c.N * c.M * ((double)nin * c.Th + (double)nout * 2 * c.Th2));
-    hipLaunchKernelGGL((k_physfuse<R, 1>), dim3(nblk), dim3(PF_THREADS), lds, c.stream, a);
+    if (r512) hipLaunchKernelGGL((k_physfuse<R, true, 1>), dim3(nblk), dim3(PF_THREADS), lds, c.stream, a);
+    else hipLaunchKernelGGL((k_physfuse<R, false, 1>), dim3(nblk), dim3(PF_THREADS), lds, c.stream, a);
     LSD_HIP(hipGetLastError());
     return LSD_OK;
 }

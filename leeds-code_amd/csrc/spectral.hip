@@ -2,8 +2,8 @@
 // radial stencils of var_coll_meshmult (modules/variables.F90:476-505),
 // the toroidal/poloidal -> qst map, the qst curl, and the r/(l(l+1))
 // r-hat.curl / r-hat.curl.curl projections of the nonlinear terms.
-// One work-item per (harmonic, radial point); neighbours along r come from
-// the same contiguous column so a wave reads one coalesced run.
+// One work-item per (harmonic, radial point); a workgroup holds whole
+// columns, whose stencil inputs it stages in LDS.
 #include "lsd_internal.h"
 
 namespace lsd {
@@ -49,62 +49,116 @@ __device__ __forceinline__ double2 bandrow_t(const double* __restrict__ band, co
     return acc;
 }
 
+// Column blocking: a workgroup takes hb = max(1, 512 / N) whole columns
+// (one work-item per radial point) and stages in LDS the columns its
+// stencils read, and the band, so each stencil reads LDS rather than
+// re-fetching its neighbours through the caches, and an inner stencil is
+// computed once per point instead of once per use.
+__host__ __device__ inline int spec_hb(int N) { return N >= 512 ? 1 : 512 / N; }
+
+// Copies the N x 7 band into LDS.
+__device__ __forceinline__ void stage_band(double* dst, const double* __restrict__ band, int N) {
+    for (int i = threadIdx.x; i < N * 7; i += blockDim.x) dst[i] = band[i];
+}
+
+// Row n of A.x with the band and the column in LDS; contributions in
+// ascending j, as var_coll_meshmult accumulates them.
+__device__ __forceinline__ double2 bandrow_l(const double* band, const double2* x, int n, int N) {
+    const int lo = n - LSD_KL > 0 ? n - LSD_KL : 0;
+    const int hi = n + LSD_KL < N - 1 ? n + LSD_KL : N - 1;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int j = lo; j <= hi; j++) {
+        const double a = band[j * 7 + (LSD_KL + n - j)];
+        acc.x = fma(a, x[j].x, acc.x);
+        acc.y = fma(a, x[j].y, acc.y);
+    }
+    return acc;
+}
+
 // var_coll_TorPol2qst_mag (variables.F90:565-605) followed by
 // var_coll_qstcurl (variables.F90:713-754).
 template <typename TO>
-__global__ void k_vec_prep(int H, int N, const int* __restrict__ hl, const double* __restrict__ r,
-                           const double* __restrict__ r1dr, const double* __restrict__ ll,
-                           const double2* __restrict__ T, const double2* __restrict__ P, TO* __restrict__ q,
-                           TO* __restrict__ s, TO* __restrict__ t, TO* __restrict__ oq,
-                           TO* __restrict__ os, TO* __restrict__ ot, int L) {
-    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (size_t)H * N) return;
-    int h = (int)(idx / N), n = (int)(idx - (size_t)h * N);
-    int l = hl[h];
+__global__ void __launch_bounds__(1024) k_vec_prep(int H, int N, const int* __restrict__ hl,
+                                                  const double* __restrict__ r, const double* __restrict__ r1dr,
+                                                  const double* __restrict__ ll, const double2* __restrict__ T,
+                                                  const double2* __restrict__ P, TO* __restrict__ q,
+                                                  TO* __restrict__ s, TO* __restrict__ t, TO* __restrict__ oq,
+                                                  TO* __restrict__ os, TO* __restrict__ ot, int L) {
+    extern __shared__ double2 sm[];
+    const int hb = spec_hb(N), nc = hb * N;
+    double2* Ps = sm;            // [hb][N]
+    double2* Ts = sm + nc;       // [hb][N]  -T rt
+    double2* Ss = sm + 2 * nc;   // [hb][N]  s = rt r1dr.P
+    double* band = reinterpret_cast<double*>(sm + 3 * nc);
+    const int c = threadIdx.x / N, n = threadIdx.x - c * N;
+    const int h = blockIdx.x * hb + c;
+    const bool act = threadIdx.x < nc && h < H;
+    const size_t o = (size_t)h * N + n;
+    const int l = act ? hl[h] : 0;
+    const double rt = ll[2 * L + l];
+    stage_band(band, r1dr, N);
+    if (act) {
+        Ps[threadIdx.x] = P[o];
+        const double2 tv = T[o];
+        Ts[threadIdx.x] = make_double2(-tv.x * rt, -tv.y * rt);
+    }
+    __syncthreads();
+    const double2* Pc = Ps + c * N;
+    if (act) {
+        double2 sj = bandrow_l(band, Pc, n, N);
+        Ss[threadIdx.x] = make_double2(sj.x * rt, sj.y * rt);
+    }
+    __syncthreads();
+    if (!act) return;
     const double* rinv = r + 2 * N;
-    double ll1 = ll[l], rt = ll[2 * L + l];
-    const double2* Pc = P + (size_t)h * N;
-    const double2* Tc = T + (size_t)h * N;
-    double a = ll1 * rinv[n];
-    double2 qn = make_double2(Pc[n].x * a, Pc[n].y * a);
-    // s at the stencil of n, t likewise
-    int lo = n - LSD_KL > 0 ? n - LSD_KL : 0;
-    int hi = n + LSD_KL < N - 1 ? n + LSD_KL : N - 1;
-    double2 otn = make_double2(0.0, 0.0), osn = make_double2(0.0, 0.0), sn = make_double2(0.0, 0.0);
+    const double ll1 = ll[l];
+    const double a = ll1 * rinv[n];
+    const double2 pn = Pc[n];
+    const double2 qn = make_double2(pn.x * a, pn.y * a);
+    const double2* Sc = Ss + c * N;
+    const double2* Tc = Ts + c * N;
+    const int lo = n - LSD_KL > 0 ? n - LSD_KL : 0;
+    const int hi = n + LSD_KL < N - 1 ? n + LSD_KL : N - 1;
+    double2 otn = make_double2(0.0, 0.0), osn = make_double2(0.0, 0.0);
     for (int j = lo; j <= hi; j++) {
-        double2 sj = bandrow(r1dr, Pc, j, N);
-        sj.x = sj.x * rt;
-        sj.y = sj.y * rt;
-        double2 tj = make_double2(-Tc[j].x * rt, -Tc[j].y * rt);
-        double b = r1dr[j * 7 + (LSD_KL + n - j)];
+        const double2 sj = Sc[j], tj = Tc[j];
+        const double b = band[j * 7 + (LSD_KL + n - j)];
         otn.x = fma(b, sj.x, otn.x);
         otn.y = fma(b, sj.y, otn.y);
         osn.x = fma(b, tj.x, osn.x);
         osn.y = fma(b, tj.y, osn.y);
-        if (j == n) sn = sj;
     }
-    double2 tn = make_double2(-Tc[n].x * rt, -Tc[n].y * rt);
-    double c = rt * rinv[n];
-    size_t o = (size_t)h * N + n;
+    const double2 sn = Sc[n], tn = Tc[n];
+    const double cc = rt * rinv[n];
     put(q + o, qn);
     put(s + o, sn);
     put(t + o, tn);
-    put(ot + o, make_double2(otn.x - c * qn.x, otn.y - c * qn.y));
+    put(ot + o, make_double2(otn.x - cc * qn.x, otn.y - cc * qn.y));
     put(os + o, make_double2(-osn.x, -osn.y));
-    double d = -rt * rinv[n];
+    const double d = -rt * rinv[n];
     put(oq + o, make_double2(tn.x * d, tn.y * d));
 }
 
 // dC/dr by the first-derivative stencil (codensity.F90:207); Cc, when
 // given, receives C itself in the transform precision.
 template <typename TO>
-__global__ void k_scal_prep(int H, int N, const double* __restrict__ dr1, const double2* __restrict__ C,
-                            TO* __restrict__ dC, TO* __restrict__ Cc) {
-    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (size_t)H * N) return;
-    int h = (int)(idx / N), n = (int)(idx - (size_t)h * N);
-    put(dC + idx, bandrow(dr1, C + (size_t)h * N, n, N));
-    if (Cc) put(Cc + idx, C[idx]);
+__global__ void __launch_bounds__(1024) k_scal_prep(int H, int N, const double* __restrict__ dr1,
+                                                   const double2* __restrict__ C, TO* __restrict__ dC,
+                                                   TO* __restrict__ Cc) {
+    extern __shared__ double2 sm[];
+    const int hb = spec_hb(N), nc = hb * N;
+    double2* Cs = sm;
+    double* band = reinterpret_cast<double*>(sm + nc);
+    const int c = threadIdx.x / N, n = threadIdx.x - c * N;
+    const int h = blockIdx.x * hb + c;
+    const bool act = threadIdx.x < nc && h < H;
+    const size_t o = (size_t)h * N + n;
+    stage_band(band, dr1, N);
+    if (act) Cs[threadIdx.x] = C[o];
+    __syncthreads();
+    if (!act) return;
+    put(dC + o, bandrow_l(band, Cs + c * N, n, N));
+    if (Cc) put(Cc + o, Cs[threadIdx.x]);
 }
 
 struct PostParams {
@@ -117,34 +171,47 @@ struct PostParams {
 // 432-471): buoyancy on q, r/(l(l+1)) r.curl and r.curl.curl projections,
 // l = 0 zeroed, sources on the l = 0 scalar modes.
 template <typename TI>
-__global__ void k_spec_post(int H, int N, int L, const int* __restrict__ hl, const double* __restrict__ r,
-                            const double* __restrict__ r1dr, const double* __restrict__ ll, PostParams pp,
-                            const TI* __restrict__ so, const double2* __restrict__ Cst,
-                            const double2* __restrict__ Xst, double2* __restrict__ NuT, double2* __restrict__ NuP,
-                            double2* __restrict__ NC, double2* __restrict__ NX, double2* __restrict__ NbT,
-                            double2* __restrict__ NbP) {
-    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t HN = (size_t)H * N;
-    if (idx >= HN) return;
-    int h = (int)(idx / N), n = (int)(idx - (size_t)h * N);
-    int l = hl[h];
+__global__ void __launch_bounds__(1024) k_spec_post(int H, int N, int L, const int* __restrict__ hl,
+                                                   const double* __restrict__ r, const double* __restrict__ r1dr,
+                                                   const double* __restrict__ ll, PostParams pp,
+                                                   const TI* __restrict__ so, const double2* __restrict__ Cst,
+                                                   const double2* __restrict__ Xst, double2* __restrict__ NuT,
+                                                   double2* __restrict__ NuP, double2* __restrict__ NC,
+                                                   double2* __restrict__ NX, double2* __restrict__ NbT,
+                                                   double2* __restrict__ NbP) {
+    extern __shared__ double2 sm[];
+    const int hb = spec_hb(N), nc = hb * N;
+    double2* Us = sm;            // analysis s of u
+    double2* Es = sm + nc;       // analysis s of E (magnetic)
+    double* band = reinterpret_cast<double*>(sm + 2 * nc);
+    const int c = threadIdx.x / N, n = threadIdx.x - c * N;
+    const int h = blockIdx.x * hb + c;
+    const bool act = threadIdx.x < nc && h < H;
+    const size_t HN = (size_t)H * N;
+    const size_t idx = (size_t)h * N + n;
+    stage_band(band, r1dr, N);
+    if (act) {
+        Us[threadIdx.x] = get(so + SO_US * HN + idx);
+        if (pp.mag) Es[threadIdx.x] = get(so + SO_ES * HN + idx);
+    }
+    __syncthreads();
+    if (!act) return;
+    const int l = hl[h];
     const double* rinv = r + 2 * N;
-    double rn = r[4 * N + n];
-    double rt11 = ll[3 * L + l];
-    const TI* uq = so + SO_UQ * HN;
-    const TI* us = so + SO_US * HN + (size_t)h * N;
-    const TI* ut = so + SO_UT * HN;
+    const double rn = r[4 * N + n];
+    const double rt11 = ll[3 * L + l];
     // buoyancy (nonlinear.F90:158-185)
-    double2 cq = get(uq + idx);
+    double2 cq = get(so + SO_UQ * HN + idx);
     double d2 = pp.Ra * rn * pp.Pm2 / pp.Pr;
-    cq.x = cq.x + d2 * Cst[idx].x;
-    cq.y = cq.y + d2 * Cst[idx].y;
+    const double2 cs = Cst[idx], xs = Xst[idx];
+    cq.x = cq.x + d2 * cs.x;
+    cq.y = cq.y + d2 * cs.y;
     d2 = pp.Ra_comp * rn * pp.Pm2 / pp.Sc;
-    cq.x = cq.x + d2 * Xst[idx].x;
-    cq.y = cq.y + d2 * Xst[idx].y;
-    const double2 utv = get(ut + idx);
+    cq.x = cq.x + d2 * xs.x;
+    cq.y = cq.y + d2 * xs.y;
+    const double2 utv = get(so + SO_UT * HN + idx);
     double2 nt = make_double2(-utv.x * rt11, -utv.y * rt11);
-    double2 v = bandrow_t(r1dr, us, n, N);
+    double2 v = bandrow_l(band, Us + c * N, n, N);
     v.x = v.x * rt11;
     v.y = v.y * rt11;
     double2 np = make_double2(cq.x * rinv[n] - v.x, cq.y * rinv[n] - v.y);
@@ -152,22 +219,19 @@ __global__ void k_spec_post(int H, int N, int L, const int* __restrict__ hl, con
     NuT[idx] = nt;
     NuP[idx] = np;
     if (pp.cod) {
-        double2 c = get(so + SO_C * HN + idx);
-        if (h == 0) c.x = c.x + pp.src_c;
-        NC[idx] = c;
+        double2 cv = get(so + SO_C * HN + idx);
+        if (h == 0) cv.x = cv.x + pp.src_c;
+        NC[idx] = cv;
     }
     if (pp.comp) {
-        double2 c = get(so + SO_X * HN + idx);
-        if (h == 0) c.x = c.x + pp.src_x;
-        NX[idx] = c;
+        double2 cv = get(so + SO_X * HN + idx);
+        if (h == 0) cv.x = cv.x + pp.src_x;
+        NX[idx] = cv;
     }
     if (pp.mag) {
-        const TI* eq = so + SO_EQ * HN;
-        const TI* es = so + SO_ES * HN + (size_t)h * N;
-        const TI* et = so + SO_ET * HN;
-        const double2 etv = get(et + idx), eqv = get(eq + idx);
+        const double2 etv = get(so + SO_ET * HN + idx), eqv = get(so + SO_EQ * HN + idx);
         double2 bp = make_double2(-etv.x * rt11, -etv.y * rt11);
-        double2 w = bandrow_t(r1dr, es, n, N);
+        double2 w = bandrow_l(band, Es + c * N, n, N);
         w.x = w.x * rt11;
         w.y = w.y * rt11;
         double2 bt = make_double2(eqv.x * rinv[n] - w.x, eqv.y * rinv[n] - w.y);
@@ -198,11 +262,19 @@ __global__ void k_coll2spec(int H, int N, const double2* __restrict__ in, double
 
 static dim3 grid1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
+// Column-blocked launch shape of the spectral kernels, and their LDS
+// (ncols staged columns of double2 plus the band).
+static dim3 spec_grid(const Ctx& c) { return dim3((unsigned)((c.H + spec_hb(c.N) - 1) / spec_hb(c.N))); }
+static dim3 spec_block(const Ctx& c) { return dim3((unsigned)(spec_hb(c.N) * c.N)); }
+static size_t spec_lds(const Ctx& c, int ncols) {
+    return sizeof(double2) * (size_t)ncols * spec_hb(c.N) * c.N + sizeof(double) * 7 * (size_t)c.N;
+}
+
 template <typename TO>
 static int vec_prep_t(Ctx& c, const double2* T, const double2* P, TO* const* o) {
     ProfScope ps(c, LSD_K_SPEC, (2.0 * 16.0 + 6.0 * sizeof(TO)) * c.coll_elems());
-    hipLaunchKernelGGL(k_vec_prep<TO>, grid1(c.coll_elems()), dim3(256), 0, c.stream, c.H, c.N, c.d_hl, c.d_r,
-                       c.d_r1dr, c.d_ll, T, P, o[0], o[1], o[2], o[3], o[4], o[5], c.L);
+    hipLaunchKernelGGL(k_vec_prep<TO>, spec_grid(c), spec_block(c), spec_lds(c, 3), c.stream, c.H, c.N, c.d_hl,
+                       c.d_r, c.d_r1dr, c.d_ll, T, P, o[0], o[1], o[2], o[3], o[4], o[5], c.L);
     LSD_HIP(hipGetLastError());
     return LSD_OK;
 }
@@ -223,11 +295,11 @@ int launch_scal_prep(Ctx& c, const double2* C, void* dC, void* Cc, bool f32) {
     const size_t out = f32 ? sizeof(float2) : sizeof(double2);
     ProfScope ps(c, LSD_K_SPEC, (16.0 + out * (Cc ? 2.0 : 1.0)) * c.coll_elems());
     if (f32)
-        hipLaunchKernelGGL(k_scal_prep<float2>, grid1(c.coll_elems()), dim3(256), 0, c.stream, c.H, c.N, c.d_dr1, C,
-                           reinterpret_cast<float2*>(dC), reinterpret_cast<float2*>(Cc));
+        hipLaunchKernelGGL(k_scal_prep<float2>, spec_grid(c), spec_block(c), spec_lds(c, 1), c.stream, c.H, c.N,
+                           c.d_dr1, C, reinterpret_cast<float2*>(dC), reinterpret_cast<float2*>(Cc));
     else
-        hipLaunchKernelGGL(k_scal_prep<double2>, grid1(c.coll_elems()), dim3(256), 0, c.stream, c.H, c.N, c.d_dr1,
-                           C, reinterpret_cast<double2*>(dC), reinterpret_cast<double2*>(Cc));
+        hipLaunchKernelGGL(k_scal_prep<double2>, spec_grid(c), spec_block(c), spec_lds(c, 1), c.stream, c.H, c.N,
+                           c.d_dr1, C, reinterpret_cast<double2*>(dC), reinterpret_cast<double2*>(Cc));
     LSD_HIP(hipGetLastError());
     return LSD_OK;
 }
@@ -251,13 +323,13 @@ int launch_spec_post(Ctx& c, bool f32) {
     pp.comp = k.comp;
     pp.mag = k.mag;
     if (f32)
-        hipLaunchKernelGGL(k_spec_post<float2>, grid1(c.coll_elems()), dim3(256), 0, c.stream, c.H, c.N, c.L, c.d_hl,
-                           c.d_r, c.d_r1dr, c.d_ll, pp, reinterpret_cast<const float2*>(c.d_spec_out),
+        hipLaunchKernelGGL(k_spec_post<float2>, spec_grid(c), spec_block(c), spec_lds(c, 2), c.stream, c.H, c.N, c.L,
+                           c.d_hl, c.d_r, c.d_r1dr, c.d_ll, pp, reinterpret_cast<const float2*>(c.d_spec_out),
                            c.d_f[LSD_COD], c.d_f[LSD_COMP], c.d_f[LSD_N_VEL_TOR], c.d_f[LSD_N_VEL_POL],
                            c.d_f[LSD_N_COD], c.d_f[LSD_N_COMP], c.d_f[LSD_N_MAG_TOR], c.d_f[LSD_N_MAG_POL]);
     else
-        hipLaunchKernelGGL(k_spec_post<double2>, grid1(c.coll_elems()), dim3(256), 0, c.stream, c.H, c.N, c.L,
-                           c.d_hl, c.d_r, c.d_r1dr, c.d_ll, pp, c.d_spec_out, c.d_f[LSD_COD], c.d_f[LSD_COMP],
+        hipLaunchKernelGGL(k_spec_post<double2>, spec_grid(c), spec_block(c), spec_lds(c, 2), c.stream, c.H, c.N,
+                           c.L, c.d_hl, c.d_r, c.d_r1dr, c.d_ll, pp, c.d_spec_out, c.d_f[LSD_COD], c.d_f[LSD_COMP],
                            c.d_f[LSD_N_VEL_TOR], c.d_f[LSD_N_VEL_POL], c.d_f[LSD_N_COD], c.d_f[LSD_N_COMP],
                            c.d_f[LSD_N_MAG_TOR], c.d_f[LSD_N_MAG_POL]);
     LSD_HIP(hipGetLastError());
