@@ -75,19 +75,37 @@ __device__ __forceinline__ T partner(T v) {
     return __shfl_xor(v, 1);
 }
 
+// Up to LG_MAXJ transforms of one kind run as one launch (jobs): they read
+// the same tables, so their column chunks are placed in the same table
+// groups.
+constexpr int LG_MAXJ = 6;
+
+// v[j] for a job index that is the same across the workgroup, as scalar
+// selects (indexing the kernel-argument array directly would copy it to
+// scratch memory).
+template <typename P>
+__device__ __forceinline__ P pickj(const P (&v)[LG_MAXJ], int j) {
+    P r = v[0];
+#pragma unroll
+    for (int k = 1; k < LG_MAXJ; k++)
+        if (j == k) r = v[k];
+    return r;
+}
+
 template <typename T>
 struct SynArgs {
     int N, Th, Th2, M, H;
-    int nth, ncc, ngroups;  // theta chunks, column chunks, nslot * nth
+    int nth, ncc, ngroups;  // theta chunks, column chunks per job, nslot * nth
+    int njob;
     const Slot* slots;
     const int* order;
     const T* tabA;   // [h][Th2]
     const T* tabB;   // second table (vector / gradient)
-    const T* in0;    // coll [h][2N] (re, im interleaved)
-    const T* in1;
     const double* rinv;
-    T* out0;         // fourier [n][M][Th] complex
-    T* out1;
+    const T* in0[LG_MAXJ];  // coll [h][2N] (re, im interleaved)
+    const T* in1[LG_MAXJ];
+    T* out0[LG_MAXJ];       // fourier [n][M][Th] complex
+    T* out1[LG_MAXJ];
 };
 
 // KIND 0: scalar (P);  1: vector (llP_ flipped, llsin1mP);  2: gradient
@@ -97,13 +115,16 @@ __global__ void __launch_bounds__(64) k_syn(SynArgs<T> a) {
     typedef typename Mfma<T>::acc acc_t;
     constexpr int NOUT = KIND == 0 ? 1 : 2;
     constexpr int CB = KIND == 0 ? 2 : 1;
-    const BlockId bid = decode_block(a.ngroups, a.ncc);
+    const BlockId bid = decode_block(a.ngroups, a.ncc * a.njob);
     if (!bid.valid) return;
     const int sl = uniform(a.order[bid.group / a.nth]);
     const Slot s = a.slots[sl];
     const int lane = threadIdx.x, l16 = lane & 15, q4 = lane >> 4;
     const int j0 = (bid.group % a.nth) * 64;
-    const int c0 = bid.cc * 16 * CB;
+    const int job = bid.cc / a.ncc;
+    const int c0 = (bid.cc - job * a.ncc) * 16 * CB;
+    const T* in0 = pickj(a.in0, job);
+    const T* in1 = pickj(a.in1, job);
     const int C2 = 2 * a.N;
     acc_t Y[NOUT][2][CB][4];
 #pragma unroll
@@ -115,53 +136,75 @@ __global__ void __launch_bounds__(64) k_syn(SynArgs<T> a) {
 #pragma unroll
                 for (int jb = 0; jb < 4; jb++) Y[o][p][cb][jb] = acc_t{0, 0, 0, 0};
     const bool odd_col = (l16 & 1);
-    for (int k0 = 0; k0 < s.len; k0 += 8) {
+    // Operands of one step of 8 harmonics (4 per parity): B = table values
+    // at this lane's theta points, A = coefficients at this lane's columns.
+    // The next step's operands are loaded while this step's products run.
+    struct Ops {
+        T ta[2][4], tb[2][4], x[2][CB], t[2][CB];
+    };
+    auto load = [&](Ops& o, int k0) {
 #pragma unroll
         for (int p = 0; p < 2; p++) {
             const int k = k0 + 2 * q4 + p;
             const bool kv = k < s.len;
             const size_t hrow = (size_t)s.h0 + (kv ? k : 0);
-            T ta[4], tb[4];
 #pragma unroll
             for (int jb = 0; jb < 4; jb++) {
                 int j = j0 + 16 * jb + l16;
                 int jc = j < a.Th2 ? j : a.Th2 - 1;
-                ta[jb] = kv ? a.tabA[hrow * a.Th2 + jc] : T(0);
-                tb[jb] = (KIND != 0 && kv) ? a.tabB[hrow * a.Th2 + jc] : T(0);
+                o.ta[p][jb] = kv ? a.tabA[hrow * a.Th2 + jc] : T(0);
+                o.tb[p][jb] = (KIND != 0 && kv) ? a.tabB[hrow * a.Th2 + jc] : T(0);
             }
 #pragma unroll
             for (int cb = 0; cb < CB; cb++) {
                 const int c = c0 + 16 * cb + l16;
                 const bool cv = kv && c < C2;
-                const T x = cv ? a.in0[hrow * C2 + c] : T(0);
+                o.x[p][cb] = cv ? in0[hrow * C2 + c] : T(0);
+                o.t[p][cb] = (KIND == 1 && cv) ? in1[hrow * C2 + c] : T(0);
+            }
+        }
+    };
+    auto step = [&](const Ops& o) {
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+#pragma unroll
+            for (int cb = 0; cb < CB; cb++) {
+                const T x = o.x[p][cb];
                 if (KIND == 0) {
 #pragma unroll
-                    for (int jb = 0; jb < 4; jb++) Y[0][p][cb][jb] = Mfma<T>::run(x, ta[jb], Y[0][p][cb][jb]);
+                    for (int jb = 0; jb < 4; jb++) Y[0][p][cb][jb] = Mfma<T>::run(x, o.ta[p][jb], Y[0][p][cb][jb]);
                 } else if (KIND == 2) {
                     // theta: c * dP (bucket 1-p);  phi: (i c) * mP/sin (bucket p)
                     T xr = partner(x);
                     xr = odd_col ? xr : -xr;
 #pragma unroll
                     for (int jb = 0; jb < 4; jb++) {
-                        Y[0][1 - p][cb][jb] = Mfma<T>::run(x, ta[jb], Y[0][1 - p][cb][jb]);
-                        Y[NOUT - 1][p][cb][jb] = Mfma<T>::run(xr, tb[jb], Y[NOUT - 1][p][cb][jb]);
+                        Y[0][1 - p][cb][jb] = Mfma<T>::run(x, o.ta[p][jb], Y[0][1 - p][cb][jb]);
+                        Y[NOUT - 1][p][cb][jb] = Mfma<T>::run(xr, o.tb[p][jb], Y[NOUT - 1][p][cb][jb]);
                     }
                 } else {
-                    const T t = cv ? a.in1[hrow * C2 + c] : T(0);
+                    const T t = o.t[p][cb];
                     // (-i t): re <- t_im, im <- -t_re ;  (i s): re <- -s_im, im <- s_re
                     T tr = partner(t), sr = partner(x);
                     tr = odd_col ? -tr : tr;
                     sr = odd_col ? sr : -sr;
 #pragma unroll
                     for (int jb = 0; jb < 4; jb++) {
-                        Y[0][1 - p][cb][jb] = Mfma<T>::run(x, ta[jb], Y[0][1 - p][cb][jb]);
-                        Y[0][p][cb][jb] = Mfma<T>::run(tr, tb[jb], Y[0][p][cb][jb]);
-                        Y[NOUT - 1][p][cb][jb] = Mfma<T>::run(sr, tb[jb], Y[NOUT - 1][p][cb][jb]);
-                        Y[NOUT - 1][1 - p][cb][jb] = Mfma<T>::run(t, ta[jb], Y[NOUT - 1][1 - p][cb][jb]);
+                        Y[0][1 - p][cb][jb] = Mfma<T>::run(x, o.ta[p][jb], Y[0][1 - p][cb][jb]);
+                        Y[0][p][cb][jb] = Mfma<T>::run(tr, o.tb[p][jb], Y[0][p][cb][jb]);
+                        Y[NOUT - 1][p][cb][jb] = Mfma<T>::run(sr, o.tb[p][jb], Y[NOUT - 1][p][cb][jb]);
+                        Y[NOUT - 1][1 - p][cb][jb] = Mfma<T>::run(t, o.ta[p][jb], Y[NOUT - 1][1 - p][cb][jb]);
                     }
                 }
             }
         }
+    };
+    Ops cur, nxt;
+    load(cur, 0);
+    for (int k0 = 0; k0 < s.len; k0 += 8) {
+        if (k0 + 8 < s.len) load(nxt, k0 + 8);
+        step(cur);
+        cur = nxt;
     }
     // Output layout of v_mfma_*_16x16x4: the lane holds column j = l16 of
     // rows mrow(q4, r); rows are our columns c (re/im of a shell), columns
@@ -169,7 +212,7 @@ __global__ void __launch_bounds__(64) k_syn(SynArgs<T> a) {
     typedef typename std::conditional<sizeof(T) == 8, double2, float2>::type T2;
 #pragma unroll
     for (int o = 0; o < NOUT; o++) {
-        T2* F = reinterpret_cast<T2*>(o == 0 ? a.out0 : a.out1);
+        T2* F = reinterpret_cast<T2*>(o == 0 ? pickj(a.out0, job) : pickj(a.out1, job));
 #pragma unroll
         for (int cb = 0; cb < CB; cb++) {
 #pragma unroll
@@ -230,7 +273,12 @@ __global__ void __launch_bounds__(64) k_syn(SynArgs<T> a) {
 }
 
 template <typename T>
-static int syn_launch(Ctx& c, int kind, const T* tA, const T* tB, const T* in0, const T* in1, T* out0, T* out1) {
+static int syn_launch(Ctx& c, int kind, int njob, const T* tA, const T* tB, const void* const* in0,
+                      const void* const* in1, void* const* out0, void* const* out1) {
+    if (njob < 1 || njob > LG_MAXJ) {
+        set_error("synthesis: 1..6 transforms per launch");
+        return LSD_E_ARG;
+    }
     SynArgs<T> a;
     a.N = c.N;
     a.Th = c.Th;
@@ -241,16 +289,20 @@ static int syn_launch(Ctx& c, int kind, const T* tA, const T* tB, const T* in0, 
     a.order = c.d_slot_order;
     a.tabA = tA;
     a.tabB = tB;
-    a.in0 = in0;
-    a.in1 = in1;
     a.rinv = c.d_r + 2 * c.N;
-    a.out0 = out0;
-    a.out1 = out1;
+    for (int k = 0; k < LG_MAXJ; k++) {
+        const int q = k < njob ? k : 0;
+        a.in0[k] = reinterpret_cast<const T*>(in0[q]);
+        a.in1[k] = in1 ? reinterpret_cast<const T*>(in1[q]) : nullptr;
+        a.out0[k] = reinterpret_cast<T*>(out0[q]);
+        a.out1[k] = out1 ? reinterpret_cast<T*>(out1[q]) : nullptr;
+    }
     const int cb = kind == SYN_SCALAR ? 2 : 1;
     a.nth = (c.Th2 + 63) / 64;
     a.ncc = (2 * c.N + 16 * cb - 1) / (16 * cb);
     a.ngroups = c.nslot * a.nth;
-    dim3 grid(grid_blocks(a.ngroups, a.ncc));
+    a.njob = njob;
+    dim3 grid(grid_blocks(a.ngroups, a.ncc * njob));
     if (kind == SYN_SCALAR) hipLaunchKernelGGL((k_syn<T, 0>), grid, dim3(64), 0, c.stream, a);
     else if (kind == SYN_VECTOR) hipLaunchKernelGGL((k_syn<T, 1>), grid, dim3(64), 0, c.stream, a);
     else hipLaunchKernelGGL((k_syn<T, 2>), grid, dim3(64), 0, c.stream, a);
@@ -258,23 +310,28 @@ static int syn_launch(Ctx& c, int kind, const T* tA, const T* tB, const T* in0, 
     return LSD_OK;
 }
 
-// kind SYN_SCALAR: in0 -> out0 (table P)
-// kind SYN_VECTOR: (in0 = s, in1 = t) -> (out0 = theta, out1 = phi)
-// kind SYN_GRAD:   in0 -> (out0 = theta, out1 = phi)
+// njob transforms of one kind in one launch:
+// kind SYN_SCALAR: in0[k] -> out0[k] (table P)
+// kind SYN_VECTOR: (in0[k] = s, in1[k] = t) -> (out0[k] = theta, out1[k] = phi)
+// kind SYN_GRAD:   in0[k] -> (out0[k] = theta, out1[k] = phi)
 // f32: FP32 tables, inputs and outputs (float2 spectral / Fourier layouts).
-int launch_synthesis_p(Ctx& c, int kind, const void* in0, const void* in1, void* out0, void* out1, bool f32) {
+int launch_synthesis_group(Ctx& c, int kind, int njob, const void* const* in0, const void* const* in1,
+                           void* const* out0, void* const* out1, bool f32) {
     size_t HT = (size_t)c.H * c.Th2;
     const double unit = 4.0 * (double)c.H * c.Th2 * c.N;  // flops per product unit
-    ProfScope ps(c, LSD_K_SYNTH, unit * (kind == SYN_SCALAR ? 1 : kind == SYN_VECTOR ? 4 : 2));
+    ProfScope ps(c, LSD_K_SYNTH, unit * (kind == SYN_SCALAR ? 1 : kind == SYN_VECTOR ? 4 : 2) * njob);
     const size_t o0 = (kind == SYN_SCALAR ? 0 : kind == SYN_VECTOR ? 3 : 1) * HT;
     const size_t o1 = (kind == SYN_VECTOR ? 4 : 2) * HT;
-    if (f32)
-        return syn_launch<float>(c, kind, c.d_tab32 + o0, c.d_tab32 + o1, reinterpret_cast<const float*>(in0),
-                                 reinterpret_cast<const float*>(in1), reinterpret_cast<float*>(out0),
-                                 reinterpret_cast<float*>(out1));
-    return syn_launch<double>(c, kind, c.d_tab + o0, c.d_tab + o1, reinterpret_cast<const double*>(in0),
-                              reinterpret_cast<const double*>(in1), reinterpret_cast<double*>(out0),
-                              reinterpret_cast<double*>(out1));
+    if (f32) return syn_launch<float>(c, kind, njob, c.d_tab32 + o0, c.d_tab32 + o1, in0, in1, out0, out1);
+    return syn_launch<double>(c, kind, njob, c.d_tab + o0, c.d_tab + o1, in0, in1, out0, out1);
+}
+
+int launch_synthesis_p(Ctx& c, int kind, const void* in0, const void* in1, void* out0, void* out1, bool f32) {
+    const void* i0[1] = {in0};
+    const void* i1[1] = {in1};
+    void* o0[1] = {out0};
+    void* o1[1] = {out1};
+    return launch_synthesis_group(c, kind, 1, i0, in1 ? i1 : nullptr, o0, out1 ? o1 : nullptr, f32);
 }
 
 int launch_synthesis(Ctx& c, int kind, const double2* in0, const double2* in1, double2* out0, double2* out1) {
@@ -286,18 +343,19 @@ int launch_synthesis(Ctx& c, int kind, const double2* in0, const double2* in1, d
 template <typename T>
 struct AnaArgs {
     int N, Th2, M, H;
-    int ncc, ngroups;  // column chunks, slots
+    int ncc, ngroups;  // column chunks per job, slots
+    int njob;
     const Slot* slots;
     const int* order;
     const T* tP;   // gwP        [Th2][H] (parity-grouped h)
     const T* tL;   // gwllP_
     const T* tS;   // gwllsin1mP
-    const T* g1;   // folded [n][M][2][Th2] complex
-    const T* g2;
-    const T* g3;
-    T* o1;         // coll [h][2N]
-    T* o2;
-    T* o3;
+    const T* g1[LG_MAXJ];  // folded [n][M][2][Th2] complex
+    const T* g2[LG_MAXJ];
+    const T* g3[LG_MAXJ];
+    T* o1[LG_MAXJ];        // coll [h][2N]
+    T* o2[LG_MAXJ];
+    T* o3[LG_MAXJ];
 };
 
 constexpr int ANA_JC = 16;     // theta points per LDS chunk
@@ -313,13 +371,16 @@ __global__ void __launch_bounds__(256) k_ana(AnaArgs<T> a) {
     constexpr int NO = VEC ? 3 : 1;
     // LDS: [field][parity][theta][column]
     __shared__ T G[NF][2][ANA_JC][ANA_COLS + 1];
-    const BlockId bid = decode_block(a.ngroups, a.ncc);
+    const BlockId bid = decode_block(a.ngroups, a.ncc * a.njob);
     if (!bid.valid) return;
     const int sl = uniform(a.order[bid.group]);
     const Slot s = a.slots[sl];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int l16 = lane & 15, q4 = lane >> 4;
-    const int c0 = bid.cc * ANA_COLS;
+    const int job = bid.cc / a.ncc;
+    const int c0 = (bid.cc - job * a.ncc) * ANA_COLS;
+    const T* gf[3] = {pickj(a.g1, job), pickj(a.g2, job), pickj(a.g3, job)};
+    T* of[3] = {pickj(a.o1, job), pickj(a.o2, job), pickj(a.o3, job)};
     const int C2 = 2 * a.N;
     const int n_even = (s.len + 1) / 2, n_odd = s.len / 2;
     const int t_even = (n_even + 15) / 16, t_odd = (n_odd + 15) / 16;
@@ -339,7 +400,7 @@ __global__ void __launch_bounds__(256) k_ana(AnaArgs<T> a) {
             const bool ok = n < a.N && j < a.Th2;
 #pragma unroll
             for (int f = 0; f < NF; f++) {
-                const T* g = f == 0 ? a.g1 : (f == 1 ? a.g2 : a.g3);
+                const T* g = gf[f];
 #pragma unroll
                 for (int p = 0; p < 2; p++) {
                     T re = T(0), im = T(0);
@@ -414,11 +475,11 @@ __global__ void __launch_bounds__(256) k_ana(AnaArgs<T> a) {
                 const bool zero_im = (s.m == 0) && (c & 1);
                 const size_t o = h * C2 + c;
                 if constexpr (VEC) {
-                    a.o1[o] = acc[t][0][cb][r];  // the reference leaves q%Im(m=0) as summed
-                    a.o2[o] = zero_im ? T(0) : acc[t][1][cb][r];
-                    a.o3[o] = zero_im ? T(0) : acc[t][2][cb][r];
+                    of[0][o] = acc[t][0][cb][r];  // the reference leaves q%Im(m=0) as summed
+                    of[1][o] = zero_im ? T(0) : acc[t][1][cb][r];
+                    of[2][o] = zero_im ? T(0) : acc[t][2][cb][r];
                 } else {
-                    a.o1[o] = zero_im ? T(0) : acc[t][0][cb][r];
+                    of[0][o] = zero_im ? T(0) : acc[t][0][cb][r];
                 }
             }
         }
@@ -426,14 +487,19 @@ __global__ void __launch_bounds__(256) k_ana(AnaArgs<T> a) {
 }
 
 template <typename T>
-static int ana_launch(Ctx& c, int kind, const T* tP, const T* tL, const T* tS, const T* g0, const T* g1,
-                      const T* g2, T* o0, T* o1, T* o2) {
+static int ana_launch(Ctx& c, int kind, int njob, const T* tP, const T* tL, const T* tS, const void* const* g0,
+                      const void* const* g1, const void* const* g2, void* const* o0, void* const* o1,
+                      void* const* o2) {
     int maxlen = 0;
     for (auto& s : c.slots) maxlen = maxlen > s.len ? maxlen : s.len;
     int tiles = ((maxlen + 1) / 2 + 15) / 16 + (maxlen / 2 + 15) / 16;
     if (tiles > 4 * ANA_TPW) {
         set_error("analysis: L too large for the tile plan");
         return LSD_E_CONFIG;
+    }
+    if (njob < 1 || njob > LG_MAXJ) {
+        set_error("analysis: 1..6 transforms per launch");
+        return LSD_E_ARG;
     }
     AnaArgs<T> a;
     a.N = c.N;
@@ -445,35 +511,49 @@ static int ana_launch(Ctx& c, int kind, const T* tP, const T* tL, const T* tS, c
     a.tP = tP;
     a.tL = tL;
     a.tS = tS;
-    a.g1 = g0;
-    a.g2 = g1;
-    a.g3 = g2;
-    a.o1 = o0;
-    a.o2 = o1;
-    a.o3 = o2;
+    for (int k = 0; k < LG_MAXJ; k++) {
+        const int q = k < njob ? k : 0;
+        a.g1[k] = reinterpret_cast<const T*>(g0[q]);
+        a.g2[k] = g1 ? reinterpret_cast<const T*>(g1[q]) : nullptr;
+        a.g3[k] = g2 ? reinterpret_cast<const T*>(g2[q]) : nullptr;
+        a.o1[k] = reinterpret_cast<T*>(o0[q]);
+        a.o2[k] = o1 ? reinterpret_cast<T*>(o1[q]) : nullptr;
+        a.o3[k] = o2 ? reinterpret_cast<T*>(o2[q]) : nullptr;
+    }
     a.ncc = (2 * c.N + ANA_COLS - 1) / ANA_COLS;
     a.ngroups = c.nslot;
-    dim3 grid(grid_blocks(a.ngroups, a.ncc));
+    a.njob = njob;
+    dim3 grid(grid_blocks(a.ngroups, a.ncc * njob));
     if (kind == ANA_SCALAR) hipLaunchKernelGGL((k_ana<T, false>), grid, dim3(256), 0, c.stream, a);
     else hipLaunchKernelGGL((k_ana<T, true>), grid, dim3(256), 0, c.stream, a);
     LSD_HIP(hipGetLastError());
     return LSD_OK;
 }
 
-int launch_analysis_p(Ctx& c, int kind, const void* g0, const void* g1, const void* g2, void* o0, void* o1,
-                      void* o2, bool f32) {
+// njob analyses of one kind in one launch: ANA_VECTOR (g0, g1, g2)[k] ->
+// (o0, o1, o2)[k] = q, s, t; ANA_SCALAR g0[k] -> o0[k].
+int launch_analysis_group(Ctx& c, int kind, int njob, const void* const* g0, const void* const* g1,
+                          const void* const* g2, void* const* o0, void* const* o1, void* const* o2, bool f32) {
     size_t HT = (size_t)c.H * c.Th2;
     const double unit = 4.0 * (double)c.H * c.Th2 * c.N;
-    ProfScope ps(c, LSD_K_ANAL, unit * (kind == ANA_SCALAR ? 1 : 5));
+    ProfScope ps(c, LSD_K_ANAL, unit * (kind == ANA_SCALAR ? 1 : 5) * njob);
     if (f32)
-        return ana_launch<float>(c, kind, c.d_atab32, c.d_atab32 + HT, c.d_atab32 + 2 * HT,
-                                 reinterpret_cast<const float*>(g0), reinterpret_cast<const float*>(g1),
-                                 reinterpret_cast<const float*>(g2), reinterpret_cast<float*>(o0),
-                                 reinterpret_cast<float*>(o1), reinterpret_cast<float*>(o2));
-    return ana_launch<double>(c, kind, c.d_atab, c.d_atab + HT, c.d_atab + 2 * HT,
-                              reinterpret_cast<const double*>(g0), reinterpret_cast<const double*>(g1),
-                              reinterpret_cast<const double*>(g2), reinterpret_cast<double*>(o0),
-                              reinterpret_cast<double*>(o1), reinterpret_cast<double*>(o2));
+        return ana_launch<float>(c, kind, njob, c.d_atab32, c.d_atab32 + HT, c.d_atab32 + 2 * HT, g0, g1, g2, o0,
+                                 o1, o2);
+    return ana_launch<double>(c, kind, njob, c.d_atab, c.d_atab + HT, c.d_atab + 2 * HT, g0, g1, g2, o0, o1, o2);
+}
+
+int launch_analysis_p(Ctx& c, int kind, const void* g0, const void* g1, const void* g2, void* o0, void* o1,
+                      void* o2, bool f32) {
+    const void* a0[1] = {g0};
+    const void* a1[1] = {g1};
+    const void* a2[1] = {g2};
+    void* b0[1] = {o0};
+    void* b1[1] = {o1};
+    void* b2[1] = {o2};
+    const bool v = kind == ANA_VECTOR;
+    return launch_analysis_group(c, kind, 1, a0, v ? a1 : nullptr, v ? a2 : nullptr, b0, v ? b1 : nullptr,
+                                 v ? b2 : nullptr, f32);
 }
 
 int launch_analysis(Ctx& c, int kind, const double2* g0, const double2* g1, const double2* g2, double2* o0,

@@ -324,46 +324,65 @@ static void* fourier_p(Ctx& c, int k, bool f) { return slot_p(c.d_fourier, c.fou
 static void* folded_p(Ctx& c, int k, bool f) { return slot_p(c.d_folded, c.folded_elems(), k, f); }
 
 // Synthesis of the fields the nonlinear terms need (vel_transform,
-// mag_transform, cod_transform, comp_transform), with the c2r FFTs to the
-// physical buffers when `to_phys` (FP64 only).
+// mag_transform, cod_transform, comp_transform): the spectral preparations
+// first, then one launch per transform kind over all the fields of that
+// kind, then (when `to_phys`, FP64 only) the c2r FFTs to the physical
+// buffers.
 static int synthesise(Ctx& c, bool f, bool to_phys) {
-    const bool fft = to_phys && !f;
-    {
-        void* o[6] = {spec_in_p(c, SI_UQ, f), spec_in_p(c, SI_US, f), spec_in_p(c, SI_UT, f),
-                      spec_in_p(c, SI_WQ, f), spec_in_p(c, SI_WS, f), spec_in_p(c, SI_WT, f)};
-        RC(launch_vec_prep(c, c.d_f[LSD_VEL_TOR], c.d_f[LSD_VEL_POL], o, f));
-        RC(launch_synthesis_p(c, SYN_SCALAR, o[0], nullptr, fourier_p(c, LSD_P_VEL_R, f), nullptr, f));
-        RC(launch_synthesis_p(c, SYN_VECTOR, o[1], o[2], fourier_p(c, LSD_P_VEL_T, f), fourier_p(c, LSD_P_VEL_P, f),
-                              f));
-        RC(launch_synthesis_p(c, SYN_SCALAR, o[3], nullptr, fourier_p(c, LSD_P_VCURL_R, f), nullptr, f));
-        RC(launch_synthesis_p(c, SYN_VECTOR, o[4], o[5], fourier_p(c, LSD_P_VCURL_T, f),
-                              fourier_p(c, LSD_P_VCURL_P, f), f));
-        if (fft) RC(launch_fft_c2r(c, fourier(c, 0), phys(c, 0), 6));
-    }
-    if (c.cfg.mag) {
-        void* o[6] = {spec_in_p(c, SI_BQ, f), spec_in_p(c, SI_BS, f), spec_in_p(c, SI_BT, f),
-                      spec_in_p(c, SI_JQ, f), spec_in_p(c, SI_JS, f), spec_in_p(c, SI_JT, f)};
-        RC(launch_vec_prep(c, c.d_f[LSD_MAG_TOR], c.d_f[LSD_MAG_POL], o, f));
-        RC(launch_synthesis_p(c, SYN_SCALAR, o[0], nullptr, fourier_p(c, LSD_P_MAG_R, f), nullptr, f));
-        RC(launch_synthesis_p(c, SYN_VECTOR, o[1], o[2], fourier_p(c, LSD_P_MAG_T, f), fourier_p(c, LSD_P_MAG_P, f),
-                              f));
-        RC(launch_synthesis_p(c, SYN_SCALAR, o[3], nullptr, fourier_p(c, LSD_P_MCURL_R, f), nullptr, f));
-        RC(launch_synthesis_p(c, SYN_VECTOR, o[4], o[5], fourier_p(c, LSD_P_MCURL_T, f),
-                              fourier_p(c, LSD_P_MCURL_P, f), f));
-        if (fft) RC(launch_fft_c2r(c, fourier(c, 6), phys(c, 6), 6));
-    }
-    // scalars: radial derivative, then (dC/dr) P and the angular gradient
-    const int sf[2] = {LSD_COD, LSD_COMP}, sd[2] = {SI_DC, SI_DX}, sc[2] = {SI_CC, SI_XC};
+    const void* sc_in[LG_SCALARS];
+    void* sc_out[LG_SCALARS];
+    const void* v_s[4];
+    const void* v_t[4];
+    void* v_th[4];
+    void* v_ph[4];
+    const void* g_in[2];
+    void* g_th[2];
+    void* g_ph[2];
+    int ns = 0, nv = 0, ng = 0;
+    auto vec = [&](int T, int P, int si, int pr) {
+        void* o[6] = {spec_in_p(c, si, f), spec_in_p(c, si + 1, f), spec_in_p(c, si + 2, f),
+                      spec_in_p(c, si + 3, f), spec_in_p(c, si + 4, f), spec_in_p(c, si + 5, f)};
+        int rc = launch_vec_prep(c, c.d_f[T], c.d_f[P], o, f);
+        // q of the field and of its curl (scalar), s and t of both (vector)
+        sc_in[ns] = o[0];
+        sc_out[ns++] = fourier_p(c, pr, f);
+        sc_in[ns] = o[3];
+        sc_out[ns++] = fourier_p(c, pr + 3, f);
+        v_s[nv] = o[1];
+        v_t[nv] = o[2];
+        v_th[nv] = fourier_p(c, pr + 1, f);
+        v_ph[nv++] = fourier_p(c, pr + 2, f);
+        v_s[nv] = o[4];
+        v_t[nv] = o[5];
+        v_th[nv] = fourier_p(c, pr + 4, f);
+        v_ph[nv++] = fourier_p(c, pr + 5, f);
+        return rc;
+    };
+    RC(vec(LSD_VEL_TOR, LSD_VEL_POL, SI_UQ, LSD_P_VEL_R));
+    if (c.cfg.mag) RC(vec(LSD_MAG_TOR, LSD_MAG_POL, SI_BQ, LSD_P_MAG_R));
+    // scalars: radial derivative (scalar synthesis), the field itself
+    // (angular gradient)
+    const int sf[2] = {LSD_COD, LSD_COMP}, sd[2] = {SI_DC, SI_DX}, scc[2] = {SI_CC, SI_XC};
     const int pr[2] = {LSD_P_COD_GR, LSD_P_COMP_GR};
     for (int q = 0; q < 2; q++) {
         if (!(q == 0 ? c.cfg.cod : c.cfg.comp)) continue;
         void* dC = spec_in_p(c, sd[q], f);
-        void* Cc = f ? spec_in_p(c, sc[q], f) : nullptr;
+        void* Cc = f ? spec_in_p(c, scc[q], f) : nullptr;
         RC(launch_scal_prep(c, c.d_f[sf[q]], dC, Cc, f));
-        RC(launch_synthesis_p(c, SYN_SCALAR, dC, nullptr, fourier_p(c, pr[q], f), nullptr, f));
-        RC(launch_synthesis_p(c, SYN_GRAD, f ? Cc : (void*)c.d_f[sf[q]], nullptr, fourier_p(c, pr[q] + 1, f),
-                              fourier_p(c, pr[q] + 2, f), f));
-        if (fft) RC(launch_fft_c2r(c, fourier(c, pr[q]), phys(c, pr[q]), 3));
+        sc_in[ns] = dC;
+        sc_out[ns++] = fourier_p(c, pr[q], f);
+        g_in[ng] = f ? Cc : (void*)c.d_f[sf[q]];
+        g_th[ng] = fourier_p(c, pr[q] + 1, f);
+        g_ph[ng++] = fourier_p(c, pr[q] + 2, f);
+    }
+    RC(launch_synthesis_group(c, SYN_SCALAR, ns, sc_in, nullptr, sc_out, nullptr, f));
+    RC(launch_synthesis_group(c, SYN_VECTOR, nv, v_s, v_t, v_th, v_ph, f));
+    if (ng) RC(launch_synthesis_group(c, SYN_GRAD, ng, g_in, nullptr, g_th, g_ph, f));
+    if (to_phys && !f) {
+        RC(launch_fft_c2r(c, fourier(c, 0), phys(c, 0), 6));
+        if (c.cfg.mag) RC(launch_fft_c2r(c, fourier(c, 6), phys(c, 6), 6));
+        for (int q = 0; q < 2; q++)
+            if (q == 0 ? c.cfg.cod : c.cfg.comp) RC(launch_fft_c2r(c, fourier(c, pr[q]), phys(c, pr[q]), 3));
     }
     return LSD_OK;
 }
@@ -375,17 +394,39 @@ static int transform(Ctx& c) { return synthesise(c, false, true); }
 // Analysis of the folded nonlinear products and the coll-side completion of
 // non_velocity, non_codensity, non_composition, non_magnetic.
 static int analyse(Ctx& c, bool f) {
-    RC(launch_analysis_p(c, ANA_VECTOR, folded_p(c, NO_UR, f), folded_p(c, NO_UT, f), folded_p(c, NO_UP, f),
-                         spec_out_p(c, SO_UQ, f), spec_out_p(c, SO_US, f), spec_out_p(c, SO_UT, f), f));
-    if (c.cfg.cod)
-        RC(launch_analysis_p(c, ANA_SCALAR, folded_p(c, NO_C, f), nullptr, nullptr, spec_out_p(c, SO_C, f), nullptr,
-                             nullptr, f));
-    if (c.cfg.comp)
-        RC(launch_analysis_p(c, ANA_SCALAR, folded_p(c, NO_X, f), nullptr, nullptr, spec_out_p(c, SO_X, f), nullptr,
-                             nullptr, f));
-    if (c.cfg.mag)
-        RC(launch_analysis_p(c, ANA_VECTOR, folded_p(c, NO_ER, f), folded_p(c, NO_ET, f), folded_p(c, NO_EP, f),
-                             spec_out_p(c, SO_EQ, f), spec_out_p(c, SO_ES, f), spec_out_p(c, SO_ET, f), f));
+    const void* v0[2];
+    const void* v1[2];
+    const void* v2[2];
+    void* q[2];
+    void* s_[2];
+    void* t[2];
+    const void* sg[2];
+    void* so[2];
+    int nv = 0, ns = 0;
+    v0[nv] = folded_p(c, NO_UR, f);
+    v1[nv] = folded_p(c, NO_UT, f);
+    v2[nv] = folded_p(c, NO_UP, f);
+    q[nv] = spec_out_p(c, SO_UQ, f);
+    s_[nv] = spec_out_p(c, SO_US, f);
+    t[nv++] = spec_out_p(c, SO_UT, f);
+    if (c.cfg.mag) {
+        v0[nv] = folded_p(c, NO_ER, f);
+        v1[nv] = folded_p(c, NO_ET, f);
+        v2[nv] = folded_p(c, NO_EP, f);
+        q[nv] = spec_out_p(c, SO_EQ, f);
+        s_[nv] = spec_out_p(c, SO_ES, f);
+        t[nv++] = spec_out_p(c, SO_ET, f);
+    }
+    if (c.cfg.cod) {
+        sg[ns] = folded_p(c, NO_C, f);
+        so[ns++] = spec_out_p(c, SO_C, f);
+    }
+    if (c.cfg.comp) {
+        sg[ns] = folded_p(c, NO_X, f);
+        so[ns++] = spec_out_p(c, SO_X, f);
+    }
+    RC(launch_analysis_group(c, ANA_VECTOR, nv, v0, v1, v2, q, s_, t, f));
+    if (ns) RC(launch_analysis_group(c, ANA_SCALAR, ns, sg, nullptr, nullptr, so, nullptr, nullptr, f));
     RC(launch_spec_post(c, f));
     return LSD_OK;
 }

@@ -47,6 +47,10 @@ struct Slot {
     int mfft;  // FFT index m_/Mp
 };
 
+// Scalar syntheses of one evaluation at most: q of u, curl u, B, curl B,
+// dC/dr, dxi/dr.
+constexpr int LG_SCALARS = 6;
+
 // Synthesis job kinds.
 enum SynthKind { SYN_SCALAR = 0, SYN_VECTOR = 1, SYN_GRAD = 2 };
 
@@ -193,6 +197,10 @@ int launch_analysis(Ctx& c, int kind, const double2* g0, const double2* g1, cons
                      double2* o0, double2* o1, double2* o2);
 
 int launch_synthesis_p(Ctx& c, int kind, const void* in0, const void* in1, void* out0, void* out1, bool f32);
+int launch_synthesis_group(Ctx& c, int kind, int njob, const void* const* in0, const void* const* in1,
+                           void* const* out0, void* const* out1, bool f32);
+int launch_analysis_group(Ctx& c, int kind, int njob, const void* const* g0, const void* const* g1,
+                          const void* const* g2, void* const* o0, void* const* o1, void* const* o2, bool f32);
 int launch_analysis_p(Ctx& c, int kind, const void* g0, const void* g1, const void* g2, void* o0, void* o1,
                       void* o2, bool f32);
 
