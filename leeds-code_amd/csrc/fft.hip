@@ -16,22 +16,22 @@
 
 namespace lsd {
 
-// fourier [f][n][M][Th] -> phys [f][n][Ph][Th]; grid (row blocks, N, fields).
-__global__ void __launch_bounds__(FFT_THREADS) k_fft_c2r(FftPlan pl, int nf, int N, int Th, int M,
+// fourier [f][n][Th][MP] -> phys [f][n][Ph][Th]; grid (row blocks, N, fields).
+__global__ void __launch_bounds__(FFT_THREADS) k_fft_c2r(FftPlan pl, int nf, int N, int Th, int M, int MP,
                                                          const double2* __restrict__ fourier,
                                                          double* __restrict__ phys, const double2* __restrict__ tw) {
     extern __shared__ double2 Z[];
     const int n = pl.n;
     const int f = blockIdx.z, sh = blockIdx.y;
     const int j0 = blockIdx.x * 2 * nf;
-    const double2* X = fourier + ((size_t)f * N + sh) * (size_t)M * Th;
+    const double2* X = fourier + ((size_t)f * N + sh) * (size_t)Th * MP;
     for (int i = threadIdx.x; i < nf * n; i += FFT_THREADS) Z[i] = make_double2(0.0, 0.0);
     __syncthreads();
     for (int t = threadIdx.x; t < M * nf; t += FFT_THREADS) {
         int k = t / nf, p = t - k * nf;
         int ja = j0 + 2 * p, jb = ja + 1;
-        double2 a = ja < Th ? X[(size_t)k * Th + ja] : make_double2(0.0, 0.0);
-        double2 b = jb < Th ? X[(size_t)k * Th + jb] : make_double2(0.0, 0.0);
+        double2 a = ja < Th ? X[(size_t)ja * MP + k] : make_double2(0.0, 0.0);
+        double2 b = jb < Th ? X[(size_t)jb * MP + k] : make_double2(0.0, 0.0);
         double2* z = Z + (size_t)p * n;
         if (k == 0) {
             z[0] = make_double2(a.x, b.x);
@@ -142,7 +142,7 @@ int launch_fft_c2r(Ctx& c, const double2* fourier, double* phys, int nfield) {
     dim3 grid((c.Th + 2 * nf - 1) / (2 * nf), c.N, nfield);
     size_t lds = sizeof(double2) * nf * c.Ph;
     ProfScope ps(c, LSD_K_FFT_INV, (double)nfield * c.N * ((double)c.M * c.Th * 16 + (double)c.Ph * c.Th * 8));
-    hipLaunchKernelGGL(k_fft_c2r, grid, dim3(FFT_THREADS), lds, c.stream, p, nf, c.N, c.Th, c.M, fourier, phys,
+    hipLaunchKernelGGL(k_fft_c2r, grid, dim3(FFT_THREADS), lds, c.stream, p, nf, c.N, c.Th, c.M, c.MP, fourier, phys,
                        c.d_tw);
     LSD_HIP(hipGetLastError());
     return LSD_OK;

@@ -9,11 +9,12 @@
 // with columns c = 2n + (re, im).  The MFMA A operand is the coefficient
 // block (16 columns x 4 harmonics, read straight from the coll layout), the B
 // operand the table block (4 harmonics x 16 theta points, tables stored
-// [h][j]); each lane ends with 4 consecutive columns (two complex values)
-// of one theta point, so the epilogue writes 16 consecutive theta points
-// per shell.  Parity-flipped tables (dtheta P) accumulate into the other
+// [h][j]).  Parity-flipped tables (dtheta P) accumulate into the other
 // bucket; the i and -i rotations of the vector/gradient terms swap the
-// re/im column of a lane pair (one lane exchange).
+// re/im column of a lane pair (one lane exchange).  The Fourier output is
+// stored [n][j][m] (m fastest), the order in which the physical-space pass
+// reads a theta row, so a workgroup covers several consecutive m and
+// writes each row's run of them at once.
 //
 // Analysis (modules/transform.fftw3.F90:527-602, 825-949).  The forward FFT
 // leaves folded data G_p[j, c] (north+south, south-north).  A workgroup
@@ -94,181 +95,164 @@ __device__ __forceinline__ P pickj(const P (&v)[LG_MAXJ], int j) {
 
 template <typename T>
 struct SynArgs {
-    int N, Th, Th2, M, H;
-    int nth, ncc, ngroups;  // theta chunks, column chunks per job, nslot * nth
-    int njob;
+    int N, Th, Th2, M, MP, H;
+    int nth, ncc, nmg, njob;  // theta blocks, column blocks per job, m groups, jobs
     const Slot* slots;
-    const int* order;
-    const T* tabA;   // [h][Th2]
-    const T* tabB;   // second table (vector / gradient)
+    const int* mslot;         // [M] slot of each FFT index m
+    const T* tabA;            // [h][Th2]
+    const T* tabB;            // second table (vector / gradient)
     const double* rinv;
-    const T* in0[LG_MAXJ];  // coll [h][2N] (re, im interleaved)
+    const T* in0[LG_MAXJ];    // coll [h][2N] (re, im interleaved)
     const T* in1[LG_MAXJ];
-    T* out0[LG_MAXJ];       // fourier [n][M][Th] complex
+    T* out0[LG_MAXJ];         // fourier [n][Th][MP] complex
     T* out1[LG_MAXJ];
 };
 
-// KIND 0: scalar (P);  1: vector (llP_ flipped, llsin1mP);  2: gradient
-// (P_ flipped, sin1mP, scaled by 1/r).
+constexpr int SY_TH = 16;     // theta points per tile (the MFMA's 16 columns)
+constexpr int SY_COLS = 16;   // coefficient columns per tile (8 shells, re/im)
+constexpr int SY_MG = 8;      // FFT indices m per workgroup
+constexpr int SY_WAVES = 4;   // waves per workgroup (SY_MG / SY_WAVES m each)
+constexpr int SY_SP = 2 * SY_MG + 1;  // LDS row of one (output, mirror, shell, theta): MG re/im pairs + pad
+constexpr int SY_PD = 4;      // K steps per prefetch group
+
+// Row (coefficient column) of accumulator element r held by lane group q4:
+// v_mfma_f32_16x16x4f32 gives rows 4 q4 + r, the FP64 instruction q4 + 4 r.
+template <typename T>
+__device__ __forceinline__ int acc_row(int q4, int r) {
+    return sizeof(T) == 8 ? q4 + 4 * r : 4 * q4 + r;
+}
+
+// Synthesis into the transposed Fourier layout [n][j][m].  A workgroup
+// takes one tile of 16 theta points x 16 columns for SY_MG consecutive m:
+// each wave contracts its m values one after another (KIND 0: scalar P;
+// 1: vector (llP_ flipped, llsin1mP); 2: gradient (P_ flipped, sin1mP,
+// scaled by 1/r)), with the operands of the next group of SY_PD K steps
+// loaded while the current group's products run, and leaves its results in
+// LDS; the workgroup then writes, for every (shell, theta row), the SY_MG
+// values of consecutive m as one contiguous run.
 template <typename T, int KIND>
-__global__ void __launch_bounds__(64) k_syn(SynArgs<T> a) {
+__global__ void __launch_bounds__(64 * SY_WAVES) k_syn(SynArgs<T> a) {
     typedef typename Mfma<T>::acc acc_t;
+    typedef typename std::conditional<sizeof(T) == 8, double2, float2>::type T2;
     constexpr int NOUT = KIND == 0 ? 1 : 2;
-    constexpr int CB = KIND == 0 ? 2 : 1;
-    const BlockId bid = decode_block(a.ngroups, a.ncc * a.njob);
+    __shared__ T St[NOUT * 2 * 8 * SY_TH * SY_SP];
+    const BlockId bid = decode_block(a.nmg * a.nth, a.ncc * a.njob);
     if (!bid.valid) return;
-    const int sl = uniform(a.order[bid.group / a.nth]);
-    const Slot s = a.slots[sl];
-    const int lane = threadIdx.x, l16 = lane & 15, q4 = lane >> 4;
-    const int j0 = (bid.group % a.nth) * 64;
-    const int job = bid.cc / a.ncc;
-    const int c0 = (bid.cc - job * a.ncc) * 16 * CB;
+    const int mg = bid.group / a.nth, tb = bid.group - mg * a.nth;
+    const int job = bid.cc / a.ncc, cc = bid.cc - job * a.ncc;
+    const int j0 = tb * SY_TH, c0 = cc * SY_COLS, m0 = mg * SY_MG;
     const T* in0 = pickj(a.in0, job);
     const T* in1 = pickj(a.in1, job);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l16 = lane & 15, q4 = lane >> 4;
     const int C2 = 2 * a.N;
-    acc_t Y[NOUT][2][CB][4];
-#pragma unroll
-    for (int o = 0; o < NOUT; o++)
-#pragma unroll
-        for (int p = 0; p < 2; p++)
-#pragma unroll
-            for (int cb = 0; cb < CB; cb++)
-#pragma unroll
-                for (int jb = 0; jb < 4; jb++) Y[o][p][cb][jb] = acc_t{0, 0, 0, 0};
     const bool odd_col = (l16 & 1);
-    // Operands of one step of 8 harmonics (4 per parity): B = table values
-    // at this lane's theta points, A = coefficients at this lane's columns.
-    // The next step's operands are loaded while this step's products run.
+    const int jc = j0 + l16 < a.Th2 ? j0 + l16 : a.Th2 - 1;
+    const int c = c0 + l16;
     struct Ops {
-        T ta[2][4], tb[2][4], x[2][CB], t[2][CB];
+        T ta[2], tb[2], x[2], t[2];
     };
-    auto load = [&](Ops& o, int k0) {
+#pragma unroll 1
+    for (int mi = wave; mi < SY_MG; mi += SY_WAVES) {
+        const int m = m0 + mi;
+        acc_t Y[NOUT][2];
 #pragma unroll
-        for (int p = 0; p < 2; p++) {
-            const int k = k0 + 2 * q4 + p;
-            const bool kv = k < s.len;
-            const size_t hrow = (size_t)s.h0 + (kv ? k : 0);
+        for (int o = 0; o < NOUT; o++) Y[o][0] = Y[o][1] = acc_t{0, 0, 0, 0};
+        if (m < a.M) {
+            const Slot s = a.slots[a.mslot[m]];
+            auto load = [&](Ops& op, int k0) {
 #pragma unroll
-            for (int jb = 0; jb < 4; jb++) {
-                int j = j0 + 16 * jb + l16;
-                int jc = j < a.Th2 ? j : a.Th2 - 1;
-                o.ta[p][jb] = kv ? a.tabA[hrow * a.Th2 + jc] : T(0);
-                o.tb[p][jb] = (KIND != 0 && kv) ? a.tabB[hrow * a.Th2 + jc] : T(0);
-            }
+                for (int p = 0; p < 2; p++) {
+                    const int k = k0 + 2 * q4 + p;
+                    const bool kv = k < s.len;
+                    const size_t hrow = (size_t)s.h0 + (kv ? k : 0);
+                    op.ta[p] = kv ? a.tabA[hrow * a.Th2 + jc] : T(0);
+                    op.tb[p] = (KIND != 0 && kv) ? a.tabB[hrow * a.Th2 + jc] : T(0);
+                    const bool cv = kv && c < C2;
+                    op.x[p] = cv ? in0[hrow * C2 + c] : T(0);
+                    op.t[p] = (KIND == 1 && cv) ? in1[hrow * C2 + c] : T(0);
+                }
+            };
+            auto step = [&](const Ops& op) {
 #pragma unroll
-            for (int cb = 0; cb < CB; cb++) {
-                const int c = c0 + 16 * cb + l16;
-                const bool cv = kv && c < C2;
-                o.x[p][cb] = cv ? in0[hrow * C2 + c] : T(0);
-                o.t[p][cb] = (KIND == 1 && cv) ? in1[hrow * C2 + c] : T(0);
-            }
-        }
-    };
-    auto step = [&](const Ops& o) {
-#pragma unroll
-        for (int p = 0; p < 2; p++) {
-#pragma unroll
-            for (int cb = 0; cb < CB; cb++) {
-                const T x = o.x[p][cb];
-                if (KIND == 0) {
-#pragma unroll
-                    for (int jb = 0; jb < 4; jb++) Y[0][p][cb][jb] = Mfma<T>::run(x, o.ta[p][jb], Y[0][p][cb][jb]);
-                } else if (KIND == 2) {
-                    // theta: c * dP (bucket 1-p);  phi: (i c) * mP/sin (bucket p)
-                    T xr = partner(x);
-                    xr = odd_col ? xr : -xr;
-#pragma unroll
-                    for (int jb = 0; jb < 4; jb++) {
-                        Y[0][1 - p][cb][jb] = Mfma<T>::run(x, o.ta[p][jb], Y[0][1 - p][cb][jb]);
-                        Y[NOUT - 1][p][cb][jb] = Mfma<T>::run(xr, o.tb[p][jb], Y[NOUT - 1][p][cb][jb]);
-                    }
-                } else {
-                    const T t = o.t[p][cb];
-                    // (-i t): re <- t_im, im <- -t_re ;  (i s): re <- -s_im, im <- s_re
-                    T tr = partner(t), sr = partner(x);
-                    tr = odd_col ? -tr : tr;
-                    sr = odd_col ? sr : -sr;
-#pragma unroll
-                    for (int jb = 0; jb < 4; jb++) {
-                        Y[0][1 - p][cb][jb] = Mfma<T>::run(x, o.ta[p][jb], Y[0][1 - p][cb][jb]);
-                        Y[0][p][cb][jb] = Mfma<T>::run(tr, o.tb[p][jb], Y[0][p][cb][jb]);
-                        Y[NOUT - 1][p][cb][jb] = Mfma<T>::run(sr, o.tb[p][jb], Y[NOUT - 1][p][cb][jb]);
-                        Y[NOUT - 1][1 - p][cb][jb] = Mfma<T>::run(t, o.ta[p][jb], Y[NOUT - 1][1 - p][cb][jb]);
+                for (int p = 0; p < 2; p++) {
+                    const T x = op.x[p];
+                    if (KIND == 0) {
+                        Y[0][p] = Mfma<T>::run(x, op.ta[p], Y[0][p]);
+                    } else if (KIND == 2) {
+                        // theta: c * dP (bucket 1-p);  phi: (i c) * mP/sin (bucket p)
+                        T xr = partner(x);
+                        xr = odd_col ? xr : -xr;
+                        Y[0][1 - p] = Mfma<T>::run(x, op.ta[p], Y[0][1 - p]);
+                        Y[NOUT - 1][p] = Mfma<T>::run(xr, op.tb[p], Y[NOUT - 1][p]);
+                    } else {
+                        const T t = op.t[p];
+                        // (-i t): re <- t_im, im <- -t_re ;  (i s): re <- -s_im, im <- s_re
+                        T tr = partner(t), sr = partner(x);
+                        tr = odd_col ? -tr : tr;
+                        sr = odd_col ? sr : -sr;
+                        Y[0][1 - p] = Mfma<T>::run(x, op.ta[p], Y[0][1 - p]);
+                        Y[0][p] = Mfma<T>::run(tr, op.tb[p], Y[0][p]);
+                        Y[NOUT - 1][p] = Mfma<T>::run(sr, op.tb[p], Y[NOUT - 1][p]);
+                        Y[NOUT - 1][1 - p] = Mfma<T>::run(t, op.ta[p], Y[NOUT - 1][1 - p]);
                     }
                 }
+            };
+            // groups of SY_PD steps (8 harmonics each), two groups in flight
+            Ops ga[SY_PD], gb[SY_PD];
+#pragma unroll
+            for (int u = 0; u < SY_PD; u++) load(ga[u], 8 * u);
+            for (int k0 = 0; k0 < s.len; k0 += 16 * SY_PD) {
+                const int k1 = k0 + 8 * SY_PD;
+                if (k1 < s.len) {
+#pragma unroll
+                    for (int u = 0; u < SY_PD; u++) load(gb[u], k1 + 8 * u);
+                }
+#pragma unroll
+                for (int u = 0; u < SY_PD; u++)
+                    if (k0 + 8 * u < s.len) step(ga[u]);
+                if (k1 >= s.len) break;
+                if (k1 + 8 * SY_PD < s.len) {
+#pragma unroll
+                    for (int u = 0; u < SY_PD; u++) load(ga[u], k1 + 8 * SY_PD + 8 * u);
+                }
+#pragma unroll
+                for (int u = 0; u < SY_PD; u++)
+                    if (k1 + 8 * u < s.len) step(gb[u]);
             }
         }
-    };
-    Ops cur, nxt;
-    load(cur, 0);
-    for (int k0 = 0; k0 < s.len; k0 += 8) {
-        if (k0 + 8 < s.len) load(nxt, k0 + 8);
-        step(cur);
-        cur = nxt;
+        // F[j] = Y0 + Y1, F[Th-1-j] = Y0 - Y1; lane: theta l16, rows acc_row
+#pragma unroll
+        for (int o = 0; o < NOUT; o++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = acc_row<T>(q4, r), sh = row >> 1, part = row & 1;
+                const T y0 = Y[o][0][r], y1 = Y[o][1][r];
+                T* d = St + (((o * 2 + 0) * 8 + sh) * SY_TH + l16) * SY_SP + 2 * mi + part;
+                d[0] = y0 + y1;
+                d[8 * SY_TH * SY_SP] = y0 - y1;
+            }
     }
-    // Output layout of v_mfma_*_16x16x4: the lane holds column j = l16 of
-    // rows mrow(q4, r); rows are our columns c (re/im of a shell), columns
-    // are theta points j0 + 16jb + l16.
-    typedef typename std::conditional<sizeof(T) == 8, double2, float2>::type T2;
-#pragma unroll
-    for (int o = 0; o < NOUT; o++) {
-        T2* F = reinterpret_cast<T2*>(o == 0 ? pickj(a.out0, job) : pickj(a.out1, job));
-#pragma unroll
-        for (int cb = 0; cb < CB; cb++) {
-#pragma unroll
-            for (int jb = 0; jb < 4; jb++) {
-                const int j = j0 + 16 * jb + l16;
-                T re0[2], im0[2], re1[2], im1[2];
-                int shell[2];
-                if constexpr (sizeof(T) == 8) {
-                    // rows q4 + 4r: even q4 holds re, odd q4 the im of the same
-                    // shells; pair with lane ^ 16, each lane writes two shells
-                    const bool lo = !(q4 & 1);
-#pragma unroll
-                    for (int hp = 0; hp < 2; hp++) {
-                        const int r0 = hp, r1 = 2 + hp;
-                        // values this lane sends: the half its partner writes
-                        const T s0 = lo ? Y[o][0][cb][jb][r1] : Y[o][0][cb][jb][r0];
-                        const T s1 = lo ? Y[o][1][cb][jb][r1] : Y[o][1][cb][jb][r0];
-                        const T g0 = __shfl_xor(s0, 16);
-                        const T g1 = __shfl_xor(s1, 16);
-                        const int r = lo ? r0 : r1;
-                        re0[hp] = lo ? Y[o][0][cb][jb][r] : g0;
-                        im0[hp] = lo ? g0 : Y[o][0][cb][jb][r];
-                        re1[hp] = lo ? Y[o][1][cb][jb][r] : g1;
-                        im1[hp] = lo ? g1 : Y[o][1][cb][jb][r];
-                        shell[hp] = (q4 >> 1) + 2 * r;
-                    }
-                } else {
-                    // rows 4q4 + r: the lane holds two whole complex values
-#pragma unroll
-                    for (int hp = 0; hp < 2; hp++) {
-                        re0[hp] = Y[o][0][cb][jb][2 * hp];
-                        im0[hp] = Y[o][0][cb][jb][2 * hp + 1];
-                        re1[hp] = Y[o][1][cb][jb][2 * hp];
-                        im1[hp] = Y[o][1][cb][jb][2 * hp + 1];
-                        shell[hp] = 2 * q4 + hp;
-                    }
-                }
-                if (j >= a.Th2) continue;
-#pragma unroll
-                for (int hp = 0; hp < 2; hp++) {
-                    const int n = ((c0 + 16 * cb) >> 1) + shell[hp];
-                    if (n >= a.N) continue;
-                    const T f = KIND == 2 ? (T)a.rinv[n] : T(1);
-                    T2* row = F + ((size_t)n * a.M + s.mfft) * a.Th;
-                    T2 v;
-                    v.x = f * (re0[hp] + re1[hp]);
-                    v.y = f * (im0[hp] + im1[hp]);
-                    row[j] = v;
-                    if (a.Th - 1 - j >= a.Th2) {
-                        v.x = f * (re0[hp] - re1[hp]);
-                        v.y = f * (im0[hp] - im1[hp]);
-                        row[a.Th - 1 - j] = v;
-                    }
-                }
-            }
-        }
+    __syncthreads();
+    constexpr int NITEM = NOUT * 2 * 8 * SY_TH * SY_MG;
+    for (int idx = tid; idx < NITEM; idx += 64 * SY_WAVES) {
+        const int mi = idx % SY_MG;
+        int rest = idx / SY_MG;
+        const int th = rest % SY_TH;
+        rest /= SY_TH;
+        const int sh = rest % 8;
+        rest /= 8;
+        const int mir = rest & 1, o = rest >> 1;
+        const int n = (c0 >> 1) + sh, j = j0 + th;
+        if (n >= a.N || j >= a.Th2) continue;
+        const int jrow = mir ? a.Th - 1 - j : j;
+        if (mir && jrow < a.Th2) continue;
+        const T* sp = St + (((o * 2 + mir) * 8 + sh) * SY_TH + th) * SY_SP + 2 * mi;
+        const T f = KIND == 2 ? (T)a.rinv[n] : T(1);
+        T2 v;
+        v.x = f * sp[0];
+        v.y = f * sp[1];
+        T* out = o == 0 ? pickj(a.out0, job) : pickj(a.out1, job);
+        reinterpret_cast<T2*>(out)[((size_t)n * a.Th + jrow) * a.MP + m0 + mi] = v;
     }
 }
 
@@ -284,9 +268,10 @@ static int syn_launch(Ctx& c, int kind, int njob, const T* tA, const T* tB, cons
     a.Th = c.Th;
     a.Th2 = c.Th2;
     a.M = c.M;
+    a.MP = c.MP;
     a.H = c.H;
     a.slots = c.d_slots;
-    a.order = c.d_slot_order;
+    a.mslot = c.d_mslot;
     a.tabA = tA;
     a.tabB = tB;
     a.rinv = c.d_r + 2 * c.N;
@@ -297,15 +282,15 @@ static int syn_launch(Ctx& c, int kind, int njob, const T* tA, const T* tB, cons
         a.out0[k] = reinterpret_cast<T*>(out0[q]);
         a.out1[k] = out1 ? reinterpret_cast<T*>(out1[q]) : nullptr;
     }
-    const int cb = kind == SYN_SCALAR ? 2 : 1;
-    a.nth = (c.Th2 + 63) / 64;
-    a.ncc = (2 * c.N + 16 * cb - 1) / (16 * cb);
-    a.ngroups = c.nslot * a.nth;
+    a.nth = (c.Th2 + SY_TH - 1) / SY_TH;
+    a.ncc = (2 * c.N + SY_COLS - 1) / SY_COLS;
+    a.nmg = (c.M + SY_MG - 1) / SY_MG;
     a.njob = njob;
-    dim3 grid(grid_blocks(a.ngroups, a.ncc * njob));
-    if (kind == SYN_SCALAR) hipLaunchKernelGGL((k_syn<T, 0>), grid, dim3(64), 0, c.stream, a);
-    else if (kind == SYN_VECTOR) hipLaunchKernelGGL((k_syn<T, 1>), grid, dim3(64), 0, c.stream, a);
-    else hipLaunchKernelGGL((k_syn<T, 2>), grid, dim3(64), 0, c.stream, a);
+    dim3 grid(grid_blocks(a.nmg * a.nth, a.ncc * njob));
+    dim3 blk(64 * SY_WAVES);
+    if (kind == SYN_SCALAR) hipLaunchKernelGGL((k_syn<T, 0>), grid, blk, 0, c.stream, a);
+    else if (kind == SYN_VECTOR) hipLaunchKernelGGL((k_syn<T, 1>), grid, blk, 0, c.stream, a);
+    else hipLaunchKernelGGL((k_syn<T, 2>), grid, blk, 0, c.stream, a);
     LSD_HIP(hipGetLastError());
     return LSD_OK;
 }

@@ -78,7 +78,7 @@ static void drop_graphs(Ctx& c);
 static void release(Ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
-    void* ptrs[] = {c->d_hl, c->d_hm, c->d_pidx, c->d_apos, c->d_slots, c->d_slot_order, c->d_gp, c->d_gw,
+    void* ptrs[] = {c->d_hl, c->d_hm, c->d_pidx, c->d_apos, c->d_slots, c->d_slot_order, c->d_mslot, c->d_gp, c->d_gw,
                     c->d_sin, c->d_tab, c->d_atab, c->d_r, c->d_dr1, c->d_dr2, c->d_radlap, c->d_r1dr,
                     c->d_intr2dr, c->d_ll, c->d_lu, c->d_piv, c->d_pivpk, c->d_y, c->d_gre, c->d_invg, c->d_info,
                     c->d_lut32, c->d_xband, c->d_sblocks, c->d_sorder, c->d_tab32,
@@ -116,6 +116,7 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
     c->Ph = Ph;
     c->H = H;
     c->Th2 = (Th + 1) / 2;
+    c->MP = (c->M + 15) / 16 * 16;
     c->dt = cfg->dt;
     c->mixed = cfg->precision != LSD_PLAIN_F64;
     *out = c;
@@ -143,6 +144,7 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
     size_t HN = c->coll_elems();
     if (dalloc(&c->d_hl, H) || dalloc(&c->d_hm, H) || dalloc(&c->d_pidx, pidx.size()) ||
         dalloc(&c->d_apos, H) || dalloc(&c->d_slots, c->nslot) || dalloc(&c->d_slot_order, c->nslot) ||
+        dalloc(&c->d_mslot, c->M) ||
         dalloc(&c->d_gp, Th) || dalloc(&c->d_gw, Th) || dalloc(&c->d_sin, Th) ||
         dalloc(&c->d_tab, (size_t)5 * H * c->Th2) || dalloc(&c->d_atab, (size_t)3 * H * c->Th2) ||
         dalloc(&c->d_r, (size_t)7 * N) || dalloc(&c->d_dr1, (size_t)7 * N) || dalloc(&c->d_dr2, (size_t)7 * N) ||
@@ -203,6 +205,12 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
                            st));
     LSD_HIP(hipMemcpyAsync(c->d_slots, c->slots.data(), sizeof(Slot) * c->nslot, hipMemcpyHostToDevice, st));
     LSD_HIP(hipMemcpyAsync(c->d_slot_order, c->slot_order.data(), sizeof(int) * c->nslot, hipMemcpyHostToDevice, st));
+    {
+        std::vector<int> mslot(c->M, 0);
+        for (int k = 0; k < c->nslot; k++) mslot[c->slots[k].mfft] = k;
+        LSD_HIP(hipMemcpyAsync(c->d_mslot, mslot.data(), sizeof(int) * c->M, hipMemcpyHostToDevice, st));
+        LSD_HIP(hipStreamSynchronize(st));
+    }
     LSD_HIP(hipMemsetAsync(c->d_tab, 0, sizeof(double) * 5 * H * c->Th2, st));
     LSD_HIP(hipMemsetAsync(c->d_atab, 0, sizeof(double) * 3 * H * c->Th2, st));
     LSD_HIP(hipMemsetAsync(c->d_lu, 0, sizeof(double) * NLU * L * N * LSD_LDAB, st));

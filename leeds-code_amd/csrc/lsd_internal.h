@@ -3,7 +3,8 @@
 // Device layouts (all resident in HBM for the life of a context):
 //   coll     double2 [h][n]            complex, radius fastest (the reference's
 //                                       coll type with Re/Im interleaved)
-//   fourier  double2 [f][n][m][j]      j = 0..Th-1 fastest, m = FFT index
+//   fourier  double2 [f][n][j][MP]     m = FFT index fastest, MP = M rounded
+//                                       up to 16 (a theta row is one run)
 //   folded   double2 [f][n][m][p][j]   j < Th2, p = equatorial parity bucket
 //                                       (north+south, south-north)
 //   phys     double  [f][n][k][j]      the reference's (Th, Ph, N) layout
@@ -82,6 +83,7 @@ struct Ctx {
     lsd_config cfg;
     int device = 0;
     int L, M, Mp, N, Th, Ph, H, Th2, nslot;
+    int MP;                        // Fourier row length: M rounded up to 16
     double dt;
     bool mixed = false;            // FP32 transforms and solves (LSD_MIXED_*)
     hipStream_t stream = nullptr;
@@ -97,6 +99,7 @@ struct Ctx {
     int* d_apos = nullptr;         // analysis-table position of each harmonic [H]
     Slot* d_slots = nullptr;       // [nslot]
     int* d_slot_order = nullptr;   // [nslot]
+    int* d_mslot = nullptr;        // [M] slot of each FFT index
 
     // device: quadrature and tables
     double* d_gp = nullptr;        // [Th]
@@ -170,7 +173,7 @@ struct Ctx {
 
     size_t coll_elems() const { return (size_t)H * N; }
     size_t phys_elems() const { return (size_t)Th * Ph * N; }
-    size_t fourier_elems() const { return (size_t)N * M * Th; }
+    size_t fourier_elems() const { return (size_t)N * Th * MP; }
     size_t folded_elems() const { return (size_t)N * M * 2 * Th2; }
 };
 

@@ -40,13 +40,13 @@ constexpr int PF_GROUP = 128 / (int)sizeof(typename Cx<R>::t);
 template <typename R>
 struct PhysArgs {
     FftPlan pl;
-    int N, Th, Th2, M, ngroups, gpr;  // gpr: groups of PF_GROUP<R> row pairs per shell
+    int N, Th, Th2, M, MP, ngroups, gpr;  // gpr: groups of PF_GROUP<R> row pairs per shell
     float inv_m, inv_w, inv_2m;       // 1/M, 1/(Ph - 2M + 1), 1/(2M) for quot()
     R cor, cmag, qc, qx, scale;
     int cod, comp, mag;
     const double* gp;   // [Th] cos(theta)
     const double* sn;   // [Th] sin(theta)
-    const typename Cx<R>::t* fourier;  // [18][N][M][Th]
+    const typename Cx<R>::t* fourier;  // [18][N][Th][MP]
     typename Cx<R>::t* folded;         // [8][N][M][2][Th2]
     const typename Cx<R>::t* tw;
 };
@@ -63,12 +63,12 @@ __device__ __forceinline__ int quot(int i, float inv) { return (int)(((float)i +
 // does.
 template <typename V>
 __device__ void pack_inputs(V* Z, const V* __restrict__ fourier, int f0, int f1, int nb, int sh, int N, int M,
-                            int Th, int Ph, int jp, int jm, float inv_m, float inv_w) {
+                            int MP, int Th, int Ph, int jp, int jm, float inv_m, float inv_w) {
     for (int idx = threadIdx.x; idx < nb * M; idx += PF_THREADS) {
         const int b = quot(idx, inv_m), k = idx - b * M;
         const int fid = b < 3 ? f0 + b : f1 + b - 3;
-        const V* X = fourier + (((size_t)fid * N + sh) * M + k) * Th;
-        const V A = X[jp], B = X[jm];
+        const V* X = fourier + ((size_t)fid * N + sh) * Th * MP + k;
+        const V A = X[(size_t)jp * MP], B = X[(size_t)jm * MP];
         const int z = b * Ph;
         if (k == 0) {
             Z[lidx<true>(z)] = cx<V>(A.x, B.x);
@@ -200,7 +200,7 @@ __global__ void __launch_bounds__(PF_THREADS) k_physfuse(PhysArgs<R> a) {
 
     // u and curl u: advection and Coriolis
     {
-        pack_inputs(Z, a.fourier, LSD_P_VEL_R, LSD_P_VCURL_R, 6, sh, N, M, Th, Ph, jp, jm, a.inv_m, a.inv_w);
+        pack_inputs(Z, a.fourier, LSD_P_VEL_R, LSD_P_VCURL_R, 6, sh, N, M, a.MP, Th, Ph, jp, jm, a.inv_m, a.inv_w);
         fft_pf<1, R512>(Z, a.pl, 6, twl);
 #pragma unroll
         for (int p = 0; p < PP; p++) {
@@ -232,7 +232,7 @@ __global__ void __launch_bounds__(PF_THREADS) k_physfuse(PhysArgs<R> a) {
     }
     // B and curl B: Lorentz force and induction E = u x B
     if (a.mag) {
-        pack_inputs(Z, a.fourier, LSD_P_MAG_R, LSD_P_MCURL_R, 6, sh, N, M, Th, Ph, jp, jm, a.inv_m, a.inv_w);
+        pack_inputs(Z, a.fourier, LSD_P_MAG_R, LSD_P_MCURL_R, 6, sh, N, M, a.MP, Th, Ph, jp, jm, a.inv_m, a.inv_w);
         fft_pf<1, R512>(Z, a.pl, 6, twl);
 #pragma unroll
         for (int p = 0; p < PP; p++) {
@@ -259,8 +259,8 @@ __global__ void __launch_bounds__(PF_THREADS) k_physfuse(PhysArgs<R> a) {
     // grad C and grad xi: advection of codensity and composition
     if (a.cod || a.comp) {
         const int nb = 3 * (a.cod + a.comp);
-        pack_inputs(Z, a.fourier, a.cod ? LSD_P_COD_GR : LSD_P_COMP_GR, LSD_P_COMP_GR, nb, sh, N, M, Th, Ph, jp, jm,
-                    a.inv_m, a.inv_w);
+        pack_inputs(Z, a.fourier, a.cod ? LSD_P_COD_GR : LSD_P_COMP_GR, LSD_P_COMP_GR, nb, sh, N, M, a.MP, Th, Ph, jp,
+                    jm, a.inv_m, a.inv_w);
         fft_pf<1, R512>(Z, a.pl, nb, twl);
         const int oc = 0, ox = a.cod ? 3 : 0;
 #pragma unroll
@@ -336,6 +336,7 @@ static int physfuse_launch(Ctx& c, const void* fourier, void* folded, const void
     a.Th = c.Th;
     a.Th2 = c.Th2;
     a.M = c.M;
+    a.MP = c.MP;
     a.gpr = (c.Th2 + PF_GROUP<R> - 1) / PF_GROUP<R>;
     a.inv_m = 1.0f / (float)c.M;
     a.inv_w = 1.0f / (float)(c.Ph - 2 * c.M + 1);
