@@ -1,12 +1,11 @@
 // Predictor / corrector solves in two passes, for the radial resolutions
 // the library is built for (N fixed at compile time).
 //
-// Pass 1, k_rhs: one work-item per (harmonic, radial point), a wave over
-// 64 consecutive points of one coll column, so every read and write is
-// coalesced.  It forms each field's right-hand side in FP64 -- Y f + N
-// with the boundary rows of the *_setbc routines (predictor), c (N* - N)
-// (corrector) -- saves N for the next corrector, and writes the right-hand
-// side in the solve precision.
+// Pass 1 forms each field's right-hand side in FP64 -- Y f + N with the
+// boundary rows of the *_setbc routines (k_rhs_pred, one workgroup per
+// block of harmonics of one degree, with the Y band and the state columns
+// staged in LDS), c (N* - N) (k_rhs_corr, one work-item per point) -- and
+// saves N for the next corrector.
 //
 // Pass 2, k_sdir: one 64-lane wave per block of up to 32 harmonics of one
 // degree l (lane = harmonic, Re/Im part), so every lane needs the same
@@ -82,8 +81,9 @@ __device__ __forceinline__ double2 band_acc(double2 acc, const double* __restric
     return acc;
 }
 
-template <bool PRED>
-__global__ void __launch_bounds__(256) k_rhs(DirArgs a) {
+// Corrector right-hand sides d = c (N* - N), with the saved N advanced;
+// one work-item per (harmonic, radial point), coalesced.
+__global__ void __launch_bounds__(256) k_rhs_corr(DirArgs a) {
     const int f = blockIdx.y;
     if (!a.on[f]) return;
     const int N = a.N;
@@ -95,33 +95,75 @@ __global__ void __launch_bounds__(256) k_rhs(DirArgs a) {
     // (which the compiler would route through scratch memory)
     const double keep = (n == 0 || n == N - 1) ? 0.0 : 1.0;
     const double2 nv = pick(a.nn, f)[idx];
-    const double2* xc = pick(a.x, f) + (size_t)h * N;
+    double2* hist = pick(a.hist, f);
+    const double2 hv = hist[idx];
+    hist[idx] = nv;
+    double2 d = make_double2(keep * (a.cimp * (nv.x - hv.x)), keep * (a.cimp * (nv.y - hv.y)));
+    // vel_setbc_Tor for (1,0) once tim_it /= 0
+    if (f == 0 && l == 1 && a.hm[h] == 0 && n == 0) d.x = 0.0 - pick(a.x, f)[(size_t)h * N].x;
+    pick(a.rhs, f)[idx] = d;
+}
+
+// Predictor right-hand sides, one workgroup per solve block (harmonics of one
+// degree l) and field: the Y band of (field, l) and the block's state
+// columns are staged in LDS, so each point's 7-term band row reads LDS only.
+constexpr int RHS_BLK = 1024;
+constexpr int RHS_IT = (32 * 96 + RHS_BLK - 1) / RHS_BLK;  // points per thread at 32 harmonics, N <= 96
+
+__global__ void __launch_bounds__(RHS_BLK) k_rhs_pred(DirArgs a, float inv_n) {
+    extern __shared__ double2 xs_l[];  // [nh][N] state columns, then the band
+    const int f = blockIdx.y;
+    if (!a.on[f]) return;
+    const int N = a.N;
+    const int4 blk = a.blocks[blockIdx.x];
+    const int l = blk.x, first = blk.y, nh = blk.z;
+    double* band = reinterpret_cast<double*>(xs_l + (size_t)nh * N);
+    const double2* x = pick(a.x, f);
+    const double2* nn = pick(a.nn, f);
     double2* hist = pick(a.hist, f);
     double2* rhs = pick(a.rhs, f);
-    if constexpr (PRED) {
+    const bool has_y = f == 2 || f == 3 || l != 0;
+    const double* yb = a.y + ((size_t)f * a.L + l) * N * 7;
+    for (int i = threadIdx.x; i < N * 7; i += RHS_BLK) band[i] = yb[i];
+    const int ne = nh * N;
+    // every point's state and N values are requested before any is used
+    double2 nvs[RHS_IT];
+#pragma unroll
+    for (int k = 0; k < RHS_IT; k++) {
+        const int e = threadIdx.x + RHS_BLK * k;
+        if (e < ne) {
+            const int hh = (int)(((float)e + 0.5f) * inv_n), n = e - hh * N;
+            const size_t idx = (size_t)a.order[first + hh] * N + n;
+            xs_l[e] = x[idx];
+            nvs[k] = nn[idx];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < RHS_IT; k++) {
+        const int e = threadIdx.x + RHS_BLK * k;
+        if (e >= ne) break;
+        const int hh = (int)(((float)e + 0.5f) * inv_n), n = e - hh * N;
+        const int h = a.order[first + hh];
+        const size_t idx = (size_t)h * N + n;
+        const double keep = (n == 0 || n == N - 1) ? 0.0 : 1.0;
+        const double2 nv = nvs[k];
         hist[idx] = nv;
+        const double2* xc = xs_l + hh * N;
         if (f == 1 && l != 0) {
             // XGre acts on N with zero boundary rows; the Y x part of the
             // XPol right-hand side (its boundary rows are zero too)
             rhs[idx] = make_double2(keep * nv.x, keep * nv.y);
-            const double2 r = band_acc(make_double2(0.0, 0.0), a.y + ((size_t)1 * a.L + l) * N * 7, xc, n, N);
+            const double2 r = band_acc(make_double2(0.0, 0.0), band, xc, n, N);
             a.rhs2[idx] = make_double2(keep * r.x, keep * r.y);
         } else {
-            const bool has_y = f == 2 || f == 3 || l != 0;
             double2 b = nv;
-            if (has_y) b = band_acc(b, a.y + ((size_t)f * a.L + l) * N * 7, xc, n, N);
+            if (has_y) b = band_acc(b, band, xc, n, N);
             b.x = keep * b.x;
             b.y = keep * b.y;
             if ((f == 2 || f == 3) && h == 0 && n == 0) b.x = 1.0;
             rhs[idx] = b;
         }
-    } else {
-        const double2 hv = hist[idx];
-        hist[idx] = nv;
-        double2 d = make_double2(keep * (a.cimp * (nv.x - hv.x)), keep * (a.cimp * (nv.y - hv.y)));
-        // vel_setbc_Tor for (1,0) once tim_it /= 0
-        if (f == 0 && l == 1 && a.hm[h] == 0 && n == 0) d.x = 0.0 - xc[0].x;
-        rhs[idx] = d;
     }
 }
 
@@ -441,9 +483,14 @@ int launch_solve_direct(Ctx& c, bool pred) {
     const DirArgs a = dir_args(c);
     const int nf = 2 + (c.cfg.cod ? 1 : 0) + (c.cfg.comp ? 1 : 0) + (c.cfg.mag ? 2 : 0);
     ProfScope ps(c, LSD_K_SOLVE, (4.0 * 16.0 + (pred ? 2.0 : 3.0) * 16.0) * c.coll_elems() * nf);
-    const dim3 g1((unsigned)((c.coll_elems() + 255) / 256), 6);
-    if (pred) hipLaunchKernelGGL((k_rhs<true>), g1, dim3(256), 0, c.stream, a);
-    else hipLaunchKernelGGL((k_rhs<false>), g1, dim3(256), 0, c.stream, a);
+    if (pred) {
+        const size_t lds = sizeof(double2) * (size_t)c.solve_hb * c.N + sizeof(double) * 7 * (size_t)c.N;
+        hipLaunchKernelGGL(k_rhs_pred, dim3(c.n_partial_blocks, 6), dim3(RHS_BLK), lds, c.stream, a,
+                           1.0f / (float)c.N);
+    } else {
+        const dim3 g1((unsigned)((c.coll_elems() + 255) / 256), 6);
+        hipLaunchKernelGGL(k_rhs_corr, g1, dim3(256), 0, c.stream, a);
+    }
     LSD_HIP(hipGetLastError());
     return pred ? launch_sdir_n<true>(c, a) : launch_sdir_n<false>(c, a);
 }
