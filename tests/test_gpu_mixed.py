@@ -1,8 +1,8 @@
 """Mixed-precision timestep (LSD_MIXED_F32) against the plain FP64 path.
 
-The mixed form runs the Legendre contractions on FP32 MFMA, the phi FFTs
-and nonlinear products in FP32, and the banded solves on FP32 factors with
-one FP64-residual refinement sweep; the state, the radial stencils and the
+The mixed form runs the Legendre contractions on FP32 MFMA and the phi FFTs
+and nonlinear products in FP32; the state, the radial stencils, the banded
+solves (at the radial resolutions of the two-pass solve kernels) and the
 time-stepping stay FP64.  Its error is measured per field as
 sqrt(sum |coeff|^2) of the difference over all (l, m, r), relative to the
 same norm of the FP64 field, against the plain FP64 GPU path, which the
