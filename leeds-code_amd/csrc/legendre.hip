@@ -136,10 +136,13 @@ __global__ void __launch_bounds__(64 * SY_WAVES) k_syn(SynArgs<T> a) {
     typedef typename std::conditional<sizeof(T) == 8, double2, float2>::type T2;
     constexpr int NOUT = KIND == 0 ? 1 : 2;
     __shared__ T St[NOUT * 2 * 8 * SY_TH * SY_SP];
-    const BlockId bid = decode_block(a.nmg * a.nth, a.ncc * a.njob);
+    // groups (m group, job, column block), the theta blocks of one group
+    // consecutive on one XCD: they read the same coefficient rows
+    const BlockId bid = decode_block(a.nmg * a.njob * a.ncc, a.nth);
     if (!bid.valid) return;
-    const int mg = bid.group / a.nth, tb = bid.group - mg * a.nth;
-    const int job = bid.cc / a.ncc, cc = bid.cc - job * a.ncc;
+    const int mg = bid.group / (a.njob * a.ncc);
+    const int gjc = bid.group - mg * a.njob * a.ncc;
+    const int job = gjc / a.ncc, cc = gjc - job * a.ncc, tb = bid.cc;
     const int j0 = tb * SY_TH, c0 = cc * SY_COLS, m0 = mg * SY_MG;
     const T* in0 = pickj(a.in0, job);
     const T* in1 = pickj(a.in1, job);
@@ -286,7 +289,7 @@ static int syn_launch(Ctx& c, int kind, int njob, const T* tA, const T* tB, cons
     a.ncc = (2 * c.N + SY_COLS - 1) / SY_COLS;
     a.nmg = (c.M + SY_MG - 1) / SY_MG;
     a.njob = njob;
-    dim3 grid(grid_blocks(a.nmg * a.nth, a.ncc * njob));
+    dim3 grid(grid_blocks(a.nmg * njob * a.ncc, a.nth));
     dim3 blk(64 * SY_WAVES);
     if (kind == SYN_SCALAR) hipLaunchKernelGGL((k_syn<T, 0>), grid, blk, 0, c.stream, a);
     else if (kind == SYN_VECTOR) hipLaunchKernelGGL((k_syn<T, 1>), grid, blk, 0, c.stream, a);
