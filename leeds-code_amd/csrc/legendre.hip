@@ -379,48 +379,94 @@ __global__ void __launch_bounds__(256) k_ana(AnaArgs<T> a) {
 #pragma unroll
         for (int o = 0; o < NO; o++) acc[t][o][0] = acc[t][o][1] = acc_t{0, 0, 0, 0};
     const size_t gstride = (size_t)2 * a.Th2;  // complex values per (n, m)
-    for (int jc0 = 0; jc0 < a.Th2; jc0 += ANA_JC) {
-        __syncthreads();
-        {   // stage: thread -> (shell in tile 0..15, theta in chunk 0..15)
-            const int sh = tid >> 4, jj = tid & 15;
-            const int n = (c0 >> 1) + sh;
-            const int j = jc0 + jj;
-            const bool ok = n < a.N && j < a.Th2;
+    // staging: thread -> (shell in tile 0..15, theta in chunk 0..15); the next
+    // chunk's values are loaded into registers while the current chunk's
+    // products run
+    const int sh_s = tid >> 4, jj_s = tid & 15;
+    const int n_s = (c0 >> 1) + sh_s;
+    T gv[NF][2][2];
+    auto gload = [&](int jc0) {
+        const int j = jc0 + jj_s;
+        const bool ok = n_s < a.N && j < a.Th2;
 #pragma unroll
-            for (int f = 0; f < NF; f++) {
-                const T* g = gf[f];
+        for (int f = 0; f < NF; f++)
 #pragma unroll
-                for (int p = 0; p < 2; p++) {
-                    T re = T(0), im = T(0);
-                    if (ok) {
-                        size_t idx = (((size_t)n * a.M + s.mfft) * gstride + (size_t)p * a.Th2 + j) * 2;
-                        re = g[idx];
-                        im = g[idx + 1];
-                    }
-                    G[f][p][jj][2 * sh] = re;
-                    G[f][p][jj][2 * sh + 1] = im;
+            for (int p = 0; p < 2; p++) {
+                T re = T(0), im = T(0);
+                if (ok) {
+                    const size_t idx = (((size_t)n_s * a.M + s.mfft) * gstride + (size_t)p * a.Th2 + j) * 2;
+                    re = gf[f][idx];
+                    im = gf[f][idx + 1];
                 }
+                gv[f][p][0] = re;
+                gv[f][p][1] = im;
             }
-        }
-        __syncthreads();
+    };
+    // table values of this lane for one chunk: [theta step][tile][P, L, S];
+    // the scalar kind loads the next chunk's while the current one runs (the
+    // vector kind has no registers to spare for it and loads them as it goes)
+    T wc[ANA_JC / 4][VEC ? 1 : ANA_TPW][3], wn[ANA_JC / 4][VEC ? 1 : ANA_TPW][3];
+    auto tload = [&](auto& w, int jc0) {
 #pragma unroll
         for (int js = 0; js < ANA_JC / 4; js++) {
-            const int jl = 4 * js + q4;  // chunk-local theta of this lane's K index
-            const int j = jc0 + jl;
+            const int j = jc0 + 4 * js + q4;
             const bool jv = j < a.Th2;
             const size_t trow = (size_t)(jv ? j : a.Th2 - 1) * a.H;
 #pragma unroll
-            for (int t = 0; t < ANA_TPW; t++) {
+            for (int t = 0; t < (int)(sizeof(w[0]) / sizeof(w[0][0])); t++) {
                 const int tile = wave + 4 * t;
-                if (tile >= ntiles) break;
+                const bool tv = tile < ntiles && jv;
                 const int par = tile < t_even ? 0 : 1;
                 const int kt = par ? tile - t_even : tile;  // tile within the class
                 const int ncls = par ? n_odd : n_even;
                 const int kc = kt * 16 + l16;               // class-local row of this lane
                 const int ap = (kc < ncls) ? s.h0 + (par ? n_even : 0) + kc : s.h0;
-                const T wP = jv ? a.tP[trow + ap] : T(0);
-                const T wL = (VEC && jv) ? a.tL[trow + ap] : T(0);
-                const T wS = (VEC && jv) ? a.tS[trow + ap] : T(0);
+                w[js][t][0] = tv ? a.tP[trow + ap] : T(0);
+                w[js][t][1] = (VEC && tv) ? a.tL[trow + ap] : T(0);
+                w[js][t][2] = (VEC && tv) ? a.tS[trow + ap] : T(0);
+            }
+        }
+    };
+    gload(0);
+    if constexpr (!VEC) tload(wc, 0);
+    for (int jc0 = 0; jc0 < a.Th2; jc0 += ANA_JC) {
+        __syncthreads();
+#pragma unroll
+        for (int f = 0; f < NF; f++)
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                G[f][p][jj_s][2 * sh_s] = gv[f][p][0];
+                G[f][p][jj_s][2 * sh_s + 1] = gv[f][p][1];
+            }
+        __syncthreads();
+        if (jc0 + ANA_JC < a.Th2) {
+            gload(jc0 + ANA_JC);
+            if constexpr (!VEC) tload(wn, jc0 + ANA_JC);
+        }
+#pragma unroll
+        for (int js = 0; js < ANA_JC / 4; js++) {
+            const int jl = 4 * js + q4;  // chunk-local theta of this lane's K index
+#pragma unroll
+            for (int t = 0; t < ANA_TPW; t++) {
+                const int tile = wave + 4 * t;
+                if (tile >= ntiles) break;
+                const int par = tile < t_even ? 0 : 1;
+                T wP, wL, wS;
+                if constexpr (VEC) {
+                    const int j = jc0 + jl;
+                    const bool jv = j < a.Th2;
+                    const size_t trow = (size_t)(jv ? j : a.Th2 - 1) * a.H;
+                    const int kt = par ? tile - t_even : tile;  // tile within the class
+                    const int ncls = par ? n_odd : n_even;
+                    const int kc = kt * 16 + l16;               // class-local row of this lane
+                    const int ap = (kc < ncls) ? s.h0 + (par ? n_even : 0) + kc : s.h0;
+                    wP = jv ? a.tP[trow + ap] : T(0);
+                    wL = jv ? a.tL[trow + ap] : T(0);
+                    wS = jv ? a.tS[trow + ap] : T(0);
+                } else {
+                    wP = wc[js][t][0];
+                    wL = wS = T(0);
+                }
 #pragma unroll
                 for (int cb = 0; cb < 2; cb++) {
                     const int col = 16 * cb + l16;
@@ -440,6 +486,14 @@ __global__ void __launch_bounds__(256) k_ana(AnaArgs<T> a) {
                     }
                 }
             }
+        }
+        if constexpr (!VEC) {
+#pragma unroll
+            for (int js = 0; js < ANA_JC / 4; js++)
+#pragma unroll
+                for (int t = 0; t < ANA_TPW; t++)
+#pragma unroll
+                    for (int e = 0; e < 3; e++) wc[js][t][e] = wn[js][t][e];
         }
     }
     // the lane holds class-local rows 16*kt + mrow of columns c0 + 16cb + l16
