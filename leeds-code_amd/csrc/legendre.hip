@@ -71,9 +71,17 @@ __device__ __forceinline__ BlockId decode_block(int ngroups, int ncc) {
 }
 static unsigned grid_blocks(int ngroups, int ncc) { return (unsigned)(((ngroups + 7) / 8) * 8 * ncc); }
 
-template <typename T>
-__device__ __forceinline__ T partner(T v) {
-    return __shfl_xor(v, 1);
+// The value of the neighbouring lane (lane ^ 1): a DPP quad permutation
+// [1, 0, 3, 2], a register move rather than a trip through the LDS crossbar.
+constexpr int DPP_SWAP_PAIRS = 0xB1;
+__device__ __forceinline__ float partner(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), DPP_SWAP_PAIRS, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double partner(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, DPP_SWAP_PAIRS, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), DPP_SWAP_PAIRS, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
 // Up to LG_MAXJ transforms of one kind run as one launch (jobs): they read
