@@ -37,8 +37,8 @@ static int check_config(const lsd_config* c, int* H, int* Th, int* Ph) {
         set_error("precision must be LSD_PLAIN_F64 or LSD_MIXED_F32");
         return LSD_E_CONFIG;
     }
-    if (c->precision != LSD_PLAIN_F64 && ph > 512) {
-        set_error("the mixed forms need Ph <= 512 (fused physical-space pass)");
+    if (c->precision != LSD_PLAIN_F64 && ph > 1024) {
+        set_error("the mixed forms need Ph <= 1024 (fused physical-space pass)");
         return LSD_E_CONFIG;
     }
     if (H) *H = c->L * c->M - c->Mp * (c->M - 1) * c->M / 2;

@@ -28,7 +28,10 @@
 
 namespace lsd {
 
+// Threads per workgroup: one per phi point, 512 for Ph <= 512 and 1024 up
+// to Ph = 1024.
 constexpr int PF_THREADS = 512;
+constexpr int PF_THREADS_MAX = 1024;
 constexpr int PF_EPT = 8;      // 8 sequences x 512 points = one radix-8 butterfly per thread
 constexpr int PF_MAXSEQ = 8;
 
@@ -61,10 +64,10 @@ __device__ __forceinline__ int quot(int i, float inv) { return (int)(((float)i +
 // f1+2 -- into Hermitian-extended complex sequences (row jp real part, row
 // jm imaginary part); the DC imaginary parts are dropped as a c2r transform
 // does.
-template <typename V>
+template <int NT, typename V>
 __device__ void pack_inputs(V* Z, const V* __restrict__ fourier, int f0, int f1, int nb, int sh, int N, int M,
                             int MP, int Th, int Ph, int jp, int jm, float inv_m, float inv_w) {
-    for (int idx = threadIdx.x; idx < nb * M; idx += PF_THREADS) {
+    for (int idx = threadIdx.x; idx < nb * M; idx += NT) {
         const int b = quot(idx, inv_m), k = idx - b * M;
         const int fid = b < 3 ? f0 + b : f1 + b - 3;
         const V* X = fourier + ((size_t)fid * N + sh) * Th * MP + k;
@@ -78,7 +81,7 @@ __device__ void pack_inputs(V* Z, const V* __restrict__ fourier, int f0, int f1,
         }
     }
     const int W = Ph - 2 * M + 1;  // zero band k = M .. Ph-M
-    for (int idx = threadIdx.x; idx < nb * W; idx += PF_THREADS) {
+    for (int idx = threadIdx.x; idx < nb * W; idx += NT) {
         const int b = quot(idx, inv_w), k = M + (idx - b * W);
         Z[lidx<true>(b * Ph + k)] = cx<V>(0, 0);
     }
@@ -145,14 +148,14 @@ __device__ void stage_twiddles(V* twl, const V* __restrict__ tw) {
     }
 }
 
-template <int SG, bool R512, typename V>
+template <int SG, bool R512, int NT, typename V>
 __device__ __forceinline__ void fft_pf(V* Z, const FftPlan& pl, int nf, const V* tw) {
     if constexpr (R512) {
         stage512<1, SG>(Z, nf, tw);
         stage512<8, SG>(Z, nf, tw);
         stage512<64, SG>(Z, nf, tw);
     } else {
-        fft_lds<V, PF_THREADS, PF_EPT, true>(Z, pl, nf, tw, (decltype(V().x))SG);
+        fft_lds<V, NT, PF_EPT, true>(Z, pl, nf, tw, (decltype(V().x))SG);
     }
 }
 
@@ -173,8 +176,10 @@ size_t physfuse_lds(int Ph, bool r512) {
     return sizeof(V) * ((r512 ? TW512 : Ph) + (size_t)PF_MAXSEQ * Ph / 8 * 9);
 }
 
-template <typename R, bool R512, int PP>
-__global__ void __launch_bounds__(PF_THREADS) k_physfuse(PhysArgs<R> a) {
+template <typename R, bool R512, int NT>
+__global__ void __launch_bounds__(NT) k_physfuse(PhysArgs<R> a) {
+    static_assert(!R512 || NT == PF_THREADS, "length-512 stages take one butterfly per thread of 512");
+    constexpr int PP = 1;  // phi points per thread
     typedef typename Cx<R>::t V;
     extern __shared__ unsigned char lds_raw[];
     // twiddles first (read by every FFT stage), then the padded sequences
@@ -192,7 +197,7 @@ __global__ void __launch_bounds__(PF_THREADS) k_physfuse(PhysArgs<R> a) {
     const int t = threadIdx.x;
     if constexpr (R512) stage_twiddles(twl, a.tw);
     else
-        for (int i = t; i < Ph; i += PF_THREADS) twl[i] = a.tw[i];
+        for (int i = t; i < Ph; i += NT) twl[i] = a.tw[i];
     const R sn = (R)a.sn[jp];
     const R gpr[2] = {(R)a.gp[jp], (R)a.gp[jm]};
 
@@ -200,11 +205,11 @@ __global__ void __launch_bounds__(PF_THREADS) k_physfuse(PhysArgs<R> a) {
 
     // u and curl u: advection and Coriolis
     {
-        pack_inputs(Z, a.fourier, LSD_P_VEL_R, LSD_P_VCURL_R, 6, sh, N, M, a.MP, Th, Ph, jp, jm, a.inv_m, a.inv_w);
-        fft_pf<1, R512>(Z, a.pl, 6, twl);
+        pack_inputs<NT>(Z, a.fourier, LSD_P_VEL_R, LSD_P_VCURL_R, 6, sh, N, M, a.MP, Th, Ph, jp, jm, a.inv_m, a.inv_w);
+        fft_pf<1, R512, NT>(Z, a.pl, 6, twl);
 #pragma unroll
         for (int p = 0; p < PP; p++) {
-            const int x = t + p * PF_THREADS;
+            const int x = t + p * NT;
             if (x >= Ph) continue;
             V z[6];
 #pragma unroll
@@ -232,11 +237,11 @@ __global__ void __launch_bounds__(PF_THREADS) k_physfuse(PhysArgs<R> a) {
     }
     // B and curl B: Lorentz force and induction E = u x B
     if (a.mag) {
-        pack_inputs(Z, a.fourier, LSD_P_MAG_R, LSD_P_MCURL_R, 6, sh, N, M, a.MP, Th, Ph, jp, jm, a.inv_m, a.inv_w);
-        fft_pf<1, R512>(Z, a.pl, 6, twl);
+        pack_inputs<NT>(Z, a.fourier, LSD_P_MAG_R, LSD_P_MCURL_R, 6, sh, N, M, a.MP, Th, Ph, jp, jm, a.inv_m, a.inv_w);
+        fft_pf<1, R512, NT>(Z, a.pl, 6, twl);
 #pragma unroll
         for (int p = 0; p < PP; p++) {
-            const int x = t + p * PF_THREADS;
+            const int x = t + p * NT;
             if (x >= Ph) continue;
             V z[6];
 #pragma unroll
@@ -259,13 +264,13 @@ __global__ void __launch_bounds__(PF_THREADS) k_physfuse(PhysArgs<R> a) {
     // grad C and grad xi: advection of codensity and composition
     if (a.cod || a.comp) {
         const int nb = 3 * (a.cod + a.comp);
-        pack_inputs(Z, a.fourier, a.cod ? LSD_P_COD_GR : LSD_P_COMP_GR, LSD_P_COMP_GR, nb, sh, N, M, a.MP, Th, Ph, jp,
+        pack_inputs<NT>(Z, a.fourier, a.cod ? LSD_P_COD_GR : LSD_P_COMP_GR, LSD_P_COMP_GR, nb, sh, N, M, a.MP, Th, Ph, jp,
                     jm, a.inv_m, a.inv_w);
-        fft_pf<1, R512>(Z, a.pl, nb, twl);
+        fft_pf<1, R512, NT>(Z, a.pl, nb, twl);
         const int oc = 0, ox = a.cod ? 3 : 0;
 #pragma unroll
         for (int p = 0; p < PP; p++) {
-            const int x = t + p * PF_THREADS;
+            const int x = t + p * NT;
             if (x >= Ph) continue;
 #pragma unroll
             for (int r = 0; r < 2; r++) {
@@ -293,7 +298,7 @@ __global__ void __launch_bounds__(PF_THREADS) k_physfuse(PhysArgs<R> a) {
     const R sc = a.scale;
 #pragma unroll
     for (int p = 0; p < PP; p++) {
-        const int x = t + p * PF_THREADS;
+        const int x = t + p * NT;
         if (x >= Ph) continue;
         int s = 0;
 #pragma unroll
@@ -306,10 +311,10 @@ __global__ void __launch_bounds__(PF_THREADS) k_physfuse(PhysArgs<R> a) {
         }
     }
     __syncthreads();
-    fft_pf<-1, R512>(Z, a.pl, nout, twl);
+    fft_pf<-1, R512, NT>(Z, a.pl, nout, twl);
     // separate the two rows and fold: G0 = X_j + X_mirror, G1 = X_j - X_mirror
     const size_t fstride = (size_t)N * M * 2 * a.Th2;
-    for (int idx = t; idx < nout * M * 2; idx += PF_THREADS) {
+    for (int idx = t; idx < nout * M * 2; idx += NT) {
         const int s = quot(idx, a.inv_2m), rem = idx - s * 2 * M;
         const int k = rem >> 1, par = rem & 1;
         const V zk = Z[lidx<true>(s * Ph + k)];
@@ -324,7 +329,7 @@ __global__ void __launch_bounds__(PF_THREADS) k_physfuse(PhysArgs<R> a) {
     }
 }
 
-bool physfuse_supported(const Ctx& c) { return c.Ph <= 512 && c.Th % 2 == 0; }
+bool physfuse_supported(const Ctx& c) { return c.Ph <= PF_THREADS_MAX && c.Th % 2 == 0; }
 
 template <typename R>
 static int physfuse_launch(Ctx& c, const void* fourier, void* folded, const void* tw) {
@@ -363,15 +368,19 @@ static int physfuse_launch(Ctx& c, const void* fourier, void* folded, const void
     // algorithmic bytes: Fourier fields in, folded fields out
     ProfScope ps(c, LSD_K_PHYS,
                  (double)sizeof(V) * c.N * c.M * ((double)nin * c.Th + (double)nout * 2 * c.Th2));
-    if (r512) hipLaunchKernelGGL((k_physfuse<R, true, 1>), dim3(nblk), dim3(PF_THREADS), lds, c.stream, a);
-    else hipLaunchKernelGGL((k_physfuse<R, false, 1>), dim3(nblk), dim3(PF_THREADS), lds, c.stream, a);
+    if (r512) hipLaunchKernelGGL((k_physfuse<R, true, PF_THREADS>), dim3(nblk), dim3(PF_THREADS), lds, c.stream, a);
+    else if (c.Ph <= PF_THREADS)
+        hipLaunchKernelGGL((k_physfuse<R, false, PF_THREADS>), dim3(nblk), dim3(PF_THREADS), lds, c.stream, a);
+    else
+        hipLaunchKernelGGL((k_physfuse<R, false, PF_THREADS_MAX>), dim3(nblk), dim3(PF_THREADS_MAX), lds, c.stream,
+                           a);
     LSD_HIP(hipGetLastError());
     return LSD_OK;
 }
 
 int launch_physfuse(Ctx& c, bool f32) {
     if (!physfuse_supported(c)) {
-        set_error("fused physical-space pass needs Ph <= 512");
+        set_error("fused physical-space pass needs Ph <= 1024 and Th even");
         return LSD_E_CONFIG;
     }
     if (f32) return physfuse_launch<float>(c, c.d_fourier, c.d_folded, c.d_tw32);
