@@ -11,18 +11,20 @@
 // transform of both rows share every FFT, and the forward output separates
 // directly into the folded north+south / north-south buckets the analysis
 // consumes.  Inputs go through LDS in batches of up to six fields (u and
-// curl u; B and curl B; grad C and grad xi); each of the 512 threads keeps
-// the values of its own phi point in registers between batches, and does
-// one radix-8 butterfly per FFT stage.  The sequences are stored with one
-// pad slot per 8 complex values, against bank conflicts of the stride-8
+// curl u; B and curl B; grad C and grad xi), each loaded into registers one
+// batch ahead; each thread keeps the values of its own phi point in
+// registers between batches, and does one radix-8 butterfly (length 512) or
+// one point (general lengths) per FFT stage.  The sequences are stored with
+// one pad slot per 8 complex values, against bank conflicts of the stride-8
 // Stockham writes.
 //
-// Fourier data are stored [f][n][m][j]: the complex value of row j is one
-// piece of a 128-byte line shared by 16 (FP32) or 8 (FP64) consecutive
-// row pairs, and so is each folded output value.  Blocks are numbered so
-// that the row pairs of one line are consecutive workgroups on the same
-// XCD (workgroups go round-robin over the 8 XCDs), which makes them share
-// each line through that XCD's L2.
+// Fourier data are stored [f][n][j][m] (m padded to MP), so a row's values
+// are contiguous.  Folded outputs are stored [o][n][m][parity][j]: a value
+// is one piece of a 128-byte line shared by 16 (FP32) or 8 (FP64)
+// consecutive row pairs.  Blocks are numbered so that the row pairs of one
+// line are consecutive workgroups on the same XCD (workgroups go
+// round-robin over the 8 XCDs), which makes them share each line through
+// that XCD's L2.
 #include "fft_lds.h"
 #include "lsd_internal.h"
 
@@ -60,12 +62,68 @@ struct PhysArgs {
 // its floor is q.
 __device__ __forceinline__ int quot(int i, float inv) { return (int)(((float)i + 0.5f) * inv); }
 
-// Packs nb fields of row pair (jp, jm) -- f0, f0+1, f0+2, then f1, f1+1,
-// f1+2 -- into Hermitian-extended complex sequences (row jp real part, row
-// jm imaginary part); the DC imaginary parts are dropped as a c2r transform
-// does.
+// One batch of inputs -- nb fields of row pair (jp, jm): f0, f0+1, f0+2,
+// then f1, f1+1, f1+2 -- held in registers from its loads to its LDS pack,
+// so that the loads of the next batch are in flight while the current one
+// is transformed (a barrier waits on LDS operations only).  Item i of
+// thread t is value idx = t + i NT of the batch's nb M (field, m) values;
+// since M <= Ph / 2 <= NT / 2, 6 M <= 3 NT and three items cover a batch.
+constexpr int PF_ITEMS = 3;
+
+template <typename V>
+struct Batch {
+    V A[PF_ITEMS], B[PF_ITEMS];  // rows jp and jm
+};
+
 template <int NT, typename V>
-__device__ void pack_inputs(V* Z, const V* __restrict__ fourier, int f0, int f1, int nb, int sh, int N, int M,
+__device__ __forceinline__ void load_batch(Batch<V>& bt, const V* __restrict__ fourier, int f0, int f1, int nb,
+                                           int sh, int N, int M, int MP, int Th, int jp, int jm, float inv_m) {
+    // no branch holds the loads (lanes past the end re-read the last value;
+    // an empty batch reads value 0 of f0), so the values are not copied
+    // at a join, which would wait for them before the pack
+    const int last = max(nb * M, 1) - 1;
+#pragma unroll
+    for (int i = 0; i < PF_ITEMS; i++) {
+        const int idx = min((int)threadIdx.x + i * NT, last);
+        const int b = quot(idx, inv_m), k = idx - b * M;
+        const int fid = b < 3 ? f0 + b : f1 + b - 3;
+        const V* X = fourier + ((size_t)fid * N + sh) * Th * MP + k;
+        bt.A[i] = X[(size_t)jp * MP];
+        bt.B[i] = X[(size_t)jm * MP];
+    }
+}
+
+// Packs a loaded batch into Hermitian-extended complex sequences (row jp
+// real part, row jm imaginary part); the DC imaginary parts are dropped as a
+// c2r transform does.  The caller synchronises before the transform.
+template <int NT, typename V>
+__device__ __forceinline__ void pack_batch(V* Z, const Batch<V>& bt, int nb, int M, int Ph, float inv_m,
+                                           float inv_w) {
+#pragma unroll
+    for (int i = 0; i < PF_ITEMS; i++) {
+        const int idx = (int)threadIdx.x + i * NT;
+        if (i * NT < nb * M && idx < nb * M) {
+            const int b = quot(idx, inv_m), k = idx - b * M;
+            const V A = bt.A[i], B = bt.B[i];
+            const int z = b * Ph;
+            if (k == 0) {
+                Z[lidx<true>(z)] = cx<V>(A.x, B.x);
+            } else {
+                Z[lidx<true>(z + k)] = cx<V>(A.x - B.y, A.y + B.x);
+                Z[lidx<true>(z + Ph - k)] = cx<V>(A.x + B.y, B.x - A.y);
+            }
+        }
+    }
+    const int W = Ph - 2 * M + 1;  // zero band k = M .. Ph-M
+    for (int idx = threadIdx.x; idx < nb * W; idx += NT) {
+        const int b = quot(idx, inv_w), k = M + (idx - b * W);
+        Z[lidx<true>(b * Ph + k)] = cx<V>(0, 0);
+    }
+}
+
+// Loads and packs a batch in one pass (the form without lookahead).
+template <int NT, typename V>
+__device__ void pack_direct(V* Z, const V* __restrict__ fourier, int f0, int f1, int nb, int sh, int N, int M,
                             int MP, int Th, int Ph, int jp, int jm, float inv_m, float inv_w) {
     for (int idx = threadIdx.x; idx < nb * M; idx += NT) {
         const int b = quot(idx, inv_m), k = idx - b * M;
@@ -80,12 +138,11 @@ __device__ void pack_inputs(V* Z, const V* __restrict__ fourier, int f0, int f1,
             Z[lidx<true>(z + Ph - k)] = cx<V>(A.x + B.y, B.x - A.y);
         }
     }
-    const int W = Ph - 2 * M + 1;  // zero band k = M .. Ph-M
+    const int W = Ph - 2 * M + 1;
     for (int idx = threadIdx.x; idx < nb * W; idx += NT) {
         const int b = quot(idx, inv_w), k = M + (idx - b * W);
         Z[lidx<true>(b * Ph + k)] = cx<V>(0, 0);
     }
-    __syncthreads();
 }
 
 // Radix-8 stage of a length-512 transform (512 = 8^3) with every index a
@@ -195,6 +252,23 @@ __global__ void __launch_bounds__(NT) k_physfuse(PhysArgs<R> a) {
     const int jm = a.Th - 1 - jp;
     const int Ph = a.pl.n, N = a.N, M = a.M, Th = a.Th;
     const int t = threadIdx.x;
+    // batches in order: u and curl u; B and curl B (if magnetic); grad C
+    // and grad xi (as enabled).  Each is loaded one batch ahead.
+    const int ng = 3 * (a.cod + a.comp);
+    const int gf0 = a.cod ? LSD_P_COD_GR : LSD_P_COMP_GR;
+    // (the FP64 form at 1024 threads has no registers to spare: it loads
+    // each batch as it packs it)
+    constexpr bool AHEAD = !(sizeof(R) == 8 && NT == PF_THREADS_MAX);
+    Batch<V> inb;
+    auto load_grad = [&]() {
+        load_batch<NT>(inb, a.fourier, gf0, LSD_P_COMP_GR, ng, sh, N, M, a.MP, Th, jp, jm, a.inv_m);
+    };
+    auto pack = [&](int f0, int f1, int nb) {
+        if constexpr (AHEAD) pack_batch<NT>(Z, inb, nb, M, Ph, a.inv_m, a.inv_w);
+        else pack_direct<NT>(Z, a.fourier, f0, f1, nb, sh, N, M, a.MP, Th, Ph, jp, jm, a.inv_m, a.inv_w);
+    };
+    if constexpr (AHEAD)
+        load_batch<NT>(inb, a.fourier, LSD_P_VEL_R, LSD_P_VCURL_R, 6, sh, N, M, a.MP, Th, jp, jm, a.inv_m);
     if constexpr (R512) stage_twiddles(twl, a.tw);
     else
         for (int i = t; i < Ph; i += NT) twl[i] = a.tw[i];
@@ -205,7 +279,15 @@ __global__ void __launch_bounds__(NT) k_physfuse(PhysArgs<R> a) {
 
     // u and curl u: advection and Coriolis
     {
-        pack_inputs<NT>(Z, a.fourier, LSD_P_VEL_R, LSD_P_VCURL_R, 6, sh, N, M, a.MP, Th, Ph, jp, jm, a.inv_m, a.inv_w);
+        pack(LSD_P_VEL_R, LSD_P_VCURL_R, 6);
+        __syncthreads();
+        if constexpr (AHEAD) {
+            // the next batch, chosen by uniform selects rather than branches
+            const int f0 = a.mag ? LSD_P_MAG_R : (ng ? gf0 : LSD_P_VEL_R);
+            const int f1 = a.mag ? LSD_P_MCURL_R : LSD_P_COMP_GR;
+            const int nb = a.mag ? 6 : ng;
+            load_batch<NT>(inb, a.fourier, f0, f1, nb, sh, N, M, a.MP, Th, jp, jm, a.inv_m);
+        }
         fft_pf<1, R512, NT>(Z, a.pl, 6, twl);
 #pragma unroll
         for (int p = 0; p < PP; p++) {
@@ -237,7 +319,11 @@ __global__ void __launch_bounds__(NT) k_physfuse(PhysArgs<R> a) {
     }
     // B and curl B: Lorentz force and induction E = u x B
     if (a.mag) {
-        pack_inputs<NT>(Z, a.fourier, LSD_P_MAG_R, LSD_P_MCURL_R, 6, sh, N, M, a.MP, Th, Ph, jp, jm, a.inv_m, a.inv_w);
+        pack(LSD_P_MAG_R, LSD_P_MCURL_R, 6);
+        __syncthreads();
+        if constexpr (AHEAD) {
+            if (ng) load_grad();
+        }
         fft_pf<1, R512, NT>(Z, a.pl, 6, twl);
 #pragma unroll
         for (int p = 0; p < PP; p++) {
@@ -262,11 +348,10 @@ __global__ void __launch_bounds__(NT) k_physfuse(PhysArgs<R> a) {
         __syncthreads();
     }
     // grad C and grad xi: advection of codensity and composition
-    if (a.cod || a.comp) {
-        const int nb = 3 * (a.cod + a.comp);
-        pack_inputs<NT>(Z, a.fourier, a.cod ? LSD_P_COD_GR : LSD_P_COMP_GR, LSD_P_COMP_GR, nb, sh, N, M, a.MP, Th, Ph, jp,
-                    jm, a.inv_m, a.inv_w);
-        fft_pf<1, R512, NT>(Z, a.pl, nb, twl);
+    if (ng) {
+        pack(gf0, LSD_P_COMP_GR, ng);
+        __syncthreads();
+        fft_pf<1, R512, NT>(Z, a.pl, ng, twl);
         const int oc = 0, ox = a.cod ? 3 : 0;
 #pragma unroll
         for (int p = 0; p < PP; p++) {
@@ -329,7 +414,10 @@ __global__ void __launch_bounds__(NT) k_physfuse(PhysArgs<R> a) {
     }
 }
 
-bool physfuse_supported(const Ctx& c) { return c.Ph <= PF_THREADS_MAX && c.Th % 2 == 0; }
+bool physfuse_supported(const Ctx& c) {
+    const int nt = c.Ph <= PF_THREADS ? PF_THREADS : PF_THREADS_MAX;
+    return c.Ph <= PF_THREADS_MAX && c.Th % 2 == 0 && 6 * c.M <= PF_ITEMS * nt;
+}
 
 template <typename R>
 static int physfuse_launch(Ctx& c, const void* fourier, void* folded, const void* tw) {
