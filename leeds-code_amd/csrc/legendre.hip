@@ -306,11 +306,15 @@ static int syn_launch(Ctx& c, int kind, int njob, const T* tA, const T* tB, cons
     return LSD_OK;
 }
 
+static int syn4_launch(Ctx& c, int kind, int njob, const float* tA, const float* tB, const void* const* in0,
+                       const void* const* in1, void* const* out0, void* const* out1);
+
 // njob transforms of one kind in one launch:
 // kind SYN_SCALAR: in0[k] -> out0[k] (table P)
 // kind SYN_VECTOR: (in0[k] = s, in1[k] = t) -> (out0[k] = theta, out1[k] = phi)
 // kind SYN_GRAD:   in0[k] -> (out0[k] = theta, out1[k] = phi)
-// f32: FP32 tables, inputs and outputs (float2 spectral / Fourier layouts).
+// f32: FP32 tables, inputs and outputs (float2 spectral / Fourier layouts),
+// on the FP32 kernel below.
 int launch_synthesis_group(Ctx& c, int kind, int njob, const void* const* in0, const void* const* in1,
                            void* const* out0, void* const* out1, bool f32) {
     size_t HT = (size_t)c.H * c.Th2;
@@ -318,7 +322,16 @@ int launch_synthesis_group(Ctx& c, int kind, int njob, const void* const* in0, c
     ProfScope ps(c, LSD_K_SYNTH, unit * (kind == SYN_SCALAR ? 1 : kind == SYN_VECTOR ? 4 : 2) * njob);
     const size_t o0 = (kind == SYN_SCALAR ? 0 : kind == SYN_VECTOR ? 3 : 1) * HT;
     const size_t o1 = (kind == SYN_VECTOR ? 4 : 2) * HT;
-    if (f32) return syn_launch<float>(c, kind, njob, c.d_tab32 + o0, c.d_tab32 + o1, in0, in1, out0, out1);
+    if (f32) {
+        if (njob < 1 || njob > LG_MAXJ) {
+            set_error("synthesis: 1..6 transforms per launch");
+            return LSD_E_ARG;
+        }
+        const size_t HTp = (size_t)c.Hp * c.Th2p;
+        const size_t p0 = (kind == SYN_SCALAR ? 0 : kind == SYN_VECTOR ? 3 : 1) * HTp;
+        const size_t p1 = (kind == SYN_VECTOR ? 4 : 2) * HTp;
+        return syn4_launch(c, kind, njob, c.d_tab32 + p0, c.d_tab32 + p1, in0, in1, out0, out1);
+    }
     return syn_launch<double>(c, kind, njob, c.d_tab + o0, c.d_tab + o1, in0, in1, out0, out1);
 }
 
@@ -332,6 +345,234 @@ int launch_synthesis_p(Ctx& c, int kind, const void* in0, const void* in1, void*
 
 int launch_synthesis(Ctx& c, int kind, const double2* in0, const double2* in1, double2* out0, double2* out1) {
     return launch_synthesis_p(c, kind, in0, in1, out0, out1, false);
+}
+
+// ------------------------------------------------- FP32 synthesis (mixed forms)
+//
+// The same contractions with operands loaded 16 and 8 bytes per lane.  The
+// FP32 tables are stored per slot (rows padded with zeros to a multiple of
+// 8 harmonics) in rows of Th2p = Th2 rounded up to 64 theta points, each
+// block of 64 permuted so that position 4 l + t holds theta 16 t + l: one
+// 16-byte load per lane carries the B operands of four 16-theta MFMA tiles
+// (theta 16 t + l16, t = 0..3).  A lane's 8-byte load of a coefficient row
+// is the (re, im) pair of one shell, the A operands of the two column tiles
+// (u = re, im), so the i rotations of the vector and gradient terms are a
+// swap within the lane.  A wave computes 64 theta x 16 shells for one m; per
+// two harmonics it runs 8 (scalar), 16 (gradient) or 32 (vector) MFMAs
+// against two (three, four) loads.  The four waves of a workgroup take four
+// consecutive m and stage their results through LDS, so each (shell, theta
+// row) is written as one 32-byte run of four m.
+constexpr int S4_TH = 64;                  // theta points per wave tile
+constexpr int S4_SH = 16;                  // shells per wave tile
+constexpr int S4_MQ = 4;                   // consecutive m per workgroup, one per wave
+constexpr int S4_PD = 2;                   // steps (8 harmonics) per prefetch group
+constexpr int S4_SHS = S4_TH * S4_MQ + 2;  // LDS float2 per (mirror, shell), padded
+
+struct Syn4Args {
+    int N, Th, Th2, Th2p, M, MP, H;
+    int nth, nsb, nmo, njob;  // theta tiles, shell blocks, groups of 4 m quads, jobs
+    const Slot* slots;
+    const int* mslot;
+    const int* t0;            // [nslot] first table row of each slot
+    const float* tabA;        // [Hp][Th2p]
+    const float* tabB;
+    const double* rinv;
+    const float* in0[LG_MAXJ];  // FP32 coll [h][2N]
+    const float* in1[LG_MAXJ];
+    float* out0[LG_MAXJ];       // FP32 fourier [n][Th][MP] complex
+    float* out1[LG_MAXJ];
+};
+
+struct S4Ops {
+    float4 ta[2], tb[2];  // tables, harmonics k and k+1 of this lane group
+    float2 x[2], t[2];    // coefficients (re, im) of the lane's shell
+};
+
+__device__ __forceinline__ v4f mfma4(float a, float b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(64 * S4_MQ) k_syn4(Syn4Args a) {
+    constexpr int NOUT = KIND == 0 ? 1 : 2;
+    __shared__ float2 St[2 * S4_SH * S4_SHS];
+    // groups (16 m, job, shell block); members (m quad, theta tile), the
+    // theta tiles of one m quad consecutive: they read the same coefficients
+    const BlockId bid = decode_block(a.nmo * a.njob * a.nsb, S4_MQ * a.nth);
+    if (!bid.valid) return;
+    const int mo = bid.group / (a.njob * a.nsb);
+    const int gjs = bid.group - mo * a.njob * a.nsb;
+    const int job = gjs / a.nsb, sb = gjs - job * a.nsb;
+    const int mq = mo * S4_MQ + bid.cc / a.nth, tt = bid.cc % a.nth;
+    const int m0 = mq * S4_MQ;
+    if (m0 >= a.M) return;
+    const int jb = tt * S4_TH, n0 = sb * S4_SH;
+    const float* in0 = pickj(a.in0, job);
+    const float* in1 = pickj(a.in1, job);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l16 = lane & 15, q4 = lane >> 4;
+    const int C2 = 2 * a.N;
+    const int ncl = min(n0 + l16, a.N - 1);  // rows of shells >= N are computed and not stored
+    const int m = m0 + wave;
+    v4f Y[NOUT][2][2][4];  // [output][parity bucket][re, im][theta tile]
+#pragma unroll
+    for (int o = 0; o < NOUT; o++)
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+            for (int u = 0; u < 2; u++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) Y[o][p][u][t] = v4f{0, 0, 0, 0};
+    if (m < a.M) {
+        const int si = a.mslot[m];
+        const Slot s = a.slots[si];
+        const int rows = (s.len + 7) & ~7;
+        const size_t tb0 = (size_t)a.t0[si] * a.Th2p + jb + 4 * l16;
+        // rows are clamped rather than branched on, so no copy at a join
+        // waits for the loads; steps past the slot's rows are never run
+        auto load = [&](S4Ops& op, int k0) {
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                const int k = min(k0 + 2 * q4 + p, rows - 1);
+                op.ta[p] = *reinterpret_cast<const float4*>(a.tabA + tb0 + (size_t)k * a.Th2p);
+                if (KIND != 0) op.tb[p] = *reinterpret_cast<const float4*>(a.tabB + tb0 + (size_t)k * a.Th2p);
+                // padding rows (k >= len) have zero tables; their coefficient
+                // row is any valid one
+                const size_t h = (size_t)min(s.h0 + k, a.H - 1);
+                op.x[p] = *reinterpret_cast<const float2*>(in0 + h * C2 + 2 * ncl);
+                if (KIND == 1) op.t[p] = *reinterpret_cast<const float2*>(in1 + h * C2 + 2 * ncl);
+            }
+        };
+        auto step = [&](const S4Ops& op) {
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                const float ta[4] = {op.ta[p].x, op.ta[p].y, op.ta[p].z, op.ta[p].w};
+                const float x[2] = {op.x[p].x, op.x[p].y};
+                if constexpr (KIND == 0) {
+#pragma unroll
+                    for (int u = 0; u < 2; u++)
+#pragma unroll
+                        for (int t = 0; t < 4; t++) Y[0][p][u][t] = mfma4(x[u], ta[t], Y[0][p][u][t]);
+                } else if constexpr (KIND == 2) {
+                    // theta: c * dP (bucket 1-p);  phi: (i c) * mP/sin (bucket p)
+                    const float tb[4] = {op.tb[p].x, op.tb[p].y, op.tb[p].z, op.tb[p].w};
+                    const float xr[2] = {-x[1], x[0]};
+#pragma unroll
+                    for (int u = 0; u < 2; u++)
+#pragma unroll
+                        for (int t = 0; t < 4; t++) {
+                            Y[0][1 - p][u][t] = mfma4(x[u], ta[t], Y[0][1 - p][u][t]);
+                            Y[1][p][u][t] = mfma4(xr[u], tb[t], Y[1][p][u][t]);
+                        }
+                } else {
+                    // (-i t): re <- t_im, im <- -t_re ;  (i s): re <- -s_im, im <- s_re
+                    const float tb[4] = {op.tb[p].x, op.tb[p].y, op.tb[p].z, op.tb[p].w};
+                    const float tv[2] = {op.t[p].x, op.t[p].y};
+                    const float tr[2] = {tv[1], -tv[0]}, sr[2] = {-x[1], x[0]};
+#pragma unroll
+                    for (int u = 0; u < 2; u++)
+#pragma unroll
+                        for (int t = 0; t < 4; t++) {
+                            Y[0][1 - p][u][t] = mfma4(x[u], ta[t], Y[0][1 - p][u][t]);
+                            Y[0][p][u][t] = mfma4(tr[u], tb[t], Y[0][p][u][t]);
+                            Y[1][p][u][t] = mfma4(sr[u], tb[t], Y[1][p][u][t]);
+                            Y[1][1 - p][u][t] = mfma4(tv[u], ta[t], Y[1][1 - p][u][t]);
+                        }
+                }
+            }
+        };
+        // groups of S4_PD steps (8 harmonics each), two groups in flight
+        S4Ops ga[S4_PD], gb[S4_PD];
+#pragma unroll
+        for (int u = 0; u < S4_PD; u++) load(ga[u], 8 * u);
+        for (int k0 = 0; k0 < s.len; k0 += 16 * S4_PD) {
+            const int k1 = k0 + 8 * S4_PD;
+#pragma unroll
+            for (int u = 0; u < S4_PD; u++) load(gb[u], k1 + 8 * u);
+#pragma unroll
+            for (int u = 0; u < S4_PD; u++)
+                if (k0 + 8 * u < s.len) step(ga[u]);
+            if (k1 >= s.len) break;
+#pragma unroll
+            for (int u = 0; u < S4_PD; u++) load(ga[u], k1 + 8 * S4_PD + 8 * u);
+#pragma unroll
+            for (int u = 0; u < S4_PD; u++)
+                if (k1 + 8 * u < s.len) step(gb[u]);
+        }
+    }
+    // F[j] = Y0 + Y1, F[Th-1-j] = Y0 - Y1 into LDS [mirror][shell][theta][m]
+    // (lane: theta 16 t + l16, shells 4 q4 + r), then one 32-byte run per
+    // (shell, theta row)
+#pragma unroll
+    for (int o = 0; o < NOUT; o++) {
+        if (o) __syncthreads();  // the previous output's reads are done
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int sh = 4 * q4 + r, th = 16 * t + l16;
+                const float y0r = Y[o][0][0][t][r], y1r = Y[o][1][0][t][r];
+                const float y0i = Y[o][0][1][t][r], y1i = Y[o][1][1][t][r];
+                St[sh * S4_SHS + th * S4_MQ + wave] = make_float2(y0r + y1r, y0i + y1i);
+                St[(S4_SH + sh) * S4_SHS + th * S4_MQ + wave] = make_float2(y0r - y1r, y0i - y1i);
+            }
+        __syncthreads();
+        float* out = o == 0 ? pickj(a.out0, job) : pickj(a.out1, job);
+        for (int idx = tid; idx < 2 * S4_SH * S4_TH; idx += 64 * S4_MQ) {
+            const int th = idx % S4_TH, rest = idx / S4_TH;
+            const int sh = rest % S4_SH, mir = rest / S4_SH;
+            const int n = n0 + sh, j = jb + th;
+            if (n >= a.N || j >= a.Th2) continue;
+            const int jrow = mir ? a.Th - 1 - j : j;
+            if (mir && jrow < a.Th2) continue;
+            const float2* sp = St + (mir * S4_SH + sh) * S4_SHS + th * S4_MQ;
+            float4 v0 = make_float4(sp[0].x, sp[0].y, sp[1].x, sp[1].y);
+            float4 v1 = make_float4(sp[2].x, sp[2].y, sp[3].x, sp[3].y);
+            if constexpr (KIND == 2) {
+                const float f = (float)a.rinv[n];
+                v0.x *= f; v0.y *= f; v0.z *= f; v0.w *= f;
+                v1.x *= f; v1.y *= f; v1.z *= f; v1.w *= f;
+            }
+            float4* d = reinterpret_cast<float4*>(out + 2 * (((size_t)n * a.Th + jrow) * a.MP + m0));
+            d[0] = v0;
+            d[1] = v1;
+        }
+    }
+}
+
+static int syn4_launch(Ctx& c, int kind, int njob, const float* tA, const float* tB, const void* const* in0,
+                       const void* const* in1, void* const* out0, void* const* out1) {
+    Syn4Args a;
+    a.N = c.N;
+    a.Th = c.Th;
+    a.Th2 = c.Th2;
+    a.Th2p = c.Th2p;
+    a.M = c.M;
+    a.MP = c.MP;
+    a.H = c.H;
+    a.slots = c.d_slots;
+    a.mslot = c.d_mslot;
+    a.t0 = c.d_slot_t0;
+    a.tabA = tA;
+    a.tabB = tB;
+    a.rinv = c.d_r + 2 * c.N;
+    for (int k = 0; k < LG_MAXJ; k++) {
+        const int q = k < njob ? k : 0;
+        a.in0[k] = reinterpret_cast<const float*>(in0[q]);
+        a.in1[k] = in1 ? reinterpret_cast<const float*>(in1[q]) : nullptr;
+        a.out0[k] = reinterpret_cast<float*>(out0[q]);
+        a.out1[k] = out1 ? reinterpret_cast<float*>(out1[q]) : nullptr;
+    }
+    a.nth = c.Th2p / S4_TH;
+    a.nsb = (c.N + S4_SH - 1) / S4_SH;
+    a.nmo = ((c.M + S4_MQ - 1) / S4_MQ + S4_MQ - 1) / S4_MQ;
+    a.njob = njob;
+    dim3 grid(grid_blocks(a.nmo * njob * a.nsb, S4_MQ * a.nth));
+    dim3 blk(64 * S4_MQ);
+    if (kind == SYN_SCALAR) hipLaunchKernelGGL(k_syn4<0>, grid, blk, 0, c.stream, a);
+    else if (kind == SYN_VECTOR) hipLaunchKernelGGL(k_syn4<1>, grid, blk, 0, c.stream, a);
+    else hipLaunchKernelGGL(k_syn4<2>, grid, blk, 0, c.stream, a);
+    LSD_HIP(hipGetLastError());
+    return LSD_OK;
 }
 
 // --------------------------------------------------------------- analysis

@@ -82,7 +82,7 @@ static void release(Ctx* c) {
                     c->d_sin, c->d_tab, c->d_atab, c->d_r, c->d_dr1, c->d_dr2, c->d_radlap, c->d_r1dr,
                     c->d_intr2dr, c->d_ll, c->d_lu, c->d_piv, c->d_pivpk, c->d_y, c->d_gre, c->d_invg, c->d_info,
                     c->d_lut32, c->d_xband, c->d_sblocks, c->d_sorder, c->d_tab32,
-                    c->d_atab32, c->d_tw32,
+                    c->d_atab32, c->d_tw32, c->d_slot_t0, c->d_row_h,
                     c->d_spec_in, c->d_spec_out, c->d_fourier, c->d_folded, c->d_phys, c->d_nlphys,
                     c->d_tw, c->d_partial};
     for (void* p : ptrs)
@@ -233,7 +233,20 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
     if ((rc = precompute_tables(*c))) return rc;
     if (c->mixed) {
         const size_t HT = (size_t)H * c->Th2;
-        if (dalloc(&c->d_tab32, 5 * HT) || dalloc(&c->d_atab32, 3 * HT)) return LSD_E_HIP;
+        c->Th2p = (c->Th2 + 63) / 64 * 64;
+        std::vector<int> t0(c->nslot), row_h;
+        for (int s = 0; s < c->nslot; s++) {
+            t0[s] = (int)row_h.size();
+            const int len = c->slots[s].len, rows = (len + 7) / 8 * 8;
+            for (int k = 0; k < rows; k++) row_h.push_back(k < len ? c->slots[s].h0 + k : -1);
+        }
+        c->Hp = (int)row_h.size();
+        if (dalloc(&c->d_tab32, (size_t)5 * c->Hp * c->Th2p) || dalloc(&c->d_atab32, 3 * HT) ||
+            dalloc(&c->d_slot_t0, c->nslot) || dalloc(&c->d_row_h, c->Hp))
+            return LSD_E_HIP;
+        LSD_HIP(hipMemcpyAsync(c->d_slot_t0, t0.data(), sizeof(int) * c->nslot, hipMemcpyHostToDevice, st));
+        LSD_HIP(hipMemcpyAsync(c->d_row_h, row_h.data(), sizeof(int) * c->Hp, hipMemcpyHostToDevice, st));
+        LSD_HIP(hipStreamSynchronize(st));
         if ((rc = tables_to_f32(*c))) return rc;
     }
     if ((rc = build_operators(*c))) return rc;

@@ -8,7 +8,8 @@
 //   folded   double2 [f][n][m][p][j]   j < Th2, p = equatorial parity bucket
 //                                       (north+south, south-north)
 //   phys     double  [f][n][k][j]      the reference's (Th, Ph, N) layout
-//   tables   synthesis  double [h][j]  (j < Th2 fastest, like leg_P(j,nh))
+//   tables   synthesis  double [h][j]  (j < Th2 fastest, like leg_P(j,nh));
+//                       FP32 form per slot, theta-permuted (see d_tab32)
 //            analysis   double [j][h]  (h fastest, so lanes over harmonics
 //                                       read coalesced)
 //   bands    mesh  double [n][7]       M(KL+1+n-j, j) stored per column j
@@ -107,8 +108,15 @@ struct Ctx {
     double* d_sin = nullptr;       // [Th]
     double* d_tab = nullptr;       // 5 synthesis tables [5][H][Th2]
     double* d_atab = nullptr;      // 3 analysis tables [3][Th2][H]
-    float* d_tab32 = nullptr;      // FP32 copies (mixed forms)
-    float* d_atab32 = nullptr;
+    // FP32 synthesis tables (mixed forms) [5][Hp][Th2p]: each slot's rows
+    // from row slot_t0[s], padded with zero rows to a multiple of 8; theta
+    // rounded up to Th2p = 64-multiple and permuted within each block of 64
+    // (position 4 l + t holds theta 16 t + l)
+    float* d_tab32 = nullptr;
+    int Th2p = 0, Hp = 0;
+    int* d_slot_t0 = nullptr;      // [nslot] first table row of each slot
+    int* d_row_h = nullptr;        // [Hp] harmonic of each table row, -1 for padding
+    float* d_atab32 = nullptr;     // FP32 copies of the analysis tables
     // device: radial mesh
     double* d_r = nullptr;         // [7][N]  r^p, p = -3..3
     double* d_dr1 = nullptr;       // [N][7]

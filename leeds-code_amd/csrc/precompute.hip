@@ -588,12 +588,30 @@ __global__ void k_to_f32(size_t n, const double* __restrict__ a, float* __restri
     if (i < n) b[i] = (float)a[i];
 }
 
+// FP32 synthesis tables in the per-slot, theta-permuted layout of the FP32
+// synthesis: entry (kind, row R, position q) holds table kind at harmonic
+// row_h[R] and theta 64 (q / 64) + 16 (q % 4) + (q % 64) / 4, zero for
+// padding rows and theta >= Th2.
+__global__ void k_tab_syn32(int H, int Th2, int Th2p, int Hp, const int* __restrict__ row_h,
+                            const double* __restrict__ tab, float* __restrict__ out) {
+    const size_t n = (size_t)5 * Hp * Th2p;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int q = (int)(i % Th2p);
+    const size_t kr = i / Th2p;
+    const int R = (int)(kr % Hp), kind = (int)(kr / Hp);
+    const int w = q & 63, th = (q - w) + 16 * (w & 3) + (w >> 2);
+    const int h = row_h[R];
+    out[i] = (h >= 0 && th < Th2) ? (float)tab[((size_t)kind * H + h) * Th2 + th] : 0.0f;
+}
+
 // FP32 copies of the synthesis and analysis tables, rounded from the FP64
 // recursion, for the mixed forms.
 int tables_to_f32(Ctx& c) {
     const size_t HT = (size_t)c.H * c.Th2;
-    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((5 * HT + 255) / 256)), dim3(256), 0, c.stream, 5 * HT, c.d_tab,
-                       c.d_tab32);
+    const size_t ns = (size_t)5 * c.Hp * c.Th2p;
+    hipLaunchKernelGGL(k_tab_syn32, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, c.stream, c.H, c.Th2, c.Th2p,
+                       c.Hp, c.d_row_h, c.d_tab, c.d_tab32);
     hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((3 * HT + 255) / 256)), dim3(256), 0, c.stream, 3 * HT, c.d_atab,
                        c.d_atab32);
     LSD_HIP(hipGetLastError());
