@@ -365,7 +365,9 @@ int launch_synthesis(Ctx& c, int kind, const double2* in0, const double2* in1, d
 constexpr int S4_TH = 64;                  // theta points per wave tile
 constexpr int S4_SH = 16;                  // shells per wave tile
 constexpr int S4_MQ = 4;                   // consecutive m per workgroup, one per wave
-constexpr int S4_PD = 2;                   // steps (8 harmonics) per prefetch group
+// operand buffers (one step of 8 harmonics each); with the 128 accumulators
+// of the vector and gradient kinds no more fit at two waves per SIMD
+constexpr int S4_NB = 4;
 constexpr int S4_SHS = S4_TH * S4_MQ + 2;  // LDS float2 per (mirror, shell), padded
 
 struct Syn4Args {
@@ -393,8 +395,10 @@ __device__ __forceinline__ v4f mfma4(float a, float b, v4f c) {
 }
 
 template <int KIND>
-__global__ void __launch_bounds__(64 * S4_MQ) k_syn4(Syn4Args a) {
+__global__ void __launch_bounds__(64 * S4_MQ) __attribute__((amdgpu_waves_per_eu(2)))
+k_syn4(Syn4Args a) {
     constexpr int NOUT = KIND == 0 ? 1 : 2;
+    constexpr int NB = S4_NB;
     __shared__ float2 St[2 * S4_SH * S4_SHS];
     // groups (16 m, job, shell block); members (m quad, theta tile), the
     // theta tiles of one m quad consecutive: they read the same coefficients
@@ -409,7 +413,7 @@ __global__ void __launch_bounds__(64 * S4_MQ) k_syn4(Syn4Args a) {
     const int jb = tt * S4_TH, n0 = sb * S4_SH;
     const float* in0 = pickj(a.in0, job);
     const float* in1 = pickj(a.in1, job);
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l16 = lane & 15, q4 = lane >> 4;
+    const int tid = threadIdx.x, wave = uniform(tid >> 6), lane = tid & 63, l16 = lane & 15, q4 = lane >> 4;
     const int C2 = 2 * a.N;
     const int ncl = min(n0 + l16, a.N - 1);  // rows of shells >= N are computed and not stored
     const int m = m0 + wave;
@@ -480,23 +484,24 @@ __global__ void __launch_bounds__(64 * S4_MQ) k_syn4(Syn4Args a) {
                 }
             }
         };
-        // groups of S4_PD steps (8 harmonics each), two groups in flight
-        S4Ops ga[S4_PD], gb[S4_PD];
+        // a ring of NB one-step buffers (8 harmonics each): a buffer is
+        // refilled, NB steps ahead, as soon as its step has been issued.  The
+        // refill is unconditional (rows clamped), so no join makes the loads
+        // wait early, and scheduling barriers keep the buffers' loads in ring
+        // order, so each step waits for its own buffer only.
+        S4Ops buf[NB];
 #pragma unroll
-        for (int u = 0; u < S4_PD; u++) load(ga[u], 8 * u);
-        for (int k0 = 0; k0 < s.len; k0 += 16 * S4_PD) {
-            const int k1 = k0 + 8 * S4_PD;
+        for (int u = 0; u < NB; u++) {
+            load(buf[u], 8 * u);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        for (int k0 = 0; k0 < s.len; k0 += 8 * NB) {
 #pragma unroll
-            for (int u = 0; u < S4_PD; u++) load(gb[u], k1 + 8 * u);
-#pragma unroll
-            for (int u = 0; u < S4_PD; u++)
-                if (k0 + 8 * u < s.len) step(ga[u]);
-            if (k1 >= s.len) break;
-#pragma unroll
-            for (int u = 0; u < S4_PD; u++) load(ga[u], k1 + 8 * S4_PD + 8 * u);
-#pragma unroll
-            for (int u = 0; u < S4_PD; u++)
-                if (k1 + 8 * u < s.len) step(gb[u]);
+            for (int u = 0; u < NB; u++) {
+                if (k0 + 8 * u < s.len) step(buf[u]);
+                load(buf[u], k0 + 8 * u + 8 * NB);
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
     }
     // F[j] = Y0 + Y1, F[Th-1-j] = Y0 - Y1 into LDS [mirror][shell][theta][m]
