@@ -606,8 +606,11 @@ constexpr int ANA_TPW = 4;     // max harmonic tiles per wave
 
 // VEC: q, s, t from three folded fields (tra_rtp2qst); else one field
 // (tra_phys2spec).  Workgroup: 4 waves, one slot, 32 columns.
+// FP32: two waves per SIMD (the compiler then fits the vector kind in 168
+// registers, without spills); FP64 keeps the whole register file
 template <typename T, bool VEC>
-__global__ void __launch_bounds__(256) k_ana(AnaArgs<T> a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1)))
+k_ana(AnaArgs<T> a) {
     typedef typename Mfma<T>::acc acc_t;
     constexpr int NF = VEC ? 3 : 1;
     constexpr int NO = VEC ? 3 : 1;
