@@ -359,16 +359,25 @@ int launch_synthesis(Ctx& c, int kind, const double2* in0, const double2* in1, d
 // (u = re, im), so the i rotations of the vector and gradient terms are a
 // swap within the lane.  A wave computes 64 theta x 16 shells for one m; per
 // two harmonics it runs 8 (scalar), 16 (gradient) or 32 (vector) MFMAs
-// against two (three, four) loads.  The four waves of a workgroup take four
-// consecutive m and stage their results through LDS, so each (shell, theta
-// row) is written as one 32-byte run of four m.
+// against two (three, four) loads.  The waves of a workgroup take
+// consecutive m (4 scalar, 8 vector and gradient) and stage their results
+// through LDS, so each (shell, theta row) is written as one run of 32 or 64
+// bytes.
 constexpr int S4_TH = 64;                  // theta points per wave tile
 constexpr int S4_SH = 16;                  // shells per wave tile
-constexpr int S4_MQ = 4;                   // consecutive m per workgroup, one per wave
+constexpr int S4_MG = 16;                  // m per block group (one 128-byte line of a Fourier row)
+// consecutive m per workgroup (one per wave) and hemispheres staged in LDS
+// per pass: the scalar kind 4 and both, the vector and gradient kinds 8 and
+// one (66 KB of LDS either way)
+template <int KIND>
+constexpr int S4_MQ = KIND == 0 ? 4 : 8;
+template <int KIND>
+constexpr int S4_NMIR = KIND == 0 ? 2 : 1;
 // operand buffers (one step of 8 harmonics each); with the 128 accumulators
 // of the vector and gradient kinds no more fit at two waves per SIMD
 constexpr int S4_NB = 4;
-constexpr int S4_SHS = S4_TH * S4_MQ + 2;  // LDS float2 per (mirror, shell), padded
+template <int KIND>
+constexpr int S4_SHS = S4_TH * S4_MQ<KIND> + 2;  // LDS float2 per (hemisphere, shell), padded
 
 struct Syn4Args {
     int N, Th, Th2, Th2p, M, MP, H;
@@ -395,20 +404,21 @@ __device__ __forceinline__ v4f mfma4(float a, float b, v4f c) {
 }
 
 template <int KIND>
-__global__ void __launch_bounds__(64 * S4_MQ) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(64 * S4_MQ<KIND>) __attribute__((amdgpu_waves_per_eu(2)))
 k_syn4(Syn4Args a) {
     constexpr int NOUT = KIND == 0 ? 1 : 2;
     constexpr int NB = S4_NB;
-    __shared__ float2 St[2 * S4_SH * S4_SHS];
+    constexpr int MQ = S4_MQ<KIND>, NMIR = S4_NMIR<KIND>, SHS = S4_SHS<KIND>;
+    __shared__ float2 St[NMIR * S4_SH * SHS];
     // groups (16 m, job, shell block); members (m quad, theta tile), the
     // theta tiles of one m quad consecutive: they read the same coefficients
-    const BlockId bid = decode_block(a.nmo * a.njob * a.nsb, S4_MQ * a.nth);
+    const BlockId bid = decode_block(a.nmo * a.njob * a.nsb, (S4_MG / MQ) * a.nth);
     if (!bid.valid) return;
     const int mo = bid.group / (a.njob * a.nsb);
     const int gjs = bid.group - mo * a.njob * a.nsb;
     const int job = gjs / a.nsb, sb = gjs - job * a.nsb;
-    const int mq = mo * S4_MQ + bid.cc / a.nth, tt = bid.cc % a.nth;
-    const int m0 = mq * S4_MQ;
+    const int mq = mo * (S4_MG / MQ) + bid.cc / a.nth, tt = bid.cc % a.nth;
+    const int m0 = mq * MQ;
     if (m0 >= a.M) return;
     const int jb = tt * S4_TH, n0 = sb * S4_SH;
     const float* in0 = pickj(a.in0, job);
@@ -504,12 +514,15 @@ k_syn4(Syn4Args a) {
             }
         }
     }
-    // F[j] = Y0 + Y1, F[Th-1-j] = Y0 - Y1 into LDS [mirror][shell][theta][m]
-    // (lane: theta 16 t + l16, shells 4 q4 + r), then one 32-byte run per
-    // (shell, theta row)
+    // F[j] = Y0 + Y1 (north), F[Th-1-j] = Y0 - Y1 (south), NMIR hemispheres
+    // of one output per pass, into LDS [hemisphere][shell][theta][m] (lane:
+    // theta 16 t + l16, shells 4 q4 + r), then one run of MQ m per (shell,
+    // theta row), written 16 bytes per lane by adjacent lanes (one request)
+    constexpr int NPASS = NOUT * (2 / NMIR), NQ = MQ / 2;
 #pragma unroll
-    for (int o = 0; o < NOUT; o++) {
-        if (o) __syncthreads();  // the previous output's reads are done
+    for (int pass = 0; pass < NPASS; pass++) {
+        const int o = pass / (2 / NMIR), mir0 = NMIR == 2 ? 0 : pass % 2;
+        if (pass) __syncthreads();  // the previous pass's reads are done
 #pragma unroll
         for (int t = 0; t < 4; t++)
 #pragma unroll
@@ -517,29 +530,28 @@ k_syn4(Syn4Args a) {
                 const int sh = 4 * q4 + r, th = 16 * t + l16;
                 const float y0r = Y[o][0][0][t][r], y1r = Y[o][1][0][t][r];
                 const float y0i = Y[o][0][1][t][r], y1i = Y[o][1][1][t][r];
-                St[sh * S4_SHS + th * S4_MQ + wave] = make_float2(y0r + y1r, y0i + y1i);
-                St[(S4_SH + sh) * S4_SHS + th * S4_MQ + wave] = make_float2(y0r - y1r, y0i - y1i);
+#pragma unroll
+                for (int mm = 0; mm < NMIR; mm++)
+                    St[(mm * S4_SH + sh) * SHS + th * MQ + wave] =
+                        (mir0 + mm) ? make_float2(y0r - y1r, y0i - y1i) : make_float2(y0r + y1r, y0i + y1i);
             }
         __syncthreads();
         float* out = o == 0 ? pickj(a.out0, job) : pickj(a.out1, job);
-        for (int idx = tid; idx < 2 * S4_SH * S4_TH; idx += 64 * S4_MQ) {
-            const int th = idx % S4_TH, rest = idx / S4_TH;
-            const int sh = rest % S4_SH, mir = rest / S4_SH;
+        for (int idx = tid; idx < NQ * NMIR * S4_SH * S4_TH; idx += 64 * MQ) {
+            const int qq = idx % NQ, item = idx / NQ;
+            const int th = item % S4_TH, rest = item / S4_TH;
+            const int sh = rest % S4_SH, mm = rest / S4_SH, mir = mir0 + mm;
             const int n = n0 + sh, j = jb + th;
             if (n >= a.N || j >= a.Th2) continue;
             const int jrow = mir ? a.Th - 1 - j : j;
             if (mir && jrow < a.Th2) continue;
-            const float2* sp = St + (mir * S4_SH + sh) * S4_SHS + th * S4_MQ;
-            float4 v0 = make_float4(sp[0].x, sp[0].y, sp[1].x, sp[1].y);
-            float4 v1 = make_float4(sp[2].x, sp[2].y, sp[3].x, sp[3].y);
+            const float2* sp = St + (mm * S4_SH + sh) * SHS + th * MQ + 2 * qq;
+            float4 v = make_float4(sp[0].x, sp[0].y, sp[1].x, sp[1].y);
             if constexpr (KIND == 2) {
                 const float f = (float)a.rinv[n];
-                v0.x *= f; v0.y *= f; v0.z *= f; v0.w *= f;
-                v1.x *= f; v1.y *= f; v1.z *= f; v1.w *= f;
+                v.x *= f; v.y *= f; v.z *= f; v.w *= f;
             }
-            float4* d = reinterpret_cast<float4*>(out + 2 * (((size_t)n * a.Th + jrow) * a.MP + m0));
-            d[0] = v0;
-            d[1] = v1;
+            reinterpret_cast<float4*>(out + 2 * (((size_t)n * a.Th + jrow) * a.MP + m0))[qq] = v;
         }
     }
 }
@@ -569,13 +581,15 @@ static int syn4_launch(Ctx& c, int kind, int njob, const float* tA, const float*
     }
     a.nth = c.Th2p / S4_TH;
     a.nsb = (c.N + S4_SH - 1) / S4_SH;
-    a.nmo = ((c.M + S4_MQ - 1) / S4_MQ + S4_MQ - 1) / S4_MQ;
+    a.nmo = (c.M + S4_MG - 1) / S4_MG;
     a.njob = njob;
-    dim3 grid(grid_blocks(a.nmo * njob * a.nsb, S4_MQ * a.nth));
-    dim3 blk(64 * S4_MQ);
-    if (kind == SYN_SCALAR) hipLaunchKernelGGL(k_syn4<0>, grid, blk, 0, c.stream, a);
-    else if (kind == SYN_VECTOR) hipLaunchKernelGGL(k_syn4<1>, grid, blk, 0, c.stream, a);
-    else hipLaunchKernelGGL(k_syn4<2>, grid, blk, 0, c.stream, a);
+    auto grid = [&](int mq) { return dim3(grid_blocks(a.nmo * njob * a.nsb, (S4_MG / mq) * a.nth)); };
+    if (kind == SYN_SCALAR)
+        hipLaunchKernelGGL(k_syn4<0>, grid(S4_MQ<0>), dim3(64 * S4_MQ<0>), 0, c.stream, a);
+    else if (kind == SYN_VECTOR)
+        hipLaunchKernelGGL(k_syn4<1>, grid(S4_MQ<1>), dim3(64 * S4_MQ<1>), 0, c.stream, a);
+    else
+        hipLaunchKernelGGL(k_syn4<2>, grid(S4_MQ<2>), dim3(64 * S4_MQ<2>), 0, c.stream, a);
     LSD_HIP(hipGetLastError());
     return LSD_OK;
 }
