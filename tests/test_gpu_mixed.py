@@ -14,7 +14,7 @@ FP32 rounding of the Lorentz force, explained at that test).
 import numpy as np
 import pytest
 
-from lsd import config1, config3, config_small, make_config, seeded_state, MIXED_F32, PLAIN_F64, SEEDS, AMPS
+from lsd import config1, config3, config_small, grid_for, make_config, seeded_state, MIXED_F32, PLAIN_F64, SEEDS, AMPS
 from lsd.gpu import Context
 from lsd.initial import FIELD_ID
 
@@ -55,6 +55,29 @@ def test_mixed_small_cases(which):
     assert sp.evaluations == sm.evaluations
     errs = field_errors(gm, gp, cfg)
     print(which, {k: "%.2e" % v for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v < 1e-6, (k, v)
+    gp.close()
+    gm.close()
+
+
+def test_mixed_ragged_sizes():
+    """Sizes that are not multiples of the FP32 kernels' blocks: L = M = 30
+    (M rounded up to 32 in the Fourier rows, so the last runs of four and
+    eight m of the synthesis reach the padding), N = 40 (the last block of
+    16 shells is partial, and the solves take the LDS-column kernels with
+    their FP32 refinement), Th x Ph = 46 x 96 (23 theta points of a 64-point
+    table block), with composition.  Bound: 1e-6 after 10 steps."""
+    def fn(precision):
+        th, ph = grid_for(30, 30)
+        return make_config(L=30, M=30, N=40, Th=th, Ph=ph, comp=1, precision=precision)
+
+    cfg, gp, gm = pair(fn, 5, 1e-3)
+    print("case", cfg.sizes(), flush=True)
+    sp, sm = gp.step(10), gm.step(10)
+    assert sp.evaluations == sm.evaluations
+    errs = field_errors(gm, gp, cfg)
+    print("ragged", {k: "%.2e" % v for k, v in errs.items()}, flush=True)
     for k, v in errs.items():
         assert v < 1e-6, (k, v)
     gp.close()
