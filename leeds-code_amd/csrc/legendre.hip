@@ -799,6 +799,207 @@ k_ana(AnaArgs<T> a) {
     }
 }
 
+// ----------------------------------------------------- FP32 analysis (mixed forms)
+//
+// The same contractions with 16-byte loads and no LDS.  A wave takes one
+// (slot, parity class, block of 64 harmonics) and 16 shells and contracts
+// over all theta:
+// * the A operand is the table: one 16-byte load per lane (tables stored per
+//   slot and class in 64-harmonic blocks, position 4 l + t holding class
+//   harmonic 16 t + l) carries four 16-harmonic tiles;
+// * the B operand is the folded data: one 16-byte load is (re, im) of the
+//   lane's shell at two consecutive theta, so K steps come in pairs, the
+//   first over theta 2 q4 and the second over 2 q4 + 1 of each group of 8;
+// * the i rotations are a swap of re and im within the lane.
+// Operands sit in a ring of A4_NB buffers (one pair of K steps each),
+// refilled A4_NB pairs ahead in load order; theta past Th2 meets zero table
+// rows.  The lane writes (re, im) of its shell for each of its harmonics:
+// 16 lanes write one 128-byte run of a coll row.
+constexpr int A4_SH = 16;  // shells per wave
+constexpr int A4_W = 2;    // waves (shell blocks) per workgroup
+constexpr int A4_NB = 2;   // operand buffers
+
+struct Ana4Args {
+    int N, Th2, Th2a, M, Hpa;
+    int nsbp, maxhb, njob, ngroups;
+    const Slot* slots;
+    const int* order;
+    const int* base;  // [2 nslot] first table column of (slot, class)
+    const float* tP;  // [Th2a][Hpa]
+    const float* tL;
+    const float* tS;
+    const float* g1[LG_MAXJ];  // folded [n][M][2][Th2] complex
+    const float* g2[LG_MAXJ];
+    const float* g3[LG_MAXJ];
+    float* o1[LG_MAXJ];  // coll [h][2N]
+    float* o2[LG_MAXJ];
+    float* o3[LG_MAXJ];
+};
+
+struct A4Ops {
+    float4 g[5];     // folded data: r(p), theta(1-p), phi(p), theta(p), phi(1-p); re, im at two theta
+    float4 w[2][3];  // tables P, L, S at the two theta of each K step
+};
+
+template <bool VEC>
+__global__ void __launch_bounds__(64 * A4_W) __attribute__((amdgpu_waves_per_eu(2)))
+k_ana4(Ana4Args a) {
+    constexpr int NO = VEC ? 3 : 1;
+    constexpr int NG = VEC ? 5 : 1;
+    // groups (slot, job), longest slots first; members (class, 64-harmonic
+    // block, shell-block pair) on one XCD, reading the same folded rows
+    const BlockId bid = decode_block(a.ngroups, 2 * a.maxhb * a.nsbp);
+    if (!bid.valid) return;
+    const int job = bid.group % a.njob;
+    const int sl = uniform(a.order[bid.group / a.njob]);
+    const Slot s = a.slots[sl];
+    const int par = bid.cc / (a.maxhb * a.nsbp), rest = bid.cc - par * a.maxhb * a.nsbp;
+    const int hb = rest / a.nsbp, sbp = rest - hb * a.nsbp;
+    const int n_even = (s.len + 1) / 2;
+    const int ncls = par ? s.len / 2 : n_even;
+    if (64 * hb >= ncls) return;
+    const int tid = threadIdx.x, wave = uniform(tid >> 6), lane = tid & 63, l16 = lane & 15, q4 = lane >> 4;
+    const int n0 = (sbp * A4_W + wave) * A4_SH;
+    if (n0 >= a.N) return;  // no barriers: a wave may leave alone
+    const int ntile = min(4, (ncls - 64 * hb + 15) / 16);
+    const int ncl = min(n0 + l16, a.N - 1);
+    const float* gf[3] = {pickj(a.g1, job), pickj(a.g2, job), pickj(a.g3, job)};
+    // folded rows of this lane's shell: (field, parity) -> complex row of Th2
+    const size_t grow = ((size_t)ncl * a.M + s.mfft) * 2;
+    const float* gr[NG];
+    {
+        const int fsel[5] = {0, 1, 2, 1, 2};
+        const int psel[5] = {par, 1 - par, par, par, 1 - par};
+#pragma unroll
+        for (int g = 0; g < NG; g++) gr[g] = gf[fsel[g]] + (grow + psel[g]) * a.Th2 * 2;
+    }
+    const size_t tcol = (size_t)a.base[2 * sl + par] + 64 * hb + 4 * l16;
+    auto load = [&](A4Ops& op, int tb) {
+        tb = min(tb, a.Th2a - 8);
+        const int th = min(tb + 2 * q4, a.Th2 - 2);  // theta pair of the folded loads
+#pragma unroll
+        for (int g = 0; g < NG; g++) op.g[g] = *reinterpret_cast<const float4*>(gr[g] + 2 * th);
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const size_t row = (size_t)(tb + 2 * q4 + k) * a.Hpa + tcol;
+            op.w[k][0] = *reinterpret_cast<const float4*>(a.tP + row);
+            if (VEC) {
+                op.w[k][1] = *reinterpret_cast<const float4*>(a.tL + row);
+                op.w[k][2] = *reinterpret_cast<const float4*>(a.tS + row);
+            }
+        }
+    };
+    v4f acc[4][NO][2];  // [harmonic tile][q, s, t][re, im]
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int o = 0; o < NO; o++) acc[t][o][0] = acc[t][o][1] = v4f{0, 0, 0, 0};
+    auto step = [&](const A4Ops& op) {
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            float gv[NG][2];
+#pragma unroll
+            for (int g = 0; g < NG; g++) {
+                gv[g][0] = k ? op.g[g].z : op.g[g].x;
+                gv[g][1] = k ? op.g[g].w : op.g[g].y;
+            }
+            const float wp[4] = {op.w[k][0].x, op.w[k][0].y, op.w[k][0].z, op.w[k][0].w};
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                if (t >= ntile) break;
+#pragma unroll
+                for (int u = 0; u < 2; u++) acc[t][0][u] = mfma4(wp[t], gv[0][u], acc[t][0][u]);
+                if constexpr (VEC) {
+                    const float wl = t == 0 ? op.w[k][1].x : t == 1 ? op.w[k][1].y : t == 2 ? op.w[k][1].z : op.w[k][1].w;
+                    const float ws = t == 0 ? op.w[k][2].x : t == 1 ? op.w[k][2].y : t == 2 ? op.w[k][2].z : op.w[k][2].w;
+                    // s.re += L Y2o.re + S Y3s.im ; s.im += L Y2o.im - S Y3s.re
+                    acc[t][1][0] = mfma4(wl, gv[1][0], acc[t][1][0]);
+                    acc[t][1][0] = mfma4(ws, gv[2][1], acc[t][1][0]);
+                    acc[t][1][1] = mfma4(wl, gv[1][1], acc[t][1][1]);
+                    acc[t][1][1] = mfma4(ws, -gv[2][0], acc[t][1][1]);
+                    // t.re = -Y2s.im S + Y3o.re L ; t.im = Y2s.re S + Y3o.im L
+                    acc[t][2][0] = mfma4(ws, -gv[3][1], acc[t][2][0]);
+                    acc[t][2][0] = mfma4(wl, gv[4][0], acc[t][2][0]);
+                    acc[t][2][1] = mfma4(ws, gv[3][0], acc[t][2][1]);
+                    acc[t][2][1] = mfma4(wl, gv[4][1], acc[t][2][1]);
+                }
+            }
+        }
+    };
+    A4Ops buf[A4_NB];
+#pragma unroll
+    for (int u = 0; u < A4_NB; u++) {
+        load(buf[u], 8 * u);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    for (int tb = 0; tb < a.Th2a; tb += 8 * A4_NB) {
+#pragma unroll
+        for (int u = 0; u < A4_NB; u++) {
+            if (tb + 8 * u < a.Th2a) step(buf[u]);
+            load(buf[u], tb + 8 * u + 8 * A4_NB);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    // the lane holds class harmonics 64 hb + 16 t + 4 q4 + r of shell n0 + l16
+    const int n = n0 + l16;
+    if (n >= a.N) return;
+    const int C2 = 2 * a.N;
+    float* of[3] = {pickj(a.o1, job), pickj(a.o2, job), pickj(a.o3, job)};
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        if (t >= ntile) break;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int kc = 64 * hb + 16 * t + 4 * q4 + r;
+            if (kc >= ncls) continue;
+            const size_t o = ((size_t)s.h0 + 2 * kc + par) * C2 + 2 * n;
+            if constexpr (VEC) {
+                // the reference leaves q%Im(m=0) as summed
+                *reinterpret_cast<float2*>(of[0] + o) = make_float2(acc[t][0][0][r], acc[t][0][1][r]);
+                *reinterpret_cast<float2*>(of[1] + o) = make_float2(acc[t][1][0][r], s.m == 0 ? 0.f : acc[t][1][1][r]);
+                *reinterpret_cast<float2*>(of[2] + o) = make_float2(acc[t][2][0][r], s.m == 0 ? 0.f : acc[t][2][1][r]);
+            } else {
+                *reinterpret_cast<float2*>(of[0] + o) = make_float2(acc[t][0][0][r], s.m == 0 ? 0.f : acc[t][0][1][r]);
+            }
+        }
+    }
+}
+
+static int ana4_launch(Ctx& c, int kind, int njob, const void* const* g0, const void* const* g1,
+                       const void* const* g2, void* const* o0, void* const* o1, void* const* o2) {
+    Ana4Args a;
+    a.N = c.N;
+    a.Th2 = c.Th2;
+    a.Th2a = c.Th2a;
+    a.M = c.M;
+    a.Hpa = c.Hpa;
+    a.slots = c.d_slots;
+    a.order = c.d_slot_order;
+    a.base = c.d_ana_base;
+    const size_t HT = (size_t)c.Th2a * c.Hpa;
+    a.tP = c.d_atab4;
+    a.tL = c.d_atab4 + HT;
+    a.tS = c.d_atab4 + 2 * HT;
+    for (int k = 0; k < LG_MAXJ; k++) {
+        const int q = k < njob ? k : 0;
+        a.g1[k] = reinterpret_cast<const float*>(g0[q]);
+        a.g2[k] = g1 ? reinterpret_cast<const float*>(g1[q]) : a.g1[k];
+        a.g3[k] = g2 ? reinterpret_cast<const float*>(g2[q]) : a.g1[k];
+        a.o1[k] = reinterpret_cast<float*>(o0[q]);
+        a.o2[k] = o1 ? reinterpret_cast<float*>(o1[q]) : nullptr;
+        a.o3[k] = o2 ? reinterpret_cast<float*>(o2[q]) : nullptr;
+    }
+    a.nsbp = ((c.N + A4_SH - 1) / A4_SH + A4_W - 1) / A4_W;
+    a.maxhb = c.ana_maxhb;
+    a.njob = njob;
+    a.ngroups = c.nslot * njob;
+    dim3 grid(grid_blocks(a.ngroups, 2 * a.maxhb * a.nsbp));
+    if (kind == ANA_SCALAR) hipLaunchKernelGGL(k_ana4<false>, grid, dim3(64 * A4_W), 0, c.stream, a);
+    else hipLaunchKernelGGL(k_ana4<true>, grid, dim3(64 * A4_W), 0, c.stream, a);
+    LSD_HIP(hipGetLastError());
+    return LSD_OK;
+}
+
 template <typename T>
 static int ana_launch(Ctx& c, int kind, int njob, const T* tP, const T* tL, const T* tS, const void* const* g0,
                       const void* const* g1, const void* const* g2, void* const* o0, void* const* o1,
@@ -850,6 +1051,13 @@ int launch_analysis_group(Ctx& c, int kind, int njob, const void* const* g0, con
     size_t HT = (size_t)c.H * c.Th2;
     const double unit = 4.0 * (double)c.H * c.Th2 * c.N;
     ProfScope ps(c, LSD_K_ANAL, unit * (kind == ANA_SCALAR ? 1 : 5) * njob);
+    if (f32 && c.d_atab4) {
+        if (njob < 1 || njob > LG_MAXJ) {
+            set_error("analysis: 1..6 transforms per launch");
+            return LSD_E_ARG;
+        }
+        return ana4_launch(c, kind, njob, g0, g1, g2, o0, o1, o2);
+    }
     if (f32)
         return ana_launch<float>(c, kind, njob, c.d_atab32, c.d_atab32 + HT, c.d_atab32 + 2 * HT, g0, g1, g2, o0,
                                  o1, o2);

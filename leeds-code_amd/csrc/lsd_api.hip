@@ -82,7 +82,7 @@ static void release(Ctx* c) {
                     c->d_sin, c->d_tab, c->d_atab, c->d_r, c->d_dr1, c->d_dr2, c->d_radlap, c->d_r1dr,
                     c->d_intr2dr, c->d_ll, c->d_lu, c->d_piv, c->d_pivpk, c->d_y, c->d_gre, c->d_invg, c->d_info,
                     c->d_lut32, c->d_xband, c->d_sblocks, c->d_sorder, c->d_tab32,
-                    c->d_atab32, c->d_tw32, c->d_slot_t0, c->d_row_h,
+                    c->d_atab32, c->d_tw32, c->d_slot_t0, c->d_row_h, c->d_atab4, c->d_ana_base, c->d_acol,
                     c->d_spec_in, c->d_spec_out, c->d_fourier, c->d_folded, c->d_phys, c->d_nlphys,
                     c->d_tw, c->d_partial};
     for (void* p : ptrs)
@@ -248,6 +248,35 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
         LSD_HIP(hipMemcpyAsync(c->d_row_h, row_h.data(), sizeof(int) * c->Hp, hipMemcpyHostToDevice, st));
         LSD_HIP(hipStreamSynchronize(st));
         if ((rc = tables_to_f32(*c))) return rc;
+        // wide-load analysis tables (even Th2 only: its loads are 16 bytes)
+        if (c->Th2 % 2 == 0) {
+            std::vector<int> base(2 * c->nslot), acol;
+            int maxhb = 0;
+            for (int sl = 0; sl < c->nslot; sl++) {
+                const Slot& q = c->slots[sl];
+                const int n_even = (q.len + 1) / 2;
+                for (int par = 0; par < 2; par++) {
+                    const int ncls = par ? q.len / 2 : n_even;
+                    const int nblk = (ncls + 63) / 64;
+                    maxhb = std::max(maxhb, nblk);
+                    base[2 * sl + par] = (int)acol.size();
+                    for (int w = 0; w < 64 * nblk; w++) {
+                        const int hb = w / 64, r = w % 64, kc = 64 * hb + 16 * (r & 3) + (r >> 2);
+                        acol.push_back(kc < ncls ? q.h0 + (par ? n_even : 0) + kc : -1);
+                    }
+                }
+            }
+            c->Hpa = (int)acol.size();
+            c->ana_maxhb = maxhb;
+            c->Th2a = (c->Th2 + 7) / 8 * 8;
+            if (dalloc(&c->d_atab4, (size_t)3 * c->Th2a * c->Hpa) || dalloc(&c->d_ana_base, 2 * c->nslot) ||
+                dalloc(&c->d_acol, c->Hpa))
+                return LSD_E_HIP;
+            LSD_HIP(hipMemcpyAsync(c->d_ana_base, base.data(), sizeof(int) * 2 * c->nslot, hipMemcpyHostToDevice, st));
+            LSD_HIP(hipMemcpyAsync(c->d_acol, acol.data(), sizeof(int) * c->Hpa, hipMemcpyHostToDevice, st));
+            LSD_HIP(hipStreamSynchronize(st));
+            if ((rc = tables_ana4(*c))) return rc;
+        }
     }
     if ((rc = build_operators(*c))) return rc;
     LSD_HIP(hipStreamSynchronize(st));

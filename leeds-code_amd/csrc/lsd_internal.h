@@ -117,6 +117,15 @@ struct Ctx {
     int* d_slot_t0 = nullptr;      // [nslot] first table row of each slot
     int* d_row_h = nullptr;        // [Hp] harmonic of each table row, -1 for padding
     float* d_atab32 = nullptr;     // FP32 copies of the analysis tables
+    // FP32 analysis tables of the wide-load kernel [3][Th2a][Hpa]: per (slot,
+    // parity class) a run of whole 64-harmonic blocks from column
+    // d_ana_base[2 slot + class], permuted within each block (position 4 l +
+    // t holds class harmonic 16 t + l); theta zero-padded to Th2a, a multiple
+    // of 8; zero where a block runs past the class
+    float* d_atab4 = nullptr;
+    int Th2a = 0, Hpa = 0, ana_maxhb = 0;
+    int* d_ana_base = nullptr;     // [2 nslot]
+    int* d_acol = nullptr;         // [Hpa] column of the FP64 analysis tables, -1 for padding
     // device: radial mesh
     double* d_r = nullptr;         // [7][N]  r^p, p = -3..3
     double* d_dr1 = nullptr;       // [N][7]
@@ -198,6 +207,7 @@ struct ProfScope {
 // precompute.hip
 int precompute_tables(Ctx& c);
 int tables_to_f32(Ctx& c);
+int tables_ana4(Ctx& c);
 int precompute_mesh(Ctx& c);
 int build_operators(Ctx& c);
 

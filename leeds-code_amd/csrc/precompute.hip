@@ -618,6 +618,29 @@ int tables_to_f32(Ctx& c) {
     return LSD_OK;
 }
 
+// FP32 analysis tables of the wide-load kernel: entry (kind, theta, q)
+// holds the FP64 table kind at column acol[q] and theta, zero for padding
+// columns and theta >= Th2.
+__global__ void k_tab_ana4(int H, int Th2, int Th2a, int Hpa, const int* __restrict__ acol,
+                           const double* __restrict__ atab, float* __restrict__ out) {
+    const size_t n = (size_t)3 * Th2a * Hpa;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int q = (int)(i % Hpa);
+    const size_t kt = i / Hpa;
+    const int th = (int)(kt % Th2a), kind = (int)(kt / Th2a);
+    const int col = acol[q];
+    out[i] = (col >= 0 && th < Th2) ? (float)atab[((size_t)kind * Th2 + th) * H + col] : 0.0f;
+}
+
+int tables_ana4(Ctx& c) {
+    const size_t n = (size_t)3 * c.Th2a * c.Hpa;
+    hipLaunchKernelGGL(k_tab_ana4, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, c.H, c.Th2, c.Th2a,
+                       c.Hpa, c.d_acol, c.d_atab, c.d_atab4);
+    LSD_HIP(hipGetLastError());
+    return LSD_OK;
+}
+
 int build_operators(Ctx& c) {
     OpParams op{c.dt, c.cfg.implicit, c.cfg.Pm, c.cfg.Pr, c.cfg.Sc};
     int L = c.L, N = c.N;
