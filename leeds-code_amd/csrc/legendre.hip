@@ -14,15 +14,18 @@
 // re/im column of a lane pair (one lane exchange).  The Fourier output is
 // stored [n][j][m] (m fastest), the order in which the physical-space pass
 // reads a theta row, so a workgroup covers several consecutive m and
-// writes each row's run of them at once.
+// writes each row's run of them at once.  The FP32 forms (k_syn4, below)
+// load their operands 16 and 8 bytes per lane from theta-permuted tables.
 //
 // Analysis (modules/transform.fftw3.F90:527-602, 825-949).  The forward FFT
-// leaves folded data G_p[j, c] (north+south, south-north).  A workgroup
-// takes one slot and 32 columns, stages 16-theta chunks of G through LDS,
-// and each wave contracts 16-harmonic tiles of one parity class against
-// them: D[k, c] = sum_j Tw[j, k] G[j, c] with the quadrature-weighted tables
-// stored [j][h] and each slot's even-k harmonics before its odd-k ones, so
-// table reads and result writes are 128-byte runs.
+// leaves folded data G_p[j, c] (north+south, south-north), contracted as
+// D[k, c] = sum_j Tw[j, k] G[j, c] with the quadrature-weighted tables.  In
+// k_ana a workgroup takes one slot and 32 columns, stages 16-theta chunks of
+// G through LDS, and each wave contracts 16-harmonic tiles of one parity
+// class against them, with the tables stored [j][h] and each slot's even-k
+// harmonics before its odd-k ones, so table reads and result writes are
+// 128-byte runs.  The FP32 form k_ana4 (below, for even Th2) reads both
+// operands 16 bytes per lane without LDS.
 #include <type_traits>
 
 #include "lsd_internal.h"
