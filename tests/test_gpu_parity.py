@@ -7,13 +7,15 @@ scaled by the contraction length; 100 steps at L=M=32, N=32 agree to 1e-10
 whose operators mix c1/dt ~ 5e5 with radLap entries ~1e7 at the walls: the
 CPU oracle compiled with and without fused multiply-adds differs from itself
 by up to 1.2e-11 relative L2 after one step (codensity, worst at l = 0;
-3.8e-12 for the toroidal velocity), so that spread is the tolerance there.
+3.8e-12 for the toroidal velocity), so that spread is the tolerance there;
+it holds for one step at L=M=256, N=128, 384x768 too (oracle spread 2.9e-12,
+measured 3.8e-12).
 """
 import numpy as np
 import pytest
 
 import oracle_lib as O
-from lsd import config1, config3, config_small, seeded_state
+from lsd import config1, config3, config_small, make_config, seeded_state
 from lsd.gpu import Context
 
 pytestmark = pytest.mark.gpu
@@ -148,6 +150,38 @@ def test_one_step_config3():
         a, b = g.get_coll(k), o.get_coll(k)
         assert rel_l2(a, b, not_l0) < 1e-11, k
         assert rel_l2(a, b) < 5e-11, k
+    g.close()
+
+
+def test_one_step_large_grid():
+    """One plain step at L = M = 256, N = 128, 384 x 768 (the survey's largest
+    grid, as L = M = 256 since M must be even) against the oracle.  This runs
+    the general-length fused physical pass (768 = 8.8.4.3, 1024 threads, FP64)
+    and the LDS-column solves (N = 128).  The oracle compiled with and without
+    fused multiply-adds differs from itself by at most 2.9e-12 relative L2
+    after this step (toroidal velocity), so the tolerances of the
+    L = M = 170 step hold."""
+    base = config3()
+    cfg = make_config(L=256, M=256, N=128, Th=384, Ph=768, comp=1, Ra=base.Ra, Ra_comp=base.Ra_comp,
+                      Pr=1.0, Sc=1.0, Pm=2.0, E=1e-4, dt=2e-6)
+    print("case L=M=256 384x768 N=128 plain vs oracle", flush=True)
+    st = seeded_state(cfg, 20261203, 1e-2)
+    g = Context(cfg)
+    g.set_state(st)
+    sg = g.step(1)
+    o = O.Oracle(cfg)
+    o.set_state(st)
+    so = o.step(1)
+    assert sg.evaluations == so.evaluations == 2
+    assert abs(sg.last_dterr - so.last_dterr) < 1e-10 * so.last_dterr
+    not_l0 = np.ones(o.H, dtype=bool)
+    not_l0[0] = False
+    for k in range(6):
+        a, b = g.get_coll(k), o.get_coll(k)
+        e1, e2 = rel_l2(a, b, not_l0), rel_l2(a, b)
+        print("field", k, "rel L2 %.2e (l > 0) %.2e (all)" % (e1, e2), flush=True)
+        assert e1 < 1e-11, k
+        assert e2 < 5e-11, k
     g.close()
 
 
