@@ -389,10 +389,17 @@ static int synthesise(Ctx& c, bool f, bool to_phys) {
     void* g_th[2];
     void* g_ph[2];
     int ns = 0, nv = 0, ng = 0;
+    // the vector fields (u; B) and the scalar fields (C; xi) are prepared in
+    // one launch each
+    const double2* vT[2];
+    const double2* vP[2];
+    void* vo[2][6];
+    int nvf = 0;
     auto vec = [&](int T, int P, int si, int pr) {
-        void* o[6] = {spec_in_p(c, si, f), spec_in_p(c, si + 1, f), spec_in_p(c, si + 2, f),
-                      spec_in_p(c, si + 3, f), spec_in_p(c, si + 4, f), spec_in_p(c, si + 5, f)};
-        int rc = launch_vec_prep(c, c.d_f[T], c.d_f[P], o, f);
+        void** o = vo[nvf];
+        for (int k = 0; k < 6; k++) o[k] = spec_in_p(c, si + k, f);
+        vT[nvf] = c.d_f[T];
+        vP[nvf++] = c.d_f[P];
         // q of the field and of its curl (scalar), s and t of both (vector)
         sc_in[ns] = o[0];
         sc_out[ns++] = fourier_p(c, pr, f);
@@ -406,25 +413,35 @@ static int synthesise(Ctx& c, bool f, bool to_phys) {
         v_t[nv] = o[5];
         v_th[nv] = fourier_p(c, pr + 4, f);
         v_ph[nv++] = fourier_p(c, pr + 5, f);
-        return rc;
     };
-    RC(vec(LSD_VEL_TOR, LSD_VEL_POL, SI_UQ, LSD_P_VEL_R));
-    if (c.cfg.mag) RC(vec(LSD_MAG_TOR, LSD_MAG_POL, SI_BQ, LSD_P_MAG_R));
+    vec(LSD_VEL_TOR, LSD_VEL_POL, SI_UQ, LSD_P_VEL_R);
+    if (c.cfg.mag) vec(LSD_MAG_TOR, LSD_MAG_POL, SI_BQ, LSD_P_MAG_R);
+    {
+        void* const* po[2] = {vo[0], vo[1]};
+        RC(launch_vec_prep(c, nvf, vT, vP, po, f));
+    }
     // scalars: radial derivative (scalar synthesis), the field itself
     // (angular gradient)
     const int sf[2] = {LSD_COD, LSD_COMP}, sd[2] = {SI_DC, SI_DX}, scc[2] = {SI_CC, SI_XC};
     const int pr[2] = {LSD_P_COD_GR, LSD_P_COMP_GR};
+    const double2* sC[2];
+    void* sdC[2];
+    void* sCc[2];
+    int nsf = 0;
     for (int q = 0; q < 2; q++) {
         if (!(q == 0 ? c.cfg.cod : c.cfg.comp)) continue;
         void* dC = spec_in_p(c, sd[q], f);
         void* Cc = f ? spec_in_p(c, scc[q], f) : nullptr;
-        RC(launch_scal_prep(c, c.d_f[sf[q]], dC, Cc, f));
+        sC[nsf] = c.d_f[sf[q]];
+        sdC[nsf] = dC;
+        sCc[nsf++] = Cc;
         sc_in[ns] = dC;
         sc_out[ns++] = fourier_p(c, pr[q], f);
         g_in[ng] = f ? Cc : (void*)c.d_f[sf[q]];
         g_th[ng] = fourier_p(c, pr[q] + 1, f);
         g_ph[ng++] = fourier_p(c, pr[q] + 2, f);
     }
+    if (nsf) RC(launch_scal_prep(c, nsf, sC, sdC, sCc, f));
     RC(launch_synthesis_group(c, SYN_SCALAR, ns, sc_in, nullptr, sc_out, nullptr, f));
     RC(launch_synthesis_group(c, SYN_VECTOR, nv, v_s, v_t, v_th, v_ph, f));
     if (ng) RC(launch_synthesis_group(c, SYN_GRAD, ng, g_in, nullptr, g_th, g_ph, f));

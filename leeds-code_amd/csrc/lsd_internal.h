@@ -235,8 +235,9 @@ int launch_fft_c2r(Ctx& c, const double2* fourier, double* phys, int nfield);
 int launch_fft_r2c_fold(Ctx& c, const double* phys, double2* folded, int nfield);
 
 // spectral.hip
-int launch_vec_prep(Ctx& c, const double2* T, const double2* P, void* const* out6, bool f32);
-int launch_scal_prep(Ctx& c, const double2* C, void* dC, void* Cc, bool f32);
+int launch_vec_prep(Ctx& c, int nf, const double2* const* T, const double2* const* P, void* const* const* out6,
+                    bool f32);
+int launch_scal_prep(Ctx& c, int nf, const double2* const* C, void* const* dC, void* const* Cc, bool f32);
 int launch_spec_post(Ctx& c, bool f32);
 int launch_layout_spec2coll(Ctx& c, const double* re, const double* im, double2* out);
 int launch_layout_coll2spec(Ctx& c, const double2* in, double* re, double* im);

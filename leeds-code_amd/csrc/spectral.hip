@@ -77,13 +77,29 @@ __device__ __forceinline__ double2 bandrow_l(const double* band, const double2* 
 
 // var_coll_TorPol2qst_mag (variables.F90:565-605) followed by
 // var_coll_qstcurl (variables.F90:713-754).
+// Up to two fields per launch (blockIdx.y), e.g. u and B of one evaluation.
+template <typename TO>
+struct VecPrepIO {
+    const double2* T[2];
+    const double2* P[2];
+    TO* o[2][6];  // q, s, t, curl q, curl s, curl t
+};
+
 template <typename TO>
 __global__ void __launch_bounds__(1024) k_vec_prep(int H, int N, const int* __restrict__ hl,
                                                   const double* __restrict__ r, const double* __restrict__ r1dr,
-                                                  const double* __restrict__ ll, const double2* __restrict__ T,
-                                                  const double2* __restrict__ P, TO* __restrict__ q,
-                                                  TO* __restrict__ s, TO* __restrict__ t, TO* __restrict__ oq,
-                                                  TO* __restrict__ os, TO* __restrict__ ot, int L) {
+                                                  const double* __restrict__ ll, VecPrepIO<TO> io, int L) {
+    // the field's pointers by scalar selects (indexing the argument arrays
+    // would copy them to scratch memory)
+    const bool f1 = blockIdx.y != 0;
+    const double2* __restrict__ T = f1 ? io.T[1] : io.T[0];
+    const double2* __restrict__ P = f1 ? io.P[1] : io.P[0];
+    TO* __restrict__ q = f1 ? io.o[1][0] : io.o[0][0];
+    TO* __restrict__ s = f1 ? io.o[1][1] : io.o[0][1];
+    TO* __restrict__ t = f1 ? io.o[1][2] : io.o[0][2];
+    TO* __restrict__ oq = f1 ? io.o[1][3] : io.o[0][3];
+    TO* __restrict__ os = f1 ? io.o[1][4] : io.o[0][4];
+    TO* __restrict__ ot = f1 ? io.o[1][5] : io.o[0][5];
     extern __shared__ double2 sm[];
     const int hb = spec_hb(N), nc = hb * N;
     double2* Ps = sm;            // [hb][N]
@@ -141,10 +157,21 @@ __global__ void __launch_bounds__(1024) k_vec_prep(int H, int N, const int* __re
 
 // dC/dr by the first-derivative stencil (codensity.F90:207); Cc, when
 // given, receives C itself in the transform precision.
+// Up to two fields per launch (blockIdx.y): codensity and composition.
+template <typename TO>
+struct ScalPrepIO {
+    const double2* C[2];
+    TO* dC[2];
+    TO* Cc[2];
+};
+
 template <typename TO>
 __global__ void __launch_bounds__(1024) k_scal_prep(int H, int N, const double* __restrict__ dr1,
-                                                   const double2* __restrict__ C, TO* __restrict__ dC,
-                                                   TO* __restrict__ Cc) {
+                                                   ScalPrepIO<TO> io) {
+    const bool f1 = blockIdx.y != 0;
+    const double2* __restrict__ C = f1 ? io.C[1] : io.C[0];
+    TO* __restrict__ dC = f1 ? io.dC[1] : io.dC[0];
+    TO* __restrict__ Cc = f1 ? io.Cc[1] : io.Cc[0];
     extern __shared__ double2 sm[];
     const int hb = spec_hb(N), nc = hb * N;
     double2* Cs = sm;
@@ -271,37 +298,59 @@ static size_t spec_lds(const Ctx& c, int ncols) {
 }
 
 template <typename TO>
-static int vec_prep_t(Ctx& c, const double2* T, const double2* P, TO* const* o) {
-    ProfScope ps(c, LSD_K_SPEC, (2.0 * 16.0 + 6.0 * sizeof(TO)) * c.coll_elems());
-    hipLaunchKernelGGL(k_vec_prep<TO>, spec_grid(c), spec_block(c), spec_lds(c, 3), c.stream, c.H, c.N, c.d_hl,
-                       c.d_r, c.d_r1dr, c.d_ll, T, P, o[0], o[1], o[2], o[3], o[4], o[5], c.L);
-    LSD_HIP(hipGetLastError());
-    return LSD_OK;
-}
-
-// o[0..5] = q, s, t, curl q, curl s, curl t in the layout of spec_in slots
-int launch_vec_prep(Ctx& c, const double2* T, const double2* P, void* const* o, bool f32) {
-    if (f32) {
-        float2* p[6];
-        for (int k = 0; k < 6; k++) p[k] = reinterpret_cast<float2*>(o[k]);
-        return vec_prep_t<float2>(c, T, P, p);
+static int vec_prep_t(Ctx& c, int nf, const double2* const* T, const double2* const* P, void* const* const* o) {
+    ProfScope ps(c, LSD_K_SPEC, nf * (2.0 * 16.0 + 6.0 * sizeof(TO)) * c.coll_elems());
+    VecPrepIO<TO> io;
+    for (int f = 0; f < 2; f++) {
+        const int g = f < nf ? f : 0;
+        io.T[f] = T[g];
+        io.P[f] = P[g];
+        for (int k = 0; k < 6; k++) io.o[f][k] = reinterpret_cast<TO*>(o[g][k]);
     }
-    double2* p[6];
-    for (int k = 0; k < 6; k++) p[k] = reinterpret_cast<double2*>(o[k]);
-    return vec_prep_t<double2>(c, T, P, p);
-}
-
-int launch_scal_prep(Ctx& c, const double2* C, void* dC, void* Cc, bool f32) {
-    const size_t out = f32 ? sizeof(float2) : sizeof(double2);
-    ProfScope ps(c, LSD_K_SPEC, (16.0 + out * (Cc ? 2.0 : 1.0)) * c.coll_elems());
-    if (f32)
-        hipLaunchKernelGGL(k_scal_prep<float2>, spec_grid(c), spec_block(c), spec_lds(c, 1), c.stream, c.H, c.N,
-                           c.d_dr1, C, reinterpret_cast<float2*>(dC), reinterpret_cast<float2*>(Cc));
-    else
-        hipLaunchKernelGGL(k_scal_prep<double2>, spec_grid(c), spec_block(c), spec_lds(c, 1), c.stream, c.H, c.N,
-                           c.d_dr1, C, reinterpret_cast<double2*>(dC), reinterpret_cast<double2*>(Cc));
+    dim3 grid = spec_grid(c);
+    grid.y = nf;
+    hipLaunchKernelGGL(k_vec_prep<TO>, grid, spec_block(c), spec_lds(c, 3), c.stream, c.H, c.N, c.d_hl, c.d_r,
+                       c.d_r1dr, c.d_ll, io, c.L);
     LSD_HIP(hipGetLastError());
     return LSD_OK;
+}
+
+// nf (1 or 2) fields (T[f], P[f]) -> o[f][0..5] = q, s, t, curl q, curl s,
+// curl t in the layout of spec_in slots
+int launch_vec_prep(Ctx& c, int nf, const double2* const* T, const double2* const* P, void* const* const* o,
+                    bool f32) {
+    if (nf < 1 || nf > 2) {
+        set_error("vector preparation: 1 or 2 fields per launch");
+        return LSD_E_ARG;
+    }
+    return f32 ? vec_prep_t<float2>(c, nf, T, P, o) : vec_prep_t<double2>(c, nf, T, P, o);
+}
+
+template <typename TO>
+static int scal_prep_t(Ctx& c, int nf, const double2* const* C, void* const* dC, void* const* Cc) {
+    ProfScope ps(c, LSD_K_SPEC, nf * (16.0 + sizeof(TO) * (Cc[0] ? 2.0 : 1.0)) * c.coll_elems());
+    ScalPrepIO<TO> io;
+    for (int f = 0; f < 2; f++) {
+        const int g = f < nf ? f : 0;
+        io.C[f] = C[g];
+        io.dC[f] = reinterpret_cast<TO*>(dC[g]);
+        io.Cc[f] = reinterpret_cast<TO*>(Cc[g]);
+    }
+    dim3 grid = spec_grid(c);
+    grid.y = nf;
+    hipLaunchKernelGGL(k_scal_prep<TO>, grid, spec_block(c), spec_lds(c, 1), c.stream, c.H, c.N, c.d_dr1, io);
+    LSD_HIP(hipGetLastError());
+    return LSD_OK;
+}
+
+// nf (1 or 2) scalar fields C[f] -> dC[f] = dC/dr and, when Cc[f] is not
+// null, Cc[f] = C in the transform precision
+int launch_scal_prep(Ctx& c, int nf, const double2* const* C, void* const* dC, void* const* Cc, bool f32) {
+    if (nf < 1 || nf > 2) {
+        set_error("scalar preparation: 1 or 2 fields per launch");
+        return LSD_E_ARG;
+    }
+    return f32 ? scal_prep_t<float2>(c, nf, C, dC, Cc) : scal_prep_t<double2>(c, nf, C, dC, Cc);
 }
 
 int launch_spec_post(Ctx& c, bool f32) {
