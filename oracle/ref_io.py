@@ -1,0 +1,125 @@
+"""Runs the reference's own per-timestep path built by oracle/ref_build.py
+(oracle/_ref/<case>/ref_step) and reads what it writes.  Test
+infrastructure only: the fixture generator under tests/golden/ and the CPU
+baseline of bench.py use it.
+
+File formats are those of oracle/ref_step.F90: the state file holds int32
+N, int32 H and then uT, uP, C, X, BT, BP as Re(N, 0:H-1), Im(N, 0:H-1) in
+float64 (the reference's coll layout, i.e. a C array [H][N]); the dump
+file holds one record per dumped step.
+"""
+import gzip
+import os
+import resource
+import shutil
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIELDS = ("uT", "uP", "C", "X", "BT", "BP")
+
+
+def exe(case):
+    """Path of the case's executable; where only the gzip copy is present
+    (it is what travels to other machines) it is unpacked once into the
+    system's temporary folder."""
+    p = os.path.join(HERE, "_ref", case, "ref_step")
+    if os.access(p, os.X_OK) or not os.path.exists(p + ".gz"):
+        return p
+    out = os.path.join(tempfile.gettempdir(), "lsd_ref_%s_%d" % (case, os.getuid()), "ref_step")
+    if not os.access(out, os.X_OK) or os.path.getmtime(out) < os.path.getmtime(p + ".gz"):
+        os.makedirs(os.path.dirname(out), exist_ok=True)
+        tmp = out + ".part"
+        with gzip.open(p + ".gz", "rb") as f, open(tmp, "wb") as g:
+            shutil.copyfileobj(f, g, 1 << 24)
+        os.chmod(tmp, 0o755)
+        os.replace(tmp, out)
+    return out
+
+
+def available(case):
+    p = os.path.join(HERE, "_ref", case, "ref_step")
+    return os.access(p, os.X_OK) or os.path.exists(p + ".gz")
+
+
+def write_state(path, cfg, state):
+    H = cfg.sizes()[0]
+    with open(path, "wb") as f:
+        np.array([cfg.N, H], dtype=np.int32).tofile(f)
+        for name in FIELDS:
+            re, im = state[name]
+            np.ascontiguousarray(re, dtype=np.float64).reshape(H, cfg.N).tofile(f)
+            np.ascontiguousarray(im, dtype=np.float64).reshape(H, cfg.N).tofile(f)
+
+
+def _fields(buf, off, H, N):
+    out = {}
+    n = H * N
+    for name in FIELDS:
+        re = np.frombuffer(buf, dtype=np.float64, count=n, offset=off).reshape(H, N).copy()
+        off += 8 * n
+        im = np.frombuffer(buf, dtype=np.float64, count=n, offset=off).reshape(H, N).copy()
+        off += 8 * n
+        out[name] = (re, im)
+    return out, off
+
+
+def read_dump(path, cfg):
+    """list of dicts: step, evaluations, t, dterr, fields; the first
+    (step 0) also has mesh and tables."""
+    H, Th, Ph, Th2 = cfg.sizes()
+    N = cfg.N
+    with open(path, "rb") as f:
+        buf = f.read()
+    recs, off = [], 0
+    while off < len(buf):
+        step, nev = np.frombuffer(buf, dtype=np.int32, count=2, offset=off)
+        t, err = np.frombuffer(buf, dtype=np.float64, count=2, offset=off + 8)
+        off += 24
+        fields, off = _fields(buf, off, H, N)
+        rec = {"step": int(step), "evaluations": int(nev), "t": float(t), "dterr": float(err), "fields": fields}
+        if not recs:
+            def take(count, shape):
+                nonlocal off
+                a = np.frombuffer(buf, dtype=np.float64, count=count, offset=off).reshape(shape).copy()
+                off += 8 * count
+                return a
+            mesh = {"r": take(N, (N,))}
+            for k in (1, 2, 3):
+                mesh["dr%d" % k] = take(7 * N, (N, 7))
+            mesh["radLap"] = take(7 * N, (N, 7))
+            mesh["r1dr"] = take(7 * N, (N, 7))
+            mesh["intr2dr"] = take(N, (N,))
+            tabs = {"gp": take(Th, (Th,)), "gw": take(Th, (Th,)), "P": take(Th2 * H, (H, Th2)),
+                    "gwP": take(Th2 * H, (H, Th2))}
+            rec["mesh"] = mesh
+            rec["tables"] = tabs
+        recs.append(rec)
+    return recs
+
+
+def _unlimited_stack():
+    resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+
+def run(case, cfg, state, nsteps, every=1, timing=False, threads=1):
+    """Runs ref_step; returns (records, seconds per step or None)."""
+    env = dict(os.environ)
+    env["MKL_NUM_THREADS"] = str(threads)
+    env["OMP_NUM_THREADS"] = str(threads)
+    with tempfile.TemporaryDirectory() as d:
+        sin = os.path.join(d, "state.in")
+        sout = os.path.join(d, "dump.out")
+        write_state(sin, cfg, state)
+        cmd = [exe(case), sin, sout, str(nsteps), str(every)] + (["time"] if timing else [])
+        # the reference keeps large automatic arrays on the stack
+        p = subprocess.run(cmd, cwd=d, env=env, capture_output=True, text=True, preexec_fn=_unlimited_stack)
+        if p.returncode != 0:
+            raise RuntimeError("ref_step failed (%d): %s %s" % (p.returncode, p.stdout[-2000:], p.stderr[-2000:]))
+        sec = None
+        for line in p.stdout.splitlines():
+            if line.startswith("sec/step"):
+                sec = float(line.split()[1])
+        return read_dump(sout, cfg), sec

@@ -1,0 +1,105 @@
+#!/usr/bin/env python3
+"""Writes tests/golden/ref_step_<case>.npz: per-step golden vectors of the
+reference's own FFTW/LAPACK CPU path, built from its unmodified sources by
+oracle/ref_build.py and driven through program/main.F90:100-164 by
+oracle/ref_step.F90 (run in the build container, where the reference
+sources are; the .npz files are data and travel to every machine).
+
+Each case starts from the seeded state of lsd.initial.seeded_state (the
+seed and amplitude are stored) and records, for every step, the number of
+nonlinear evaluations (1 predictor + the correctors tim_check_cgce asked
+for), tim_dterr of the accepted corrector and tim_t.  Fields are kept
+  * in full at the steps in FULL[case];
+  * as a fixed sample of (harmonic, radius) entries, the per-degree norms
+    sqrt(sum |c|^2) and the total norm at the steps in SAMPLED[case]
+    (full fields at L = M = 170, N = 96 would be 134 MB per step).
+Step 0 also keeps the radial mesh arrays (r, dr(1..3), radLap, r1dr,
+intr2dr, modules/meshs.F90:292-347) and the quadrature nodes and weights
+(leg_gp, leg_gw) with the tables leg_P and leg_gwP (full below L = 64).
+
+Usage: python tests/golden/make_ref_step_vectors.py [case ...]
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "leeds-code_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import ref_io  # noqa: E402
+from lsd.config import config1, config3, config_small  # noqa: E402
+from lsd.initial import AMPS, SEEDS, harmonics, seeded_state  # noqa: E402
+
+CASES = {
+    # name: (config, seed, amplitude, steps)
+    "small": (config_small, 7, 1e-3, 20),
+    "cfg1": (config1, SEEDS[1], AMPS[1], 100),
+    "cfg3": (config3, SEEDS[3], AMPS[3], 2),
+}
+FULL = {"small": (1, 2, 20), "cfg1": (1,), "cfg3": ()}
+SAMPLED = {"small": (), "cfg1": (10, 100), "cfg3": (1, 2)}
+NSAMPLE = 4096
+
+
+def sample_index(H, N, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    flat = rng.choice(H * N, size=min(NSAMPLE, H * N), replace=False)
+    flat.sort()
+    return (flat // N).astype(np.int32), (flat % N).astype(np.int32)
+
+
+def make(case):
+    mk, seed, amp, nsteps = CASES[case]
+    cfg = mk()
+    H, Th, Ph, Th2 = cfg.sizes()
+    state = seeded_state(cfg, seed, amp)
+    recs, _ = ref_io.run(case, cfg, state, nsteps, 1)
+    l, m = harmonics(cfg.L, cfg.M, cfg.Mp)
+    out = {"config": np.array(json.dumps(cfg.as_dict())), "seed": np.array(seed), "amp": np.array(amp),
+           "nsteps": np.array(nsteps)}
+    st = recs[1:]
+    out["step"] = np.array([r["step"] for r in st], dtype=np.int32)
+    out["evaluations"] = np.array([r["evaluations"] for r in st], dtype=np.int32)
+    out["dterr"] = np.array([r["dterr"] for r in st])
+    out["t"] = np.array([r["t"] for r in st])
+    r0 = recs[0]
+    for k, v in r0["mesh"].items():
+        out["mesh_" + k] = v
+    out["gp"] = r0["tables"]["gp"]
+    out["gw"] = r0["tables"]["gw"]
+    if cfg.L < 64:
+        out["P"] = r0["tables"]["P"]
+        out["gwP"] = r0["tables"]["gwP"]
+    hs, ns = sample_index(H, cfg.N, seed + 1)
+    out["sample_h"] = hs
+    out["sample_n"] = ns
+    by_step = {r["step"]: r for r in st}
+    for s in FULL[case]:
+        for name, (re, im) in by_step[s]["fields"].items():
+            out["s%d_%s_re" % (s, name)] = re
+            out["s%d_%s_im" % (s, name)] = im
+    for s in SAMPLED[case]:
+        for name, (re, im) in by_step[s]["fields"].items():
+            out["q%d_%s_re" % (s, name)] = re[hs, ns]
+            out["q%d_%s_im" % (s, name)] = im[hs, ns]
+            p = (re ** 2 + im ** 2).sum(axis=1)
+            out["q%d_%s_norm" % (s, name)] = np.array(np.sqrt(p.sum()))
+            out["q%d_%s_lnorm" % (s, name)] = np.sqrt(np.bincount(l, weights=p, minlength=cfg.L))
+    path = os.path.join(HERE, "ref_step_%s.npz" % case)
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path), "bytes;", "evaluations per step", out["evaluations"].tolist())
+
+
+def main(argv):
+    for c in argv or list(CASES):
+        if not ref_io.available(c):
+            raise SystemExit("oracle/_ref/%s is not built (python oracle/ref_build.py %s)" % (c, c))
+        make(c)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

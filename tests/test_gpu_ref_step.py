@@ -1,0 +1,44 @@
+"""The plain FP64 GPU step against the reference's own FFTW/LAPACK CPU path:
+golden per-step vectors of the reference's unmodified modules
+(tests/golden/ref_step_<case>.npz; bounds in ref_fixtures.py), through the
+C-ABI.  The same vectors pin the C oracle (test_oracle_ref_step.py).
+
+Every step's corrector count (tim_check_cgce) must equal the reference's;
+the fields must agree to the per-step bound times the step count.  Cases:
+L = M = 12, N = 16, 18 x 36 (LDS-column solves), config 1 (L = M = 32,
+N = 32, 48 x 96; 100 steps, seven of them with two correctors) and
+config 3 (L = M = 170, N = 96, 256 x 512).
+"""
+import numpy as np
+import pytest
+
+import ref_fixtures as rf
+from lsd.gpu import Context
+from lsd.initial import FIELD_ID
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("case,nsteps,graph", [("small", 20, True), ("small", 20, False), ("cfg1", 100, True),
+                                               ("cfg3", 2, True)])
+def test_plain_steps_equal_reference(case, nsteps, graph):
+    fx = rf.load(case)
+    cfg = rf.config(fx)
+    print("case", case, cfg.sizes(), "graph", graph, flush=True)
+    g = Context(cfg)
+    try:
+        g.set_graph(graph)
+        g.set_state(rf.initial_state(fx, cfg))
+        keep = {int(k.split("_")[0][1:]) for k in fx if k[0] in "sq" and k[1].isdigit()}
+        worst = {}
+        for k in range(1, nsteps + 1):
+            st = g.step(1)
+            assert st.evaluations == fx["evaluations"][k - 1], "step %d: corrector count differs" % k
+            assert abs(st.last_dterr - fx["dterr"][k - 1]) <= 1e-8 * fx["dterr"][k - 1], k
+            if k in keep:
+                errs = rf.compare(fx, case, k, lambda n: g.get_coll(FIELD_ID[n]), k)
+                worst[k] = max(errs.values())
+                print("step", k, {n: "%.2e" % e for n, e in errs.items()}, flush=True)
+                assert worst[k] <= rf.REL_STEP[case] * k, "step %d: %s" % (k, errs)
+    finally:
+        g.close()

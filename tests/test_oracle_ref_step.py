@@ -1,0 +1,76 @@
+"""The C oracle against the reference's own FFTW/LAPACK path: golden
+per-step vectors written by tests/golden/make_ref_step_vectors.py from the
+reference's unmodified modules (oracle/ref_build.py, oracle/ref_step.F90).
+
+This pins what the oracle's other checks leave open: the end-to-end
+operation order of a step (predictor, corrector c (N* - N), tim_addcorr),
+tim_check_cgce's corrector counts on real states (config 1 has steps with
+two correctors), the int r^2 dr weights and the FD weights of
+modules/meshs.F90:292-336.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib
+import ref_fixtures as rf
+from lsd.initial import FIELD_ID
+
+
+def _oracle(fx, threads=8):
+    cfg = rf.config(fx)
+    o = oracle_lib.Oracle(cfg, threads=threads)
+    o.set_state(rf.initial_state(fx, cfg))
+    return o
+
+
+@pytest.mark.parametrize("case", ["small", "cfg1"])
+def test_mesh_and_tables_equal_reference(case):
+    fx = rf.load(case)
+    o = _oracle(fx)
+    # Gauss nodes, weights and the Schmidt tables: the same recurrences in the
+    # same order give the same bits (auxiliary/gauwts.f, schnla.f,
+    # modules/legendre.F90:63-128)
+    assert np.array_equal(o.table(0), fx["gp"])
+    assert np.array_equal(o.table(1), fx["gw"])
+    assert np.array_equal(o.table(2), fx["P"])
+    assert np.array_equal(o.table(7), fx["gwP"])
+    assert np.array_equal(o.mesh(0), fx["mesh_r"])
+    # FD weights: the inverse of each local Taylor matrix, where the
+    # reference's dgetri is MKL's (modules/meshs.F90:292-309); the int r^2 dr
+    # weights (:311-336) follow from them
+    for k, name in ((1, "dr1"), (2, "dr2"), (3, "radLap"), (4, "r1dr")):
+        a, b = o.mesh(k), fx["mesh_" + name]
+        assert np.abs(a - b).max() <= 2e-13 * np.abs(b).max(), name
+    a, b = o.mesh(5), fx["mesh_intr2dr"]
+    assert np.abs(a - b).max() <= 2e-13 * np.abs(b).max()
+
+
+def _run_and_compare(case, nsteps):
+    fx = rf.load(case)
+    o = _oracle(fx)
+    bound = rf.REL_STEP[case]
+    keep = {int(k.split("_")[0][1:]) for k in fx if k[0] in "sq" and k[1].isdigit()}
+    for k in range(1, nsteps + 1):
+        st = o.step(1)
+        assert st.evaluations == fx["evaluations"][k - 1], "step %d: corrector count differs" % k
+        assert abs(st.last_dterr - fx["dterr"][k - 1]) <= 1e-8 * fx["dterr"][k - 1], k
+        if k in keep:
+            errs = rf.compare(fx, case, k, lambda n: o.get_coll(FIELD_ID[n]), k)
+            worst = max(errs.values())
+            assert worst <= bound * k, "step %d: %s" % (k, errs)
+
+
+def test_oracle_steps_equal_reference_small():
+    _run_and_compare("small", 20)
+
+
+def test_oracle_steps_equal_reference_config1():
+    # 100 steps of config 1: every step's corrector count (two correctors at
+    # steps 59, 66, 73, 79, 85, 95, 100) and tim_dterr match the reference
+    _run_and_compare("cfg1", 100)
+
+
+def test_oracle_steps_equal_reference_config3():
+    # L = M = 170, N = 96, 256 x 512: within the reference's own rounding
+    # spread (ref_fixtures.REL_STEP)
+    _run_and_compare("cfg3", 2)
