@@ -130,6 +130,7 @@ typedef struct lsd_step_stats {
     int    max_its;        /* max corrector iterations over the call            */
     double last_dterr;     /* tim_dterr of the last accepted corrector          */
     double t;              /* tim_t after the call                              */
+    int    status;         /* LSD_OK, or the code that stopped this run         */
 } lsd_step_stats;
 
 typedef struct lsd_ctx lsd_ctx;
@@ -164,7 +165,11 @@ int lsd_get_mesh(lsd_ctx* ctx, int which, double* out);
  *  lsd_transform   vel_transform, cod_transform, comp_transform, mag_transform
  *  lsd_nonlinear   non_velocity, non_codensity, non_composition, non_magnetic
  *  lsd_predictor   *_predictor (tim_it == 0)
- *  lsd_corrector   *_corrector; *dterr receives tim_dterr (tim_addcorr sums) */
+ *  lsd_corrector   *_corrector; *dterr receives tim_dterr (tim_addcorr sums)
+ * The predictor opens a step (tim_it = 1) and each corrector advances the
+ * context's tim_it by the tim_check_cgce rule of modules/timestep.F90:791-827
+ * (the caller's own check still decides whether the run stops), so the
+ * context knows when a step is in progress (see lsd_set_dt). */
 int lsd_transform(lsd_ctx* ctx);
 int lsd_nonlinear(lsd_ctx* ctx);
 int lsd_predictor(lsd_ctx* ctx);
@@ -179,8 +184,14 @@ int lsd_step(lsd_ctx* ctx, int nsteps, lsd_step_stats* stats);
  * truncations and parameters), with every member's step graph launched on
  * its own stream before any is waited for, so the runs execute
  * concurrently.  Each member follows the same control as lsd_step and
- * gives the same results as stepping it alone.  stats[n] may be NULL.  No
- * counterpart in the reference (one run per program there). */
+ * gives the same results as stepping it alone.  A member that
+ * tim_check_cgce stops (or whose launch fails) is left where the reference
+ * would stop it and takes no further steps; the others carry on to nsteps.
+ * Every member's stream is drained before the call returns.  stats[n] (may
+ * be NULL) receives each member's counts and status; the return code is
+ * that of the first failing member, and lsd_last_error() names every
+ * failing member.  No counterpart in the reference (one run per program
+ * there). */
 int lsd_step_many(lsd_ctx** ctxs, int n, int nsteps, lsd_step_stats* stats);
 
 /* Kinetic (which = 0) or magnetic (which = 1) energy of the current state
@@ -198,7 +209,9 @@ int lsd_energy(lsd_ctx* ctx, int which, double* E, double* El, double* Em);
 int lsd_set_graph(lsd_ctx* ctx, int on);
 
 /* Refactorise all banded operators for a new dt (vel_matrices, cod_matrices,
- * comp_matrices, mag_matrices; program/main.F90:126-135). */
+ * comp_matrices, mag_matrices; program/main.F90:126-135).  Only between
+ * steps (tim_it == 0), as the reference rebuilds them; LSD_E_ARG inside a
+ * step. */
 int lsd_set_dt(lsd_ctx* ctx, double dt);
 
 /* Drop-in transforms with host arrays in the reference spec/phys layouts

@@ -45,37 +45,65 @@ __global__ void k_energy_h(int H, int N, int L, const int* __restrict__ hl, cons
     eh[2 * H + h] = et;
 }
 
-int energy(Ctx& c, int which, double* E, double* El, double* Em) {
+// Sums of the per-harmonic parts in a fixed order: work-item 0 forms the
+// total (the q, s and t parts each summed over h, then added), work-item
+// 1 + l the degree-l part and 1 + L + m the order-m part, each over the
+// parts and then h in harmonic order; all times efac.  out = [E, El[L],
+// Em[M]].
+__global__ void k_energy_sum(int H, int L, int M, int Mp, const int* __restrict__ hl, const int* __restrict__ hm,
+                             double efac, const double* __restrict__ eh, double* __restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > L + M) return;
+    double acc = 0.0;
+    if (t == 0) {
+        for (int k = 0; k < 3; k++) {
+            double ek = 0.0;
+            for (int h = 0; h < H; h++) ek += eh[(size_t)k * H + h];
+            acc += ek;
+        }
+        acc = efac * acc;
+    } else if (t <= L) {
+        const int l = t - 1;
+        for (int k = 0; k < 3; k++)
+            for (int h = 0; h < H; h++)
+                if (hl[h] == l) acc += efac * eh[(size_t)k * H + h];
+    } else {
+        const int m = t - 1 - L;
+        for (int k = 0; k < 3; k++)
+            for (int h = 0; h < H; h++)
+                if (hm[h] / Mp == m) acc += efac * eh[(size_t)k * H + h];
+    }
+    out[t] = acc;
+}
+
+size_t energy_scratch_doubles(const Ctx& c) { return (size_t)3 * c.H + 1 + c.L + c.M; }
+
+int energy_run(Ctx& c, int which, double* scratch, double* E, double* El, double* Em) {
     const int H = c.H;
-    double* d_eh = nullptr;
-    LSD_HIP(hipMallocAsync((void**)&d_eh, sizeof(double) * 3 * H, c.stream));
+    double* d_eh = scratch;
+    double* d_out = scratch + (size_t)3 * H;
     const double2* T = c.d_f[which ? LSD_MAG_TOR : LSD_VEL_TOR];
     const double2* P = c.d_f[which ? LSD_MAG_POL : LSD_VEL_POL];
+    const double efac = which ? c.cfg.Pm / c.cfg.E : 0.5;
     hipLaunchKernelGGL(k_energy_h, dim3((H + 127) / 128), dim3(128), 0, c.stream, H, c.N, c.L, c.d_hl, c.d_hm, c.d_r,
                        c.d_r1dr, c.d_ll, c.d_intr2dr, T, P, d_eh);
     LSD_HIP(hipGetLastError());
-    std::vector<double> eh((size_t)3 * H);
-    LSD_HIP(hipMemcpyAsync(eh.data(), d_eh, sizeof(double) * 3 * H, hipMemcpyDeviceToHost, c.stream));
+    const int nt = 1 + c.L + c.M;
+    hipLaunchKernelGGL(k_energy_sum, dim3((nt + 255) / 256), dim3(256), 0, c.stream, H, c.L, c.M, c.Mp, c.d_hl,
+                       c.d_hm, efac, d_eh, d_out);
+    LSD_HIP(hipGetLastError());
+    LSD_HIP(hipMemcpyAsync(E, d_out, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    if (El) LSD_HIP(hipMemcpyAsync(El, d_out + 1, sizeof(double) * c.L, hipMemcpyDeviceToHost, c.stream));
+    if (Em) LSD_HIP(hipMemcpyAsync(Em, d_out + 1 + c.L, sizeof(double) * c.M, hipMemcpyDeviceToHost, c.stream));
     LSD_HIP(hipStreamSynchronize(c.stream));
-    LSD_HIP(hipFree(d_eh));
-    const double efac = which ? c.cfg.Pm / c.cfg.E : 0.5;
-    double tot = 0.0;
-    if (El)
-        for (int l = 0; l < c.L; l++) El[l] = 0.0;
-    if (Em)
-        for (int m = 0; m < c.M; m++) Em[m] = 0.0;
-    for (int k = 0; k < 3; k++) {
-        double ek = 0.0;
-        for (int h = 0; h < H; h++) {
-            const double e = eh[(size_t)k * H + h];
-            ek += e;
-            if (El) El[c.hl[h]] += efac * e;
-            if (Em) Em[c.hm[h] / c.Mp] += efac * e;
-        }
-        tot += ek;
-    }
-    *E = efac * tot;
     return LSD_OK;
+}
+
+int energy(Ctx& c, int which, double* E, double* El, double* Em) {
+    double* scratch = nullptr;
+    AsyncScratch s(c.stream);
+    LSD_HIP(s.alloc(&scratch, sizeof(double) * energy_scratch_doubles(c)));
+    return energy_run(c, which, scratch, E, El, Em);
 }
 
 }  // namespace lsd

@@ -655,7 +655,9 @@ static int step(Ctx& c, int nsteps, lsd_step_stats* st) {
 
 // Steps independent contexts together: each pass launches every member's
 // graph on its own stream before waiting for any, so the members run
-// concurrently; each member's tim_check_cgce then runs on the host.
+// concurrently; each member's tim_check_cgce then runs on the host.  A
+// member that fails is drained and dropped from the remaining passes; the
+// others finish all nsteps.
 static int step_many(std::vector<Ctx*>& cs, int nsteps, lsd_step_stats* st) {
     const size_t n = cs.size();
     std::vector<lsd_step_stats> z(n);
@@ -673,34 +675,61 @@ static int step_many(std::vector<Ctx*>& cs, int nsteps, lsd_step_stats* st) {
             c->g_prof_class = c->prof_class;
         }
     }
-    std::vector<int> its(n), more(n);
+    std::vector<int> its(n), more(n), status(n, LSD_OK);
     std::vector<double> last(n);
+    std::vector<std::string> why(n);
+    // member i stops with code rc: its queued work is waited for (errors of
+    // the wait are already reported through rc)
+    auto fail = [&](size_t i, int rc) {
+        status[i] = rc;
+        why[i] = g_err;
+        more[i] = 0;
+        (void)hipSetDevice(cs[i]->device);
+        (void)hipStreamSynchronize(cs[i]->stream);
+    };
     for (int k = 0; k < nsteps; k++) {
         for (size_t i = 0; i < n; i++) {
-            LSD_HIP(hipSetDevice(cs[i]->device));
-            LSD_HIP(hipGraphLaunch(cs[i]->g_first, cs[i]->stream));
+            its[i] = 0;
+            more[i] = status[i] == LSD_OK;
+            if (!more[i]) continue;
+            hipError_t e = hipSetDevice(cs[i]->device);
+            if (e == hipSuccess) e = hipGraphLaunch(cs[i]->g_first, cs[i]->stream);
+            if (e != hipSuccess) {
+                set_error(std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+                fail(i, LSD_E_HIP);
+            }
         }
         bool pending = true;
-        for (size_t i = 0; i < n; i++) {
-            its[i] = 0;
-            more[i] = 1;
-        }
         while (pending) {
             pending = false;
             for (size_t i = 0; i < n; i++) {
                 if (!more[i]) continue;
                 Ctx& c = *cs[i];
-                LSD_HIP(hipStreamSynchronize(c.stream));
+                hipError_t e = hipSetDevice(c.device);
+                if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+                if (e != hipSuccess) {
+                    set_error(std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+                    fail(i, LSD_E_HIP);
+                    continue;
+                }
                 if (its[i] == 0) {
                     z[i].evaluations += 2;
                     c.it = 1;
                 } else {
                     z[i].evaluations++;
                 }
-                RC(after_corrector(c, its[i], last[i]));
+                int rc = after_corrector(c, its[i], last[i]);
+                if (rc) {
+                    fail(i, rc);
+                    continue;
+                }
                 if (c.it != 0) {
-                    LSD_HIP(hipSetDevice(c.device));
-                    LSD_HIP(hipGraphLaunch(c.g_more, c.stream));
+                    e = hipGraphLaunch(c.g_more, c.stream);
+                    if (e != hipSuccess) {
+                        set_error(std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+                        fail(i, LSD_E_HIP);
+                        continue;
+                    }
                     pending = true;
                 } else {
                     more[i] = 0;
@@ -714,11 +743,20 @@ static int step_many(std::vector<Ctx*>& cs, int nsteps, lsd_step_stats* st) {
             }
         }
     }
+    int first = LSD_OK;
+    std::string msg;
     for (size_t i = 0; i < n; i++) {
         z[i].t = cs[i]->t;
+        z[i].status = status[i];
         if (st) st[i] = z[i];
+        if (status[i] != LSD_OK) {
+            if (first == LSD_OK) first = status[i];
+            msg += (msg.empty() ? "" : "; ") + std::string("member ") + std::to_string(i) + " after " +
+                   std::to_string(z[i].steps) + " steps: " + why[i];
+        }
     }
-    return LSD_OK;
+    if (first != LSD_OK) set_error("lsd_step_many: " + msg);
+    return first;
 }
 
 }  // namespace lsd
@@ -880,11 +918,16 @@ int lsd_nonlinear(lsd_ctx* ctx) {
     CTX_OR_FAIL(ctx);
     return lsd::nonlinear(c);
 }
+// The phase calls keep the context's tim_it in step with the caller's:
+// the predictor opens a step, and each corrector pass applies the
+// tim_check_cgce rule to its tim_dterr (the caller's own check decides
+// whether the run stops), so lsd_set_dt can refuse a step in progress.
 int lsd_predictor(lsd_ctx* ctx) {
     CTX_OR_FAIL(ctx);
     int rc = lsd::launch_predictor(c);
     if (rc) return rc;
     LSD_HIP(hipStreamSynchronize(c.stream));
+    c.it = 1;
     return LSD_OK;
 }
 int lsd_corrector(lsd_ctx* ctx, double* dterr) {
@@ -893,6 +936,10 @@ int lsd_corrector(lsd_ctx* ctx, double* dterr) {
     int rc = lsd::launch_corrector(c, &d);
     if (rc) return rc;
     if (dterr) *dterr = d;
+    if (c.it != 0) {
+        c.dterr += d;
+        (void)lsd::check_cgce(c);
+    }
     return LSD_OK;
 }
 
@@ -984,6 +1031,10 @@ int lsd_set_dt(lsd_ctx* ctx, double dt) {
     CTX_OR_FAIL(ctx);
     if (!(dt > 0.0)) {
         lsd::set_error("dt must be > 0");
+        return LSD_E_ARG;
+    }
+    if (c.it != 0) {
+        lsd::set_error("lsd_set_dt: only between timesteps (tim_it = 0)");
         return LSD_E_ARG;
     }
     c.dt = dt;

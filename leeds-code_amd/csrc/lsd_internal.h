@@ -41,6 +41,26 @@ void set_error(const std::string& msg);
         }                                                                     \
     } while (0)
 
+// Stream-ordered scratch of a set-up or diagnostic call, returned to the
+// pool on every exit path (the call synchronises the stream before using
+// what the kernels wrote, so the free follows all uses).
+struct AsyncScratch {
+    void* p = nullptr;
+    hipStream_t s;
+    explicit AsyncScratch(hipStream_t st) : s(st) {}
+    AsyncScratch(const AsyncScratch&) = delete;
+    AsyncScratch& operator=(const AsyncScratch&) = delete;
+    ~AsyncScratch() {
+        if (p) (void)hipFreeAsync(p, s);
+    }
+    template <class T>
+    hipError_t alloc(T** out, size_t bytes) {
+        hipError_t e = hipMallocAsync(&p, bytes, s);
+        *out = static_cast<T*>(p);
+        return e;
+    }
+};
+
 // One m-slot: a run of harmonics with equal m (the reference's _loop_sumth).
 struct Slot {
     int h0;    // first harmonic index
@@ -247,6 +267,8 @@ int launch_nonlinear(Ctx& c);
 
 // energy.hip
 int energy(Ctx& c, int which, double* E, double* El, double* Em);
+size_t energy_scratch_doubles(const Ctx& c);
+int energy_run(Ctx& c, int which, double* scratch, double* E, double* El, double* Em);
 
 // solve.hip
 int solve_block_harmonics(int N, bool mixed);

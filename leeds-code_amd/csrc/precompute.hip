@@ -348,8 +348,9 @@ int precompute_mesh(Ctx& c) {
     hipLaunchKernelGGL(k_grid, dim3((N + 63) / 64), dim3(64), 0, c.stream, N, c.cfg.rratio, c.d_r);
     double* ainv = nullptr;
     int* info = nullptr;
-    LSD_HIP(hipMallocAsync((void**)&ainv, sizeof(double) * 49 * N, c.stream));
-    LSD_HIP(hipMallocAsync((void**)&info, sizeof(int), c.stream));
+    AsyncScratch s_ainv(c.stream), s_info(c.stream);
+    LSD_HIP(s_ainv.alloc(&ainv, sizeof(double) * 49 * N));
+    LSD_HIP(s_info.alloc(&info, sizeof(int)));
     LSD_HIP(hipMemsetAsync(info, 0, sizeof(int), c.stream));
     LSD_HIP(hipMemsetAsync(c.d_dr1, 0, sizeof(double) * 7 * N, c.stream));
     LSD_HIP(hipMemsetAsync(c.d_dr2, 0, sizeof(double) * 7 * N, c.stream));
@@ -362,8 +363,6 @@ int precompute_mesh(Ctx& c) {
     int h_info = 0;
     LSD_HIP(hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     LSD_HIP(hipStreamSynchronize(c.stream));
-    LSD_HIP(hipFreeAsync(ainv, c.stream));
-    LSD_HIP(hipFreeAsync(info, c.stream));
     if (h_info) {
         set_error("matrix inversion error 1");
         return LSD_E_SINGULAR;
@@ -650,8 +649,9 @@ int build_operators(Ctx& c) {
     LSD_HIP(hipGetLastError());
     double* scratch = nullptr;
     int* ginfo = nullptr;
-    LSD_HIP(hipMallocAsync((void**)&scratch, sizeof(double) * 2 * N * L, c.stream));
-    LSD_HIP(hipMallocAsync((void**)&ginfo, sizeof(int) * L, c.stream));
+    AsyncScratch s_scr(c.stream), s_ginfo(c.stream);
+    LSD_HIP(s_scr.alloc(&scratch, sizeof(double) * 2 * N * L));
+    LSD_HIP(s_ginfo.alloc(&ginfo, sizeof(int) * L));
     LSD_HIP(hipMemsetAsync(ginfo, 0, sizeof(int) * L, c.stream));
     hipLaunchKernelGGL(k_green, dim3((L + 63) / 64), dim3(64), 0, c.stream, L, N, c.d_lu, c.d_piv, c.d_gre,
                        c.d_invg, ginfo, scratch);
@@ -660,8 +660,6 @@ int build_operators(Ctx& c) {
     LSD_HIP(hipMemcpyAsync(info.data(), c.d_info, sizeof(int) * NLU * L, hipMemcpyDeviceToHost, c.stream));
     LSD_HIP(hipMemcpyAsync(gi.data(), ginfo, sizeof(int) * L, hipMemcpyDeviceToHost, c.stream));
     LSD_HIP(hipStreamSynchronize(c.stream));
-    LSD_HIP(hipFreeAsync(scratch, c.stream));
-    LSD_HIP(hipFreeAsync(ginfo, c.stream));
     for (int k = 0; k < NLU; k++)
         for (int l = 0; l < L; l++)
             if (info[k * L + l]) {

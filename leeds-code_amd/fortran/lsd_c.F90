@@ -32,6 +32,7 @@
    type, bind(C) :: lsd_step_stats
       integer(c_int) :: steps, evaluations, last_its, max_its
       real(c_double) :: last_dterr, t
+      integer(c_int) :: status
    end type lsd_step_stats
 
    interface

@@ -43,7 +43,8 @@ class LsdError(RuntimeError):
 
 class StepStats(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int), ("evaluations", ctypes.c_int), ("last_its", ctypes.c_int),
-                ("max_its", ctypes.c_int), ("last_dterr", ctypes.c_double), ("t", ctypes.c_double)]
+                ("max_its", ctypes.c_int), ("last_dterr", ctypes.c_double), ("t", ctypes.c_double),
+                ("status", ctypes.c_int)]
 
 
 _lib = None
@@ -263,15 +264,25 @@ class Context:
         return (out[0], out[1]), (out[2], out[3]), (out[4], out[5])
 
 
+class StepManyError(LsdError):
+    """lsd_step_many stopped at least one member; ``stats`` holds every
+    member's StepStats (``status`` != 0 for the members that stopped)."""
+
+    def __init__(self, msg, stats):
+        super().__init__(msg)
+        self.stats = stats
+
+
 def step_many(contexts, nsteps):
-    """lsd_step_many: steps the contexts together; returns their StepStats."""
+    """lsd_step_many: steps the contexts together; returns their StepStats.
+    Raises StepManyError (carrying every member's stats) if a member stops."""
     lib = load()
     n = len(contexts)
     hs = (ctypes.c_void_p * n)(*[c.h.value for c in contexts])
     st = (StepStats * n)()
     rc = lib.lsd_step_many(hs, n, nsteps, st)
     if rc != 0:
-        raise LsdError("lsd_step_many: %s (code %d)" % (lib.lsd_last_error().decode(), rc))
+        raise StepManyError("lsd_step_many: %s (code %d)" % (lib.lsd_last_error().decode(), rc), list(st))
     return list(st)
 
 

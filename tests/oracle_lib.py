@@ -138,6 +138,9 @@ class Oracle:
         self._chk(self.lib.orc_corrector(self.h, ctypes.byref(d)), "corrector")
         return d.value
 
+    def set_dt(self, dt):
+        self._chk(self.lib.orc_set_dt(self.h, dt), "orc_set_dt")
+
     def step(self, n):
         st = StepStats()
         self._chk(self.lib.orc_step(self.h, n, ctypes.byref(st)), "step")

@@ -49,3 +49,39 @@ def test_ensemble_matches_members():
             a, b = g.get_coll(k), alone[i][0][k]
             assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), (i, k)
         g.close()
+
+
+def test_ensemble_member_that_stops():
+    """A member whose corrector cannot converge (d_dterr = 1e-30, so
+    tim_check_cgce stops it within its first step) is drained and reported;
+    the other members still finish every step and equal their solo runs bit
+    for bit."""
+    from lsd.gpu import StepManyError
+    rng = np.random.default_rng(20261205)
+    cfgs = [member(k, rng) for k in range(3)]
+    cfgs[1].dterr = 1e-30
+    states = [seeded_state(c, 200 + k, 1e-3) for k, c in enumerate(cfgs)]
+    print("ensemble", [(c.L, c.N, c.dterr) for c in cfgs], flush=True)
+    alone = {}
+    for i in (0, 2):
+        g = Context(cfgs[i])
+        g.set_state(states[i])
+        st = g.step(4)
+        alone[i] = ([g.get_coll(k) for k in range(6)], st.evaluations)
+        g.close()
+    ens = [Context(c) for c in cfgs]
+    for g, s in zip(ens, states):
+        g.set_state(s)
+    with pytest.raises(StepManyError) as ei:
+        step_many(ens, 4)
+    stats = ei.value.stats
+    print(str(ei.value), flush=True)
+    assert "member 1" in str(ei.value) and "tim_check_cgce" in str(ei.value)
+    assert stats[1].status == -5 and stats[1].steps == 0
+    for i in (0, 2):
+        assert stats[i].status == 0 and stats[i].steps == 4 and stats[i].evaluations == alone[i][1]
+        for k in range(6):
+            a, b = ens[i].get_coll(k), alone[i][0][k]
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), (i, k)
+    for g in ens:
+        g.close()

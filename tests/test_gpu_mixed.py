@@ -61,16 +61,20 @@ def test_mixed_small_cases(which):
     gm.close()
 
 
-def test_mixed_ragged_sizes():
+@pytest.mark.parametrize("th", [46, 52])
+def test_mixed_ragged_sizes(th):
     """Sizes that are not multiples of the FP32 kernels' blocks: L = M = 30
     (M rounded up to 32 in the Fourier rows, so the last runs of four and
     eight m of the synthesis reach the padding), N = 40 (the last block of
-    16 shells is partial, and the solves take the LDS-column kernels with
-    their FP32 refinement), Th x Ph = 46 x 96 (23 theta points of a 64-point
-    table block), with composition.  Bound: 1e-6 after 10 steps."""
+    16 shells is partial in the synthesis and in the wide-load analysis,
+    and the solves take the LDS-column kernels with their FP32 refinement),
+    with composition.  Th = 46: 23 theta points per hemisphere (odd, the
+    LDS-staged FP32 analysis).  Th = 52: 26 per hemisphere, even but not a
+    multiple of 8, so the wide-load FP32 analysis reads its theta pairs up
+    to 32 with the folded loads clamped and zero table rows past 26.
+    Bound: 1e-6 after 10 steps."""
     def fn(precision):
-        th, ph = grid_for(30, 30)
-        return make_config(L=30, M=30, N=40, Th=th, Ph=ph, comp=1, precision=precision)
+        return make_config(L=30, M=30, N=40, Th=th, Ph=96, comp=1, precision=precision)
 
     cfg, gp, gm = pair(fn, 5, 1e-3)
     print("case", cfg.sizes(), flush=True)
