@@ -74,7 +74,7 @@ def cpu_baseline(cfg, state, threads, ref_steps):
         if ref_io.available("cfg3"):
             _, sec = ref_io.run("cfg3", cfg, state, ref_steps + 1, every=ref_steps + 1, timing=True, threads=1)
             out = {"value": 1.0 / sec, "unit": "timesteps/s", "cores": 1, "kind": "reference",
-                   "sample": "steps 2-%d of the same L=M=170, N=96 workload from the same state: the reference's "
+                   "sample": "timed steps 2..%d of the same L=M=170, N=96 workload from the same state: the reference's "
                              "unmodified Fortran modules (serial build) with MKL FFTW3/LAPACK on 1 thread"
                              % (ref_steps + 1)}
         else:

@@ -59,11 +59,12 @@ typedef struct lsd_config {
     double Sc;      /* d_Sc                                                     */
     double Pm;      /* d_Pm                                                     */
     double E;       /* d_E                                                      */
-    double dt;      /* d_timestep (> 0: fixed step)                             */
+    double dt;      /* d_timestep (> 0: fixed step; < 0: adaptive, see lsd_get_dt) */
     double implicit;/* d_implicit                                               */
     double dterr;   /* d_dterr                                                  */
     double source;  /* d_source                                                 */
     double source_comp; /* d_source_comp                                        */
+    int step_control;   /* i_time_step_control: 2 (or 0, the default) or 1     */
 } lsd_config;
 
 /* Coll-field identifiers. */
@@ -154,6 +155,26 @@ void lsd_destroy(lsd_ctx* ctx);
  * fill of io_load_state / io_load_coll (program/io.F90:349-483,1278-1355). */
 int lsd_set_coll(lsd_ctx* ctx, int field, const double* re, const double* im);
 int lsd_get_coll(lsd_ctx* ctx, int field, double* re, double* im);
+/* State files: io_save_state / io_load_state (program/io.F90:1502-1621,
+ * 349-483) in the netCDF classic format (CDF-1 written; CDF-1 and CDF-2
+ * read), variables uT, uP, C, Comp, BT, BP (and zero icBT, icBP) as
+ * (ReIm, H, r) in the m-major io_state_nh order with attributes L, M, Mp,
+ * the radial grid r, dt, and the global attribute t.  Reading maps each
+ * (l, m) of the run to the file's index (modes the file lacks stay zero;
+ * the reference's optional d_init_pert noise is not added), interpolates
+ * radially when the file's grid differs (7-point, io_interp_wts), loads C
+ * into the composition when the file has no Comp, and leaves B zero for a
+ * run without a magnetic field.
+ *   coll: [6][2][H][N], fields uT, uP, C, Comp, BT, BP, Re then Im, each
+ *         in the coll layout.  No context or GPU needed. */
+int lsd_read_state(const char* path, const lsd_config* cfg, double* coll, double* t, double* dt);
+int lsd_write_state(const char* path, const lsd_config* cfg, const double* coll, double t, double dt);
+/* The same on a context's state: load sets the six fields and tim_t, and
+ * with cfg->dt < 0 takes tim_dt from the file (refactorising the
+ * operators); save writes the current fields, tim_t and tim_dt. */
+int lsd_load_state(lsd_ctx* ctx, const char* path);
+int lsd_save_state(lsd_ctx* ctx, const char* path);
+
 /* Physical field of the last lsd_transform (reference phys layout). */
 int lsd_get_phys(lsd_ctx* ctx, int field, double* p);
 
@@ -213,6 +234,15 @@ int lsd_set_graph(lsd_ctx* ctx, int on);
  * steps (tim_it == 0), as the reference rebuilds them; LSD_E_ARG inside a
  * step. */
 int lsd_set_dt(lsd_ctx* ctx, double dt);
+
+/* tim_dt, tim_cfl_dt and tim_corr_dt (modules/timestep.F90:13-18): the
+ * current step, the last CFL limit of non_maxtstep and the corrector's
+ * estimate of tim_check_cgce.  With cfg->dt < 0 (d_timestep < 0) lsd_step
+ * chooses dt at the start of every step as program/main.F90:124-135 does
+ * (non_maxtstep, tim_new_tstep, *_matrices on a change), starting from
+ * |cfg->dt|, which stands for the dt the reference reads from the state
+ * file (program/io.F90:429). */
+int lsd_get_dt(lsd_ctx* ctx, double* dt, double* cfl_dt, double* corr_dt);
 
 /* Drop-in transforms with host arrays in the reference spec/phys layouts
  * (modules/transform.fftw3.F90:472,527,719,825,955). */

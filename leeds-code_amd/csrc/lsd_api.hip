@@ -32,7 +32,11 @@ static int check_config(const lsd_config* c, int* H, int* Th, int* Ph) {
         set_error("only i_vel_bc = i_cod_bc = i_comp_bc = 1 are supported");
         return LSD_E_CONFIG;
     }
-    if (c->dt <= 0.0) { set_error("dt must be > 0 (fixed step)"); return LSD_E_CONFIG; }
+    if (c->dt == 0.0) { set_error("dt must be nonzero (> 0 fixed, < 0 adaptive)"); return LSD_E_CONFIG; }
+    if (c->step_control < 0 || c->step_control > 2) {
+        set_error("i_time_step_control must be 1 or 2 (0 selects 2)");
+        return LSD_E_CONFIG;
+    }
     if (c->precision != LSD_PLAIN_F64 && c->precision != LSD_MIXED_F32) {
         set_error("precision must be LSD_PLAIN_F64 or LSD_MIXED_F32");
         return LSD_E_CONFIG;
@@ -48,7 +52,7 @@ static int check_config(const lsd_config* c, int* H, int* Th, int* Ph) {
 }
 
 // Folded harmonic order (Scripts/parallel.h_template:65-80).
-static void harmonics(int L, int M, int Mp, std::vector<int>& hl, std::vector<int>& hm) {
+void harmonics(int L, int M, int Mp, std::vector<int>& hl, std::vector<int>& hm) {
     int H = L * M - Mp * (M - 1) * M / 2;
     int Hc = (M + 1) / 2, Hr = H / Hc;
     hl.resize(H);
@@ -84,7 +88,7 @@ static void release(Ctx* c) {
                     c->d_lut32, c->d_xband, c->d_sblocks, c->d_sorder, c->d_tab32,
                     c->d_atab32, c->d_tw32, c->d_slot_t0, c->d_row_h, c->d_atab4, c->d_ana_base, c->d_acol,
                     c->d_spec_in, c->d_spec_out, c->d_fourier, c->d_folded, c->d_phys, c->d_nlphys,
-                    c->d_tw, c->d_partial};
+                    c->d_tw, c->d_partial, c->d_cfl_d, c->d_cfl_part, c->d_cfl};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (int k = 0; k < LSD_NUM_COLL; k++)
@@ -95,6 +99,7 @@ static void release(Ctx* c) {
     }
     if (c->d_rhs2) (void)hipFree(c->d_rhs2);
     if (c->h_partial) (void)hipHostFree(c->h_partial);
+    if (c->h_cfl) (void)hipHostFree(c->h_cfl);
     for (hipEvent_t e : c->prof_events) (void)hipEventDestroy(e);
     drop_graphs(*c);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -117,7 +122,10 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
     c->H = H;
     c->Th2 = (Th + 1) / 2;
     c->MP = (c->M + 15) / 16 * 16;
-    c->dt = cfg->dt;
+    // d_timestep < 0: adaptive step, starting from |d_timestep| (the dt
+    // io_load_state would read from the state file, program/io.F90:429)
+    c->adaptive = cfg->dt < 0.0;
+    c->dt = std::fabs(cfg->dt);
     c->mixed = cfg->precision != LSD_PLAIN_F64;
     *out = c;
     LSD_HIP(hipSetDevice(device));
@@ -230,6 +238,11 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
 
     if ((rc = fft_setup(*c))) return rc;
     if ((rc = precompute_mesh(*c))) return rc;
+    if (dalloc(&c->d_cfl_d, (size_t)2 * N) || dalloc(&c->d_cfl_part, (size_t)2 * N * std::max(c->Th2, 1)) ||
+        dalloc(&c->d_cfl, 1))
+        return LSD_E_HIP;
+    LSD_HIP(hipHostMalloc((void**)&c->h_cfl, sizeof(double), hipHostMallocDefault));
+    if ((rc = cfl_setup(*c))) return rc;
     if ((rc = precompute_tables(*c))) return rc;
     if (c->mixed) {
         const size_t HT = (size_t)H * c->Th2;
@@ -337,7 +350,8 @@ static int graph_prof_accumulate(Ctx& c, int which) {
 static void drop_graphs(Ctx& c) {
     if (c.g_first) (void)hipGraphExecDestroy(c.g_first);
     if (c.g_more) (void)hipGraphExecDestroy(c.g_more);
-    c.g_first = c.g_more = nullptr;
+    if (c.g_eval) (void)hipGraphExecDestroy(c.g_eval);
+    c.g_first = c.g_more = c.g_eval = nullptr;
     for (int w = 0; w < 2; w++) {
         for (hipEvent_t e : c.g_ev[w]) (void)hipEventDestroy(e);
         c.g_ev[w].clear();
@@ -514,6 +528,7 @@ static int nonlinear(Ctx& c) {
 static int evaluate(Ctx& c) {
     if (!physfuse_supported(c)) {
         RC(transform(c));
+        if (c.cfl_on) RC(launch_cfl_phys(c));
         return nonlinear(c);
     }
     RC(synthesise(c, c.mixed, false));
@@ -521,12 +536,16 @@ static int evaluate(Ctx& c) {
     return analyse(c, c.mixed);
 }
 
-// tim_check_cgce, i_time_step_control = 2 (modules/timestep.F90:791-827).
-// Returns nonzero if the reference would stop the run.
+// tim_check_cgce (modules/timestep.F90:744-830).  Control 2 iterates while
+// tim_dterr > d_dterr and estimates tim_corr_dt = dt sqrt(d_dterr /
+// tim_dterr); control 1 iterates while tim_dterr > 1 and takes tim_corr_dt =
+// tim_dterr.  Returns nonzero if the reference would stop the run.
 static int check_cgce(Ctx& c) {
     int stop = 0;
+    const bool ctl1 = c.cfg.step_control == 1;
+    const double crit = ctl1 ? 1.0 : c.cfg.dterr;
     if (c.it == 1) {
-        c.corr_dt = c.dt * std::sqrt(c.cfg.dterr / c.dterr);
+        c.corr_dt = ctl1 ? c.dterr : c.dt * std::sqrt(c.cfg.dterr / c.dterr);
         c.lasterr = 1e99;
     }
     if (c.dt < 1e-12 && c.step > 30) {
@@ -536,9 +555,14 @@ static int check_cgce(Ctx& c) {
         stop = 1;
         c.it = 0;
     } else if (c.dterr > c.lasterr) {
-        if (c.dterr > 2.0 * c.cfg.dterr) stop = 1;
+        if (ctl1) {
+            if (c.dterr > 1.0) stop = 1;
+        } else {
+            if (c.dterr > 2.0 * c.cfg.dterr) stop = 1;
+            if (c.dterr < 2.0 * c.cfg.dterr) c.corr_dt = c.dt / (1e1 * COURANT);
+        }
         c.it = 0;
-    } else if (c.dterr > c.cfg.dterr) {
+    } else if (c.dterr > crit) {
         c.lasterr = c.dterr;
         c.it = c.it + 1;
     } else {
@@ -560,14 +584,29 @@ static int after_corrector(Ctx& c, int& its, double& last) {
     return LSD_OK;
 }
 
-static int capture(Ctx& c, bool first, hipGraphExec_t* out) {
+// Graph kinds: CAP_FIRST [evaluate, predictor, evaluate, corrector], CAP_MORE
+// [evaluate, corrector] (fixed dt); with the adaptive step the first pass
+// splits at the step choice: CAP_EVAL [evaluate with the CFL partials,
+// tim_cfl_dt] and CAP_REST [predictor, evaluate, corrector].  Each corrector
+// ends with the copy of its norm partials.
+enum CaptureKind { CAP_FIRST = 0, CAP_MORE = 1, CAP_EVAL = 2, CAP_REST = 3 };
+
+static int capture(Ctx& c, int kind, hipGraphExec_t* out) {
     hipGraph_t g = nullptr;
     LSD_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-    c.cap = first ? 0 : 1;
-    int rc = evaluate(c);
-    if (!rc && first) rc = launch_predictor(c);
-    if (!rc && first) rc = evaluate(c);
-    if (!rc) rc = launch_corrector_async(c);
+    c.cap = kind == CAP_MORE ? 1 : 0;
+    int rc = LSD_OK;
+    if (kind == CAP_EVAL) {
+        c.cfl_on = true;
+        rc = evaluate(c);
+        c.cfl_on = false;
+        if (!rc) rc = launch_cfl_final(c);
+    } else {
+        if (kind != CAP_REST) rc = evaluate(c);
+        if (!rc && kind != CAP_MORE) rc = launch_predictor(c);
+        if (!rc && kind != CAP_MORE) rc = evaluate(c);
+        if (!rc) rc = launch_corrector_async(c);
+    }
     c.cap = -1;
     hipError_t e = hipStreamEndCapture(c.stream, &g);
     if (rc) {
@@ -581,19 +620,40 @@ static int capture(Ctx& c, bool first, hipGraphExec_t* out) {
     return LSD_OK;
 }
 
-// main.F90:100-164 replayed from two captured graphs: the first covers the
-// predictor pass and the first corrector pass of a step, the second any
-// further corrector pass tim_check_cgce asks for.
-static int step_graph(Ctx& c, int nsteps, lsd_step_stats* z) {
+static int ensure_graphs(Ctx& c) {
     if (c.g_first && c.g_prof_class != c.prof_class) drop_graphs(c);
     if (!c.g_first) {
-        RC(capture(c, true, &c.g_first));
-        RC(capture(c, false, &c.g_more));
+        RC(capture(c, c.adaptive ? CAP_REST : CAP_FIRST, &c.g_first));
+        RC(capture(c, CAP_MORE, &c.g_more));
+        if (c.adaptive) RC(capture(c, CAP_EVAL, &c.g_eval));
         c.g_prof_class = c.prof_class;
     }
+    return LSD_OK;
+}
+
+// Adaptive step, after the predictor pass's evaluation has queued
+// tim_cfl_dt: non_maxtstep's result, tim_new_tstep, and the *_matrices
+// rebuild on a new dt (program/main.F90:124-135).
+static int choose_dt(Ctx& c) {
+    LSD_HIP(hipStreamSynchronize(c.stream));
+    c.cfl_dt = *c.h_cfl;
+    if (new_tstep(c)) RC(build_operators(c));
+    return LSD_OK;
+}
+
+// main.F90:100-164 replayed from the captured graphs: g_first covers the
+// predictor pass and the first corrector pass of a step (with the adaptive
+// step, g_eval first takes the predictor pass's evaluation up to the step
+// choice), g_more any further corrector pass tim_check_cgce asks for.
+static int step_graph(Ctx& c, int nsteps, lsd_step_stats* z) {
+    RC(ensure_graphs(c));
     for (int k = 0; k < nsteps; k++) {
         int its = 0;
         double last = 0.0;
+        if (c.adaptive) {
+            LSD_HIP(hipGraphLaunch(c.g_eval, c.stream));
+            RC(choose_dt(c));
+        }
         LSD_HIP(hipGraphLaunch(c.g_first, c.stream));
         LSD_HIP(hipStreamSynchronize(c.stream));
         RC(graph_prof_accumulate(c, 0));
@@ -629,9 +689,16 @@ static int step(Ctx& c, int nsteps, lsd_step_stats* st) {
             int its = 0;
             double last = 0.0;
             do {
-                RC(evaluate(c));
+                c.cfl_on = c.adaptive && c.it == 0;
+                int rc = evaluate(c);
+                c.cfl_on = false;
+                if (rc) return rc;
                 z.evaluations++;
                 if (c.it == 0) {
+                    if (c.adaptive) {
+                        RC(launch_cfl_final(c));
+                        RC(choose_dt(c));
+                    }
                     RC(launch_predictor(c));
                     c.it = 1;
                 } else {
@@ -667,13 +734,12 @@ static int step_many(std::vector<Ctx*>& cs, int nsteps, lsd_step_stats* st) {
             set_error("lsd_step_many: every member must be between steps, with graphs on and no profiling");
             return LSD_E_ARG;
         }
-        if (c->g_first && c->g_prof_class != c->prof_class) drop_graphs(*c);
-        if (!c->g_first) {
-            LSD_HIP(hipSetDevice(c->device));
-            RC(capture(*c, true, &c->g_first));
-            RC(capture(*c, false, &c->g_more));
-            c->g_prof_class = c->prof_class;
+        if (c->adaptive) {
+            set_error("lsd_step_many: members must have a fixed dt");
+            return LSD_E_ARG;
         }
+        LSD_HIP(hipSetDevice(c->device));
+        RC(ensure_graphs(*c));
     }
     std::vector<int> its(n), more(n), status(n, LSD_OK);
     std::vector<double> last(n);
@@ -826,6 +892,63 @@ int lsd_set_coll(lsd_ctx* ctx, int field, const double* re, const double* im) {
     LSD_HIP(hipMemcpyAsync(c.d_f[field], tmp.data(), sizeof(double2) * HN, hipMemcpyHostToDevice, c.stream));
     LSD_HIP(hipStreamSynchronize(c.stream));
     return LSD_OK;
+}
+
+int lsd_read_state(const char* path, const lsd_config* cfg, double* coll, double* t, double* dt) {
+    if (!path || !coll) {
+        lsd::set_error("lsd_read_state: null path or output");
+        return LSD_E_ARG;
+    }
+    int rc = lsd::check_config(cfg, nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    return lsd::state_read(path, *cfg, coll, t, dt);
+}
+
+int lsd_write_state(const char* path, const lsd_config* cfg, const double* coll, double t, double dt) {
+    if (!path || !coll) {
+        lsd::set_error("lsd_write_state: null path or input");
+        return LSD_E_ARG;
+    }
+    int rc = lsd::check_config(cfg, nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    return lsd::state_write(path, *cfg, coll, t, dt);
+}
+
+int lsd_load_state(lsd_ctx* ctx, const char* path) {
+    CTX_OR_FAIL(ctx);
+    if (c.it != 0) {
+        lsd::set_error("lsd_load_state: only between timesteps (tim_it = 0)");
+        return LSD_E_ARG;
+    }
+    const size_t HN = c.coll_elems();
+    std::vector<double> buf(12 * HN);
+    double t = 0.0, dt = 0.0;
+    int rc = lsd::state_read(path, c.cfg, buf.data(), &t, &dt);
+    if (rc) return rc;
+    for (int k = 0; k < 6; k++) {
+        rc = lsd_set_coll(ctx, k, buf.data() + (size_t)(2 * k) * HN, buf.data() + (size_t)(2 * k + 1) * HN);
+        if (rc) return rc;
+    }
+    // program/io.F90:394-430: tim_t from the file; with d_timestep < 0 also
+    // tim_dt, and the operators follow it
+    c.t = t;
+    if (c.adaptive && dt > 0.0 && dt != c.dt) {
+        c.dt = dt;
+        rc = lsd::build_operators(c);
+        if (rc) return rc;
+    }
+    return LSD_OK;
+}
+
+int lsd_save_state(lsd_ctx* ctx, const char* path) {
+    CTX_OR_FAIL(ctx);
+    const size_t HN = c.coll_elems();
+    std::vector<double> buf(12 * HN);
+    for (int k = 0; k < 6; k++) {
+        int rc = lsd_get_coll(ctx, k, buf.data() + (size_t)(2 * k) * HN, buf.data() + (size_t)(2 * k + 1) * HN);
+        if (rc) return rc;
+    }
+    return lsd::state_write(path, c.cfg, buf.data(), c.t, c.dt);
 }
 
 int lsd_get_coll(lsd_ctx* ctx, int field, double* re, double* im) {
@@ -1027,6 +1150,14 @@ int lsd_profile_end(lsd_ctx* ctx, double* total_ms, int* launches, double* work)
     return LSD_OK;
 }
 
+int lsd_get_dt(lsd_ctx* ctx, double* dt, double* cfl_dt, double* corr_dt) {
+    CTX_OR_FAIL(ctx);
+    if (dt) *dt = c.dt;
+    if (cfl_dt) *cfl_dt = c.cfl_dt;
+    if (corr_dt) *corr_dt = c.corr_dt;
+    return LSD_OK;
+}
+
 int lsd_set_dt(lsd_ctx* ctx, double dt) {
     CTX_OR_FAIL(ctx);
     if (!(dt > 0.0)) {
@@ -1038,7 +1169,7 @@ int lsd_set_dt(lsd_ctx* ctx, double dt) {
         return LSD_E_ARG;
     }
     c.dt = dt;
-    c.cfg.dt = dt;
+    if (!c.adaptive) c.cfg.dt = dt;
     return lsd::build_operators(c);
 }
 

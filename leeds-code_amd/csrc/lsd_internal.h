@@ -32,6 +32,10 @@ namespace lsd {
 
 void set_error(const std::string& msg);
 
+// Folded harmonic order (Scripts/parallel.h_template:65-80): l and m_ of
+// each harmonic index.
+void harmonics(int L, int M, int Mp, std::vector<int>& hl, std::vector<int>& hm);
+
 #define LSD_HIP(call)                                                         \
     do {                                                                      \
         hipError_t e_ = (call);                                               \
@@ -189,6 +193,7 @@ struct Ctx {
     // [evaluate, corrector], each ending with the copy of the norm partials
     bool use_graph = true;
     hipGraphExec_t g_first = nullptr, g_more = nullptr;
+    hipGraphExec_t g_eval = nullptr;       // adaptive step: predictor pass's evaluation + CFL
     int g_prof_class = -1;                 // class timed by events inside the graphs
     int cap = -1;                          // graph being captured (0 first, 1 more)
     std::vector<hipEvent_t> g_ev[2];       // (start, stop) pairs recorded by each graph
@@ -207,6 +212,17 @@ struct Ctx {
     int it = 0;
     int step = 0;
     double dterr = 0.0, lasterr = 1e99, corr_dt = 0.0;
+
+    // adaptive step (d_timestep < 0): CFL partial maxima of the predictor
+    // pass's evaluation, folded into tim_cfl_dt on the device
+    bool adaptive = false;
+    bool cfl_on = false;               // the next evaluation forms the CFL partials
+    int cfl_parts = 1;                 // partial pairs per shell in d_cfl_part
+    double cfl_dt = 0.0;               // tim_cfl_dt
+    double* d_cfl_d = nullptr;         // [N][2] per-shell constants
+    double* d_cfl_part = nullptr;      // [N][max(Th2, 1)][2]
+    double* d_cfl = nullptr;           // [1] tim_cfl_dt
+    double* h_cfl = nullptr;           // pinned copy
 
     size_t coll_elems() const { return (size_t)H * N; }
     size_t phys_elems() const { return (size_t)Th * Ph * N; }
@@ -269,6 +285,17 @@ int launch_nonlinear(Ctx& c);
 int energy(Ctx& c, int which, double* E, double* El, double* Em);
 size_t energy_scratch_doubles(const Ctx& c);
 int energy_run(Ctx& c, int which, double* scratch, double* E, double* El, double* Em);
+
+// state_io.hip: coll = [6][2][H][N] (uT, uP, C, Comp, BT, BP; Re, Im)
+int state_read(const char* path, const lsd_config& cfg, double* coll, double* t, double* dt);
+int state_write(const char* path, const lsd_config& cfg, const double* coll, double t, double dt);
+
+// cfl.hip
+extern const double COURANT;
+int cfl_setup(Ctx& c);
+int launch_cfl_phys(Ctx& c);
+int launch_cfl_final(Ctx& c);
+bool new_tstep(Ctx& c);
 
 // solve.hip
 int solve_block_harmonics(int N, bool mixed);

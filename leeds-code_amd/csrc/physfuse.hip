@@ -54,6 +54,12 @@ struct PhysArgs {
     const typename Cx<R>::t* fourier;  // [18][N][Th][MP]
     typename Cx<R>::t* folded;         // [8][N][M][2][Th2]
     const typename Cx<R>::t* tw;
+    // CFL partial maxima of non_maxtstep (null: not formed): per workgroup
+    // cfl[(shell Th2 + row pair) 2 + k], with the per-shell constants
+    // cfl_d[2 shell + k] and ep = E/Pm, eE = E, ePm = Pm
+    double* cfl;
+    const double* cfl_d;
+    R ep, eE, ePm;
 };
 
 // i / d for 0 <= i < 2^21, d > 0, given inv = 1/d rounded to FP32: the true
@@ -233,7 +239,9 @@ size_t physfuse_lds(int Ph, bool r512) {
     return sizeof(V) * ((r512 ? TW512 : Ph) + (size_t)PF_MAXSEQ * Ph / 8 * 9);
 }
 
-template <typename R, bool R512, int NT>
+// CFL: also form the CFL partial maxima (a separate instantiation, so the
+// usual pass carries no registers for them).
+template <typename R, bool R512, int NT, bool CFL>
 __global__ void __launch_bounds__(NT) k_physfuse(PhysArgs<R> a) {
     static_assert(!R512 || NT == PF_THREADS, "length-512 stages take one butterfly per thread of 512");
     constexpr int PP = 1;  // phi points per thread
@@ -276,6 +284,8 @@ __global__ void __launch_bounds__(NT) k_physfuse(PhysArgs<R> a) {
     const R gpr[2] = {(R)a.gp[jp], (R)a.gp[jm]};
 
     R u[PP][2][3], nu[PP][2][3], e[PP][2][3], nc[PP][2], nx[PP][2];
+    // CFL maxima over this thread's points (radial, horizontal)
+    R cm1 = (R)-1e30, cm2 = (R)-1e30;
 
     // u and curl u: advection and Coriolis
     {
@@ -310,6 +320,11 @@ __global__ void __launch_bounds__(NT) k_physfuse(PhysArgs<R> a) {
                 u[p][r][0] = ur;
                 u[p][r][1] = ut;
                 u[p][r][2] = up;
+                if (CFL && !a.mag) {
+                    // no magnetic field: the Alfven terms vanish
+                    cm1 = fmax(cm1, fabs(ur));
+                    cm2 = fmax(cm2, sqrt(ut * ut + up * up));
+                }
                 nu[p][r][0] = vr;
                 nu[p][r][1] = vt;
                 nu[p][r][2] = vp;
@@ -343,9 +358,36 @@ __global__ void __launch_bounds__(NT) k_physfuse(PhysArgs<R> a) {
                 e[p][r][0] = ut * bp - up * bt;
                 e[p][r][1] = up * br - ur * bp;
                 e[p][r][2] = ur * bt - ut * br;
+                if (CFL) {
+                    const R d1 = (R)a.cfl_d[2 * sh], d2 = (R)a.cfl_d[2 * sh + 1];
+                    const R pr = br * br, ph = bt * bt + bp * bp;
+                    cm1 = fmax(cm1, pr / sqrt(pr * a.ep + d1) + fabs(ur));
+                    cm2 = fmax(cm2, ph / sqrt(ph * a.eE / a.ePm + d2) + sqrt(ut * ut + up * up));
+                }
             }
         }
         __syncthreads();
+    }
+    if constexpr (CFL) {
+        // workgroup maxima: across the wave, then across the waves
+        __shared__ R cfl_red[2][NT / 64];
+        for (int o = 32; o > 0; o >>= 1) {
+            cm1 = fmax(cm1, __shfl_xor(cm1, o));
+            cm2 = fmax(cm2, __shfl_xor(cm2, o));
+        }
+        if ((t & 63) == 0) {
+            cfl_red[0][t >> 6] = cm1;
+            cfl_red[1][t >> 6] = cm2;
+        }
+        __syncthreads();
+        if (t == 0) {
+            for (int w = 1; w < NT / 64; w++) {
+                cm1 = fmax(cm1, cfl_red[0][w]);
+                cm2 = fmax(cm2, cfl_red[1][w]);
+            }
+            a.cfl[((size_t)sh * a.Th2 + jp) * 2] = (double)cm1;
+            a.cfl[((size_t)sh * a.Th2 + jp) * 2 + 1] = (double)cm2;
+        }
     }
     // grad C and grad xi: advection of codensity and composition
     if (ng) {
@@ -448,6 +490,12 @@ static int physfuse_launch(Ctx& c, const void* fourier, void* folded, const void
     a.fourier = reinterpret_cast<const V*>(fourier);
     a.folded = reinterpret_cast<V*>(folded);
     a.tw = reinterpret_cast<const V*>(tw);
+    a.cfl = c.cfl_on ? c.d_cfl_part : nullptr;
+    a.cfl_d = c.d_cfl_d;
+    a.ep = (R)(k.E / k.Pm);
+    a.eE = (R)k.E;
+    a.ePm = (R)k.Pm;
+    if (c.cfl_on) c.cfl_parts = c.Th2;
     const int nblk = ((a.ngroups + 7) / 8) * 8 * PF_GROUP<R>;
     const bool r512 = physfuse_r512(c.Ph);
     const size_t lds = physfuse_lds<V>(c.Ph, r512);
@@ -456,12 +504,23 @@ static int physfuse_launch(Ctx& c, const void* fourier, void* folded, const void
     // algorithmic bytes: Fourier fields in, folded fields out
     ProfScope ps(c, LSD_K_PHYS,
                  (double)sizeof(V) * c.N * c.M * ((double)nin * c.Th + (double)nout * 2 * c.Th2));
-    if (r512) hipLaunchKernelGGL((k_physfuse<R, true, PF_THREADS>), dim3(nblk), dim3(PF_THREADS), lds, c.stream, a);
-    else if (c.Ph <= PF_THREADS)
-        hipLaunchKernelGGL((k_physfuse<R, false, PF_THREADS>), dim3(nblk), dim3(PF_THREADS), lds, c.stream, a);
-    else
-        hipLaunchKernelGGL((k_physfuse<R, false, PF_THREADS_MAX>), dim3(nblk), dim3(PF_THREADS_MAX), lds, c.stream,
-                           a);
+    if (c.cfl_on) {
+        if (r512)
+            hipLaunchKernelGGL((k_physfuse<R, true, PF_THREADS, true>), dim3(nblk), dim3(PF_THREADS), lds, c.stream, a);
+        else if (c.Ph <= PF_THREADS)
+            hipLaunchKernelGGL((k_physfuse<R, false, PF_THREADS, true>), dim3(nblk), dim3(PF_THREADS), lds, c.stream,
+                               a);
+        else
+            hipLaunchKernelGGL((k_physfuse<R, false, PF_THREADS_MAX, true>), dim3(nblk), dim3(PF_THREADS_MAX), lds,
+                               c.stream, a);
+    } else if (r512) {
+        hipLaunchKernelGGL((k_physfuse<R, true, PF_THREADS, false>), dim3(nblk), dim3(PF_THREADS), lds, c.stream, a);
+    } else if (c.Ph <= PF_THREADS) {
+        hipLaunchKernelGGL((k_physfuse<R, false, PF_THREADS, false>), dim3(nblk), dim3(PF_THREADS), lds, c.stream, a);
+    } else {
+        hipLaunchKernelGGL((k_physfuse<R, false, PF_THREADS_MAX, false>), dim3(nblk), dim3(PF_THREADS_MAX), lds,
+                           c.stream, a);
+    }
     LSD_HIP(hipGetLastError());
     return LSD_OK;
 }

@@ -27,6 +27,7 @@
       integer(c_int) :: vel_bc, cod_bc, comp_bc
       integer(c_int) :: precision
       real(c_double) :: rratio, Ra, Ra_comp, Pr, Sc, Pm, E, dt, implicit, dterr, source, source_comp
+      integer(c_int) :: step_control
    end type lsd_config
 
    type, bind(C) :: lsd_step_stats

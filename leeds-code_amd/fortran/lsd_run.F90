@@ -49,7 +49,7 @@
    cfg%vel_bc = 1; cfg%cod_bc = 1; cfg%comp_bc = 1; cfg%precision = precision
    cfg%rratio = rratio; cfg%Ra = Ra; cfg%Ra_comp = Ra_comp; cfg%Pr = Pr; cfg%Sc = Sc
    cfg%Pm = Pm; cfg%E = E; cfg%dt = dt; cfg%implicit = implicit; cfg%dterr = dterr
-   cfg%source = source; cfg%source_comp = source_comp
+   cfg%source = source; cfg%source_comp = source_comp; cfg%step_control = 2
 
    call lsd_check(lsd_config_sizes(cfg, H, iTh, iPh), 'par_precompute')
    call lsd_check(lsd_create(cfg, 0_c_int, ctx), 'initialise')

@@ -10,7 +10,8 @@
  module transform
    use parameters, only : i_L, i_M, i_Mp, i_N, i_Th, i_Ph, d_rratio, d_Ra, d_Ra_comp, d_Pr, d_Sc, &
                           d_Pm, d_E, d_timestep, d_implicit, d_dterr, d_source, d_source_comp,      &
-                          b_cod_tstep, b_comp_tstep, b_mag_tstep, i_vel_bc, i_cod_bc, i_comp_bc
+                          b_cod_tstep, b_comp_tstep, b_mag_tstep, i_vel_bc, i_cod_bc, i_comp_bc, &
+                          i_time_step_control
    use lsd_c
    implicit none
    save
@@ -37,6 +38,7 @@
       cfg%rratio = d_rratio; cfg%Ra = d_Ra; cfg%Ra_comp = d_Ra_comp; cfg%Pr = d_Pr; cfg%Sc = d_Sc
       cfg%Pm = d_Pm; cfg%E = d_E; cfg%dt = abs(d_timestep); cfg%implicit = d_implicit
       cfg%dterr = d_dterr; cfg%source = d_source; cfg%source_comp = d_source_comp
+      cfg%step_control = i_time_step_control
       call lsd_check(lsd_create(cfg, 0_c_int, tra_ctx), 'tra_precompute')
       call lsd_check(lsd_get_table(tra_ctx, LSD_T_GW, tra_gw), 'tra_precompute')
    end subroutine tra_precompute

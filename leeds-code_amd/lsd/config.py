@@ -24,6 +24,7 @@ class LsdConfig(ctypes.Structure):
         ("Pr", ctypes.c_double), ("Sc", ctypes.c_double), ("Pm", ctypes.c_double),
         ("E", ctypes.c_double), ("dt", ctypes.c_double), ("implicit", ctypes.c_double),
         ("dterr", ctypes.c_double), ("source", ctypes.c_double), ("source_comp", ctypes.c_double),
+        ("step_control", ctypes.c_int),
     ]
 
     def sizes(self):
@@ -102,6 +103,20 @@ def grid_for(L, M):
         if n == 1 and ph % 2 == 0:
             return th, ph
         ph += 1
+
+
+def christensen_case1(**kw):
+    """The reference's Christensen case-1 test (tests/parameters-case1.F90,
+    at the resolution of its state file tests/state.cdf.in and of
+    Benchmark_files/Boussinesq/Christensen_case1_benchmark): N = 64, L = 64,
+    M = 16, Mp = 4, Boussinesq dynamo with codensity, E = 1e-3,
+    Ra = 6.5e4, Pr = 1, Pm = 5, d_implicit = 0.5, adaptive step
+    (d_timestep < 0) under i_time_step_control = 1, starting from the state
+    file's dt."""
+    base = dict(L=64, M=16, Mp=4, N=64, Th=96, Ph=48, comp=0, Ra=6.5e4, Pr=1.0, Pm=5.0, E=1e-3,
+                dt=-6.106330848710143e-05, implicit=0.5, step_control=1)
+    base.update(kw)
+    return make_config(**base)
 
 
 def config_small(**kw):

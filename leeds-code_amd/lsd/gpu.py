@@ -26,7 +26,8 @@ _dp = ctypes.POINTER(ctypes.c_double)
 EXPORTS = (
     "lsd_last_error", "lsd_config_sizes", "lsd_create", "lsd_destroy", "lsd_set_coll", "lsd_get_coll",
     "lsd_get_phys", "lsd_get_table", "lsd_get_mesh", "lsd_transform", "lsd_nonlinear", "lsd_predictor",
-    "lsd_corrector", "lsd_step", "lsd_set_dt", "lsd_tra_spec2phys", "lsd_tra_phys2spec", "lsd_tra_grad",
+    "lsd_corrector", "lsd_step", "lsd_set_dt", "lsd_get_dt",
+    "lsd_read_state", "lsd_write_state", "lsd_load_state", "lsd_save_state", "lsd_tra_spec2phys", "lsd_tra_phys2spec", "lsd_tra_grad",
     "lsd_tra_qst2rtp", "lsd_tra_rtp2qst", "lsd_sync", "lsd_profile_begin", "lsd_profile_end", "lsd_set_graph", "lsd_energy", "lsd_step_many",
     "lsd_mlp_create", "lsd_mlp_destroy", "lsd_mlp_num_params", "lsd_mlp_get_params", "lsd_mlp_set_params",
     "lsd_mlp_forward", "lsd_mlp_grad", "lsd_mlp_train",
@@ -72,6 +73,12 @@ def load(path=LIB_PATH):
     lib.lsd_corrector.argtypes = [ctypes.c_void_p, _dp]
     lib.lsd_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(StepStats)]
     lib.lsd_set_dt.argtypes = [ctypes.c_void_p, ctypes.c_double]
+    lib.lsd_read_state.argtypes = [ctypes.c_char_p, ctypes.POINTER(LsdConfig), _dp, _dp, _dp]
+    lib.lsd_write_state.argtypes = [ctypes.c_char_p, ctypes.POINTER(LsdConfig), _dp, ctypes.c_double,
+                                    ctypes.c_double]
+    lib.lsd_load_state.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+    lib.lsd_save_state.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+    lib.lsd_get_dt.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_double)] * 3
     lib.lsd_set_graph.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.lsd_energy.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp]
     lib.lsd_step_many.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int,
@@ -220,6 +227,20 @@ class Context:
     def set_dt(self, dt):
         self._chk(self.lib.lsd_set_dt(self.h, dt), "lsd_set_dt")
 
+    def load_state(self, path):
+        """io_load_state: fields, tim_t (and tim_dt for an adaptive run)."""
+        self._chk(self.lib.lsd_load_state(self.h, os.fsencode(path)), "lsd_load_state")
+
+    def save_state(self, path):
+        """io_save_state of the current fields, tim_t and tim_dt."""
+        self._chk(self.lib.lsd_save_state(self.h, os.fsencode(path)), "lsd_save_state")
+
+    def get_dt(self):
+        """(tim_dt, tim_cfl_dt, tim_corr_dt)"""
+        v = [ctypes.c_double() for _ in range(3)]
+        self._chk(self.lib.lsd_get_dt(self.h, *[ctypes.byref(x) for x in v]), "lsd_get_dt")
+        return tuple(x.value for x in v)
+
     def energy(self, which):
         """(E, El[L], Em[M]) of the velocity (0) or magnetic field (1)."""
         E = ctypes.c_double()
@@ -271,6 +292,35 @@ class StepManyError(LsdError):
     def __init__(self, msg, stats):
         super().__init__(msg)
         self.stats = stats
+
+
+def read_state(path, cfg):
+    """lsd_read_state (no GPU needed): ({name: (re, im)} in the coll layout
+    for uT, uP, C, X, BT, BP; t; dt)."""
+    lib = load()
+    H = cfg.sizes()[0]
+    buf = np.zeros((6, 2, H, cfg.N))
+    t, dt = ctypes.c_double(), ctypes.c_double()
+    c = LsdConfig.from_buffer_copy(cfg)
+    rc = lib.lsd_read_state(os.fsencode(path), ctypes.byref(c), _p(buf), ctypes.byref(t), ctypes.byref(dt))
+    if rc != 0:
+        raise LsdError("lsd_read_state: %s (code %d)" % (lib.lsd_last_error().decode(), rc))
+    names = ("uT", "uP", "C", "X", "BT", "BP")
+    return {n: (buf[k, 0].copy(), buf[k, 1].copy()) for k, n in enumerate(names)}, t.value, dt.value
+
+
+def write_state(path, cfg, state, t, dt):
+    """lsd_write_state (no GPU needed) of {name: (re, im)}."""
+    lib = load()
+    H = cfg.sizes()[0]
+    buf = np.zeros((6, 2, H, cfg.N))
+    for k, n in enumerate(("uT", "uP", "C", "X", "BT", "BP")):
+        if n in state:
+            buf[k, 0], buf[k, 1] = state[n]
+    c = LsdConfig.from_buffer_copy(cfg)
+    rc = lib.lsd_write_state(os.fsencode(path), ctypes.byref(c), _p(buf), t, dt)
+    if rc != 0:
+        raise LsdError("lsd_write_state: %s (code %d)" % (lib.lsd_last_error().decode(), rc))
 
 
 def step_many(contexts, nsteps):

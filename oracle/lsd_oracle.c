@@ -504,7 +504,14 @@ struct orc_sim {
     double t;
     int it, step;
     double dterr, lasterr, corr_dt;
+    int adaptive;          /* d_timestep < 0: CFL / corrector controlled dt */
+    double cfl_dt;         /* tim_cfl_dt of the last non_maxtstep */
 };
+
+/* d_courant = 0.8 is a default-real literal in the reference
+ * (Scripts/parameters.F90_template:238), so its value is 0.8 rounded to
+ * single precision. */
+static const double COURANT = (double)0.8f;
 
 static coll_t coll_new(int n) {
     coll_t c;
@@ -790,7 +797,10 @@ orc_sim* orc_create(const lsd_config* cfg) {
     s->c = *cfg;
     s->L = cfg->L; s->M = cfg->M; s->Mp = cfg->Mp; s->N = cfg->N;
     s->Th = Th; s->Ph = Ph; s->H = H; s->Th2 = (Th + 1) / 2;
-    s->dt = cfg->dt;
+    /* d_timestep < 0 selects the adaptive step; its magnitude stands for the
+     * dt io_load_state reads from the state file (program/io.F90:429) */
+    s->adaptive = cfg->dt < 0.0;
+    s->dt = fabs(cfg->dt);
     int N = s->N, L = s->L;
     s->hl = (int*)malloc(sizeof(int) * H);
     s->hm = (int*)malloc(sizeof(int) * H);
@@ -1698,20 +1708,29 @@ int orc_corrector(orc_sim* s, double* dterr_out) {
     return 0;
 }
 
-/* tim_check_cgce, i_time_step_control = 2 (modules/timestep.F90:791-827);
- * returns 1 if the run must stop. */
+/* tim_check_cgce (modules/timestep.F90:744-830): control 2 iterates while
+ * tim_dterr > d_dterr with tim_corr_dt = dt sqrt(d_dterr / tim_dterr),
+ * control 1 while tim_dterr > 1 with tim_corr_dt = tim_dterr; returns 1 if
+ * the run must stop. */
 static int check_cgce(orc_sim* s) {
     int stop = 0;
+    const int ctl1 = s->c.step_control == 1;
+    const double crit = ctl1 ? 1.0 : s->c.dterr;
     if (s->it == 1) {
-        s->corr_dt = s->dt * sqrt(s->c.dterr / s->dterr);
+        s->corr_dt = ctl1 ? s->dterr : s->dt * sqrt(s->c.dterr / s->dterr);
         s->lasterr = 1e99;
     }
     if (s->dt < 1e-12 && s->step > 30) { stop = 1; s->it = 0; }
     else if (s->it > 10) { stop = 1; s->it = 0; }
     else if (s->dterr > s->lasterr) {
-        if (s->dterr > 2.0 * s->c.dterr) stop = 1;
+        if (ctl1) {
+            if (s->dterr > 1.0) stop = 1;
+        } else {
+            if (s->dterr > 2.0 * s->c.dterr) stop = 1;
+            if (s->dterr < 2.0 * s->c.dterr) s->corr_dt = s->dt / (1e1 * COURANT);
+        }
         s->it = 0;
-    } else if (s->dterr > s->c.dterr) {
+    } else if (s->dterr > crit) {
         s->lasterr = s->dterr;
         s->it = s->it + 1;
     } else {
@@ -1721,7 +1740,88 @@ static int check_cgce(orc_sim* s) {
     return stop;
 }
 
-/* program/main.F90:100-164 with fixed dt and no I/O. */
+/* non_maxtstep (modules/nonlinear.F90:545-615): the CFL limit of the
+ * physical velocity and magnetic fields of the last transform, shell by
+ * shell: radially dr / max(Br^2 / sqrt(Br^2 E/Pm + d) + |ur|) with
+ * d = ((E + E/Pm) / (2 dr))^2 and dr the smaller neighbour spacing, and
+ * horizontally dr_h / max(Bh^2 / sqrt(Bh^2 E/Pm + d_h) + |uh|) with
+ * dr_h = r / sqrt(L (L-1)) and d_h = (E / (2 dr_h))^2; then capped by E/Pm
+ * and sqrt(E). */
+static double maxtstep(const orc_sim* s) {
+    const int N = s->N, Th = s->Th, Ph = s->Ph;
+    const double E = s->c.E, Pm = s->c.Pm;
+    const double* r = s->rp + 4 * N; /* r^1 */
+    const size_t plane = (size_t)Th * Ph;
+    const double* zero = NULL;
+    double cfl = 1e99;
+    for (int n = 0; n < N; n++) {
+        const double* ur = s->phys[LSD_P_VEL_R] + n * plane;
+        const double* ut = s->phys[LSD_P_VEL_T] + n * plane;
+        const double* up = s->phys[LSD_P_VEL_P] + n * plane;
+        const double* br = s->c.mag ? s->phys[LSD_P_MAG_R] + n * plane : zero;
+        const double* bt = s->c.mag ? s->phys[LSD_P_MAG_T] + n * plane : zero;
+        const double* bp = s->c.mag ? s->phys[LSD_P_MAG_P] + n * plane : zero;
+        double dr;
+        if (n == 0) dr = r[1] - r[0];
+        else if (n == N - 1) dr = r[N - 1] - r[N - 2];
+        else dr = fmin(r[n] - r[n - 1], r[n + 1] - r[n]);
+        double d = (E + E / Pm) / (2.0 * dr);
+        d = d * d;
+        double mx = -1e300;
+        for (size_t i = 0; i < plane; i++) {
+            const double b = br ? br[i] : 0.0;
+            const double p = b * b;
+            const double v = p / sqrt(p * (E / Pm) + d) + fabs(ur[i]);
+            if (v > mx) mx = v;
+        }
+        if (mx != 0.0) cfl = fmin(cfl, dr / mx);
+        dr = r[n] / sqrt((double)(s->L * (s->L - 1)));
+        d = E / (2.0 * dr);
+        d = d * d;
+        mx = -1e300;
+        for (size_t i = 0; i < plane; i++) {
+            const double b1 = bt ? bt[i] : 0.0, b2 = bp ? bp[i] : 0.0;
+            const double p = b1 * b1 + b2 * b2;
+            const double v = p / sqrt(p * E / Pm + d) + sqrt(ut[i] * ut[i] + up[i] * up[i]);
+            if (v > mx) mx = v;
+        }
+        if (mx != 0.0) cfl = fmin(cfl, dr / mx);
+    }
+    cfl = fmin(cfl, E / Pm);
+    cfl = fmin(cfl, sqrt(E));
+    return cfl;
+}
+
+/* tim_new_tstep (modules/timestep.F90:837-870); returns 1 if dt changed
+ * (tim_new_dt).  d_corr_upper = 0.02, d_corr_lower = 0.002
+ * (Scripts/parameters.F90_template:242-243). */
+static int new_tstep(orc_sim* s) {
+    double dt = s->dt;
+    if (s->c.step_control == 1) {
+        if (s->corr_dt > 0.020) dt = 0.5 * dt;
+        if (s->corr_dt < 0.00200) dt = 1.273974 * dt;
+        if (s->step == 0) dt = fmin(dt / 1e3, 1e-8);
+    } else {
+        if (s->corr_dt > 0.0) dt = fmin(s->corr_dt, dt * 2.0 / COURANT);
+        if (s->cfl_dt > 0.0) dt = fmin(s->cfl_dt, dt);
+        if (s->step == 0) dt = fmin(dt / 1e3, 1e-10);
+        dt = dt * COURANT;
+    }
+    const int changed = dt < s->dt * 0.97 || dt > s->dt * 1.03;
+    if (changed) s->dt = dt;
+    return changed;
+}
+
+int orc_get_dt(orc_sim* s, double* dt, double* cfl_dt, double* corr_dt) {
+    if (dt) *dt = s->dt;
+    if (cfl_dt) *cfl_dt = s->cfl_dt;
+    if (corr_dt) *corr_dt = s->corr_dt;
+    return 0;
+}
+
+/* program/main.F90:100-164 without I/O; with d_timestep < 0 the step is
+ * chosen at the start of each step by non_maxtstep and tim_new_tstep and
+ * the operators are rebuilt when it changes (main.F90:124-135). */
 int orc_step(orc_sim* s, int nsteps, lsd_step_stats* st) {
     lsd_step_stats z;
     memset(&z, 0, sizeof z);
@@ -1733,6 +1833,13 @@ int orc_step(orc_sim* s, int nsteps, lsd_step_stats* st) {
             orc_nonlinear(s);
             z.evaluations++;
             if (s->it == 0) {
+                if (s->adaptive) {
+                    s->cfl_dt = maxtstep(s);
+                    if (new_tstep(s)) {
+                        int rc = matrices(s);
+                        if (rc) return rc;
+                    }
+                }
                 orc_predictor(s);
                 s->it = 1;
             } else {

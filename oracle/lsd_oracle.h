@@ -51,6 +51,8 @@ int  orc_predictor(orc_sim* s);
 int  orc_corrector(orc_sim* s, double* dterr);
 int  orc_step(orc_sim* s, int nsteps, lsd_step_stats* st);
 int  orc_set_dt(orc_sim* s, double dt);
+/* tim_dt, tim_cfl_dt and tim_corr_dt (modules/timestep.F90:13-18). */
+int  orc_get_dt(orc_sim* s, double* dt, double* cfl_dt, double* corr_dt);
 
 int  orc_tra_spec2phys(orc_sim* s, const double* sre, const double* sim, double* p);
 int  orc_tra_phys2spec(orc_sim* s, const double* p, double* sre, double* sim);

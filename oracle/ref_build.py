@@ -50,8 +50,10 @@ AUX = ["gauwts", "schnla", "pmm", "pmm1", "plm", "dpmm", "dpmm1", "dplm"]
 def cases():
     """Case name -> lsd_config of the run (the same values the GPU path and
     the C oracle are given)."""
-    from lsd.config import config1, config3, config_small
-    return {"cfg1": config1(), "small": config_small(), "cfg3": config3()}
+    from lsd.config import config1, config3, config_small, christensen_case1
+    return {"cfg1": config1(), "small": config_small(), "cfg3": config3(),
+            "adapt": config_small(dt=-1e-4), "adapt1": config_small(dt=-1e-4, step_control=1),
+            "case1": christensen_case1()}
 
 
 def fdouble(x):
@@ -75,7 +77,7 @@ def edits(cfg):
           ("b_comp_tstep", flogical(cfg.comp)), ("b_mag_tstep", flogical(cfg.mag)),
           ("i_vel_bc", str(cfg.vel_bc)), ("i_cod_bc", str(cfg.cod_bc)), ("i_comp_bc", str(cfg.comp_bc)),
           ("d_source", fdouble(cfg.source)), ("d_source_comp", fdouble(cfg.source_comp)),
-          ("i_time_step_control", "2"), ("d_dterr", fdouble(cfg.dterr)),
+          ("i_time_step_control", "1" if cfg.step_control == 1 else "2"), ("d_dterr", fdouble(cfg.dterr)),
           ("d_timestep", fdouble(cfg.dt)), ("d_implicit", fdouble(cfg.implicit))]
     # grids that differ from the template's 3/2 rule: the rule lines
     # themselves take the case's sizes (Ph = 2 i_Ma, so i_Ma = Ph/2)

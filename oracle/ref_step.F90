@@ -24,6 +24,8 @@
 ! Initialisation follows program/main.F90:220-400 with io_load_state
 ! replaced by the raw read above and io_load_radial_txtfiles by its
 ! Boussinesq branch (program/io.F90:508-550: eta = 1, d eta/dr = 0).
+! With d_timestep < 0 each step starts with non_maxtstep, tim_new_tstep
+! and the *_matrices rebuild on a new dt (program/main.F90:124-135).
 !-------------------------------------------------------------------------
 #include "../parallel.h"
  program ref_step
@@ -32,13 +34,15 @@
    use parameters
    use legendre, only : leg_precompute, leg_gp, leg_gw, leg_P, leg_gwP
    use transform, only : tra_precompute
-   use timestep, only : tim_it, tim_t, tim_dt, tim_step, tim_dterr, tim_check_cgce, tim_precompute
+   use timestep, only : tim_it, tim_t, tim_dt, tim_step, tim_dterr, tim_check_cgce, tim_precompute, &
+                        tim_new_tstep, tim_new_dt
    use codensity, only : cod_precompute, cod_transform, cod_matrices, cod_predictor, cod_corrector
    use composition, only : comp_precompute, comp_transform, comp_matrices, comp_predictor, comp_corrector
    use velocity, only : vel_precompute, vel_angmomclr, vel_transform, vel_matrices, vel_predictor, &
                         vel_corrector
    use magnetic, only : mag_precompute, mag_transform, mag_matrices, mag_predictor, mag_corrector
-   use nonlinear, only : non_precompute, non_codensity, non_composition, non_magnetic, non_velocity
+   use nonlinear, only : non_precompute, non_codensity, non_composition, non_magnetic, non_velocity, &
+                         non_maxtstep
    use rotation, only : rot_precompute
    implicit none
 
@@ -102,8 +106,10 @@
    call read_coll(mag_tor)
    call read_coll(mag_pol)
    close(uin)
+   ! with d_timestep < 0 (adaptive step) io_load_state takes the initial dt
+   ! from the state file (program/io.F90:429); here its magnitude stands in
    tim_t = 0d0
-   tim_dt = d_timestep
+   tim_dt = abs(d_timestep)
 
    call vel_angmomclr(mes_oc, vel_tor)
    call vel_transform(mes_oc, vel_tor, vel_pol, vel_r, vel_t, vel_p, &
@@ -126,7 +132,7 @@
    write(uout) mes_oc%radLap%M, mes_oc%r1dr%M, mes_oc%intr2dr
    write(uout) leg_gp, leg_gw, leg_P, leg_gwP
 
-   ! program/main.F90:100-164 (no rotation, fixed dt, no output files)
+   ! program/main.F90:100-164 (no rotation, no output files)
    nev = 0
    accepted_err = 0d0
    call system_clock(c0, crate)
@@ -158,6 +164,16 @@
                                          mag_NTor, mag_NPol, mag_icNTor, mag_icNPol)
       nev = nev + 1
       if (tim_it == 0) then
+         if (d_timestep < 0d0) then
+            call non_maxtstep(mes_oc, vel_r, vel_t, vel_p, mag_r, mag_t, mag_p)
+            call tim_new_tstep()
+            if (tim_new_dt) then
+               call vel_matrices(mes_oc)
+               call cod_matrices(mes_oc)
+               call comp_matrices(mes_oc)
+               call mag_matrices(mes_oc, mes_ic)
+            end if
+         end if
          if (b_vel_tstep) call vel_predictor(mes_oc, vel_ntor, vel_npol, rot_omega, vel_tor, vel_pol)
          if (b_cod_tstep) call cod_predictor(cod_Nc, cod_bcRe, cod_bcIm, cod_C)
          if (b_comp_tstep) call comp_predictor(comp_NC, comp_bcRe, comp_bcIm, comp_C)

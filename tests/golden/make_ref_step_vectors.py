@@ -31,7 +31,7 @@ sys.path.insert(0, os.path.join(ROOT, "leeds-code_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 import ref_io  # noqa: E402
-from lsd.config import config1, config3, config_small  # noqa: E402
+from lsd.config import config1, config3, config_small, christensen_case1  # noqa: E402
 from lsd.initial import AMPS, SEEDS, harmonics, seeded_state  # noqa: E402
 
 CASES = {
@@ -39,9 +39,23 @@ CASES = {
     "small": (config_small, 7, 1e-3, 20),
     "cfg1": (config1, SEEDS[1], AMPS[1], 100),
     "cfg3": (config3, SEEDS[3], AMPS[3], 2),
+    # adaptive step (d_timestep < 0): non_maxtstep and tim_new_tstep choose
+    # dt at every step; tim_t records the steps taken
+    "adapt": (lambda: config_small(dt=-1e-4), 9, 1e-3, 40),
+    # the same with i_time_step_control = 1 (tim_corr_dt = tim_dterr, dt
+    # halved above d_corr_upper and grown by 1.273974 below d_corr_lower)
+    "adapt1": (lambda: config_small(dt=-1e-4, step_control=1), 10, 1e-3, 40),
+    # the Christensen case-1 dynamo (N = 64, L = 64, M = 16, Mp = 4) from the
+    # reference's test state tests/state.cdf.in (tests/golden/state_case1.cdf.in),
+    # adaptive step under control 1; seed -1 marks a state read from a file
+    "case1": (christensen_case1, -1, 0.0, 30),
 }
-FULL = {"small": (1, 2, 20), "cfg1": (1,), "cfg3": ()}
-SAMPLED = {"small": (), "cfg1": (10, 100), "cfg3": (1, 2)}
+FULL = {"small": (1, 2, 20), "cfg1": (1,), "cfg3": (), "adapt": (1, 10, 40), "adapt1": (1, 20, 40),
+        "case1": ()}
+SAMPLED = {"small": (), "cfg1": (10, 100), "cfg3": (1, 2), "adapt": (), "adapt1": (), "case1": (1, 10, 30)}
+STATE_FILE = {"case1": "state_case1.cdf.in"}
+# cases whose fixtures also record the reference's own per-step spread
+SPREAD = ("adapt", "adapt1", "case1")
 NSAMPLE = 4096
 
 
@@ -56,11 +70,42 @@ def make(case):
     mk, seed, amp, nsteps = CASES[case]
     cfg = mk()
     H, Th, Ph, Th2 = cfg.sizes()
-    state = seeded_state(cfg, seed, amp)
-    recs, _ = ref_io.run(case, cfg, state, nsteps, 1)
-    l, m = harmonics(cfg.L, cfg.M, cfg.Mp)
     out = {"config": np.array(json.dumps(cfg.as_dict())), "seed": np.array(seed), "amp": np.array(amp),
            "nsteps": np.array(nsteps)}
+    if case in STATE_FILE:
+        sys.path.insert(0, os.path.dirname(HERE))
+        import ref_fixtures
+        out["state_file"] = np.array(STATE_FILE[case])
+        out["b_scale"] = np.array(ref_fixtures.B_SCALE_OLD_STATE)
+        state = ref_fixtures.initial_state(out, cfg)
+    else:
+        state = seeded_state(cfg, seed, amp)
+    recs, _ = ref_io.run(case, cfg, state, nsteps, 1)
+    if case in SPREAD:
+        # the reference's own rounding spread: the same run on MKL's
+        # COMPATIBLE code path, per step the largest relative difference of
+        # a field from the run above
+        env = os.environ.get("MKL_CBWR")
+        os.environ["MKL_CBWR"] = "COMPATIBLE"
+        try:
+            alt, _ = ref_io.run(case, cfg, state, nsteps, 1)
+        finally:
+            if env is None:
+                del os.environ["MKL_CBWR"]
+            else:
+                os.environ["MKL_CBWR"] = env
+        spread = []
+        for a, b in zip(recs[1:], alt[1:]):
+            worst = 0.0
+            for name, (re, im) in a["fields"].items():
+                re2, im2 = b["fields"][name]
+                den = np.sqrt((re ** 2).sum() + (im ** 2).sum())
+                if den > 0:
+                    worst = max(worst, np.sqrt(((re - re2) ** 2).sum() + ((im - im2) ** 2).sum()) / den)
+            spread.append(worst)
+        out["spread"] = np.array(spread)
+        out["dterr_spread"] = np.array([abs(a["dterr"] - b["dterr"]) for a, b in zip(recs[1:], alt[1:])])
+    l, m = harmonics(cfg.L, cfg.M, cfg.Mp)
     st = recs[1:]
     out["step"] = np.array([r["step"] for r in st], dtype=np.int32)
     out["evaluations"] = np.array([r["evaluations"] for r in st], dtype=np.int32)

@@ -44,6 +44,7 @@ def load():
     lib.orc_corrector.argtypes = [ctypes.c_void_p, _dp]
     lib.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
     lib.orc_set_dt.argtypes = [ctypes.c_void_p, ctypes.c_double]
+    lib.orc_get_dt.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_double)] * 3
     lib.orc_gauwts.argtypes = [ctypes.c_double, ctypes.c_double, _dp, _dp, ctypes.c_int]
     lib.orc_schnla.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_int]
     lib.orc_harmonics.argtypes = [ctypes.c_int] * 3 + [_ip, _ip]
@@ -61,9 +62,9 @@ def load():
     return lib
 
 
-class StepStats(ctypes.Structure):
-    _fields_ = [("steps", ctypes.c_int), ("evaluations", ctypes.c_int), ("last_its", ctypes.c_int),
-                ("max_its", ctypes.c_int), ("last_dterr", ctypes.c_double), ("t", ctypes.c_double)]
+# lsd_step_stats of include/lsd.h, which orc_step fills: one definition for
+# the library and the oracle
+from lsd.gpu import StepStats  # noqa: E402
 
 
 class Oracle:
@@ -140,6 +141,12 @@ class Oracle:
 
     def set_dt(self, dt):
         self._chk(self.lib.orc_set_dt(self.h, dt), "orc_set_dt")
+
+    def get_dt(self):
+        """(tim_dt, tim_cfl_dt, tim_corr_dt)"""
+        v = [ctypes.c_double() for _ in range(3)]
+        self._chk(self.lib.orc_get_dt(self.h, *[ctypes.byref(x) for x in v]), "orc_get_dt")
+        return tuple(x.value for x in v)
 
     def step(self, n):
         st = StepStats()

@@ -19,7 +19,47 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 FIELDS = ("uT", "uP", "C", "X", "BT", "BP")
-REL_STEP = {"small": 1e-12, "cfg1": 1e-12, "cfg3": 6e-11}
+REL_STEP = {"small": 1e-12, "cfg1": 1e-12, "cfg3": 6e-11, "adapt": 1e-12, "adapt1": 1e-12, "case1": 1e-12}
+
+# tim_dterr is the norm of the corrector's increment X^-1 c (N* - N).  At
+# dt below ~1e-6 (the first steps of an adaptive run, which start from
+# 8e-11) N* - N is a difference of nearly equal terms and carries rounding
+# noise of order 1e-16 |N|, so tim_dterr differs between two correct runs by
+# up to its own size; there only the corrector count and the step choice
+# (tim_t) are compared.  From dt = 1e-6 on it agrees to 1e-8.
+DTERR_MIN_DT = 1e-6
+
+# The Dirichlet rows of the implicit operators are pivoted away by dgbtrf
+# against stencil rows that carry c1/dt, and what that leaves in the
+# boundary values grows as 1/dt: at the small steps an adaptive run starts
+# with (8e-11 .. 1e-6) two runs of the reference itself that differ only in
+# MKL's code path differ by up to 2.3e-9 (relative, per field).  The
+# fixtures of those cases record that spread per step ("spread": the
+# reference on MKL_CBWR=COMPATIBLE against the default path), and the bound
+# allows three times it on top of the per-step bound.
+
+
+def dterr_ok(fx, k, dterr):
+    """tim_dterr of step k against the fixture, from dt = DTERR_MIN_DT on:
+    to 1e-8 relative, plus three times the reference's own spread where the
+    fixture records it."""
+    ref = float(fx["dterr"][k - 1])
+    t = fx["t"]
+    dt = t[k - 1] - (t[k - 2] if k > 1 else 0.0)
+    if dt < DTERR_MIN_DT:
+        return True
+    tol = 1e-8 * ref
+    if "dterr_spread" in fx:
+        tol += 3.0 * float(fx["dterr_spread"][k - 1])
+    return abs(dterr - ref) <= tol
+
+
+def step_bound(fx, case, k):
+    """Bound on the relative per-field error after k steps."""
+    b = REL_STEP[case] * k
+    if "spread" in fx:
+        b += 3.0 * float(fx["spread"][k - 1])
+    return b
 
 
 def load(case):
@@ -38,7 +78,28 @@ def config(fx):
     return make_config(**d)
 
 
+# tests/state.cdf.in holds B in the normalisation of the reference version
+# that wrote it and its Christensen case-1 series, whose magnetic energy is
+# (Pm / 2E) int B^2: at t = 0 the series gives ME = 228799.504998
+# (Benchmark_files/Boussinesq/Christensen_case1_benchmark/mag_energy.dat:2)
+# where the present code's Pm / E (program/io.F90:238-247) gives twice that
+# for the file's B.  The present code, built from its sources and stepped
+# from the file as it is, loses kinetic energy at 1.8e7 per unit time
+# (11225.980 -> 11225.804 over a step of 1e-8), while the series grows at
+# about 450; with B / sqrt(2) the present code reproduces the series'
+# growth and its magnetic energy.  Runs from the file therefore take
+# B / sqrt(2).
+B_SCALE_OLD_STATE = 2.0 ** -0.5
+
+
 def initial_state(fx, cfg):
+    if "state_file" in fx:
+        from lsd.gpu import read_state
+        st, _, _ = read_state(os.path.join(HERE, "golden", str(fx["state_file"])), cfg)
+        b = float(fx["b_scale"])
+        for name in ("BT", "BP"):
+            st[name] = (st[name][0] * b, st[name][1] * b)
+        return st
     from lsd.initial import seeded_state
     return seeded_state(cfg, int(fx["seed"]), float(fx["amp"]))
 
