@@ -224,6 +224,16 @@ int lsd_step_many(lsd_ctx** ctxs, int n, int nsteps, lsd_step_stats* stats);
  * per-degree and per-order parts, times Efac. */
 int lsd_energy(lsd_ctx* ctx, int which, double* E, double* El, double* Em);
 
+/* lsd_energy without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The per-harmonic parts and sums live in one static device
+ * array per GPU (1.6 MB, held until the process ends), so H <= 65536 and
+ * L + M <= 2048 (LSD_E_ARG beyond).  Results equal lsd_energy's bit for
+ * bit.  All contexts on one GPU share the array: calls that overlap on one
+ * GPU (from several host threads) race. */
+int lsd_energy_noalloc(lsd_ctx* ctx, int which, double* E, double* El, double* Em);
+
 /* Whether lsd_step replays the timestep from captured HIP graphs (default 1)
  * or launches every kernel from the host (0).  No counterpart in the
  * reference; results are identical either way. */

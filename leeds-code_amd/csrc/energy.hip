@@ -3,6 +3,8 @@
 // weights w = 4 pi / (2l+1) (times 4 for m /= 0), radial integrals with
 // the int r^2 dr weights.  One work-item per harmonic integrates its q, s
 // and t columns; the sums over harmonics run on the host in a fixed order.
+#include <mutex>
+
 #include "lsd_internal.h"
 
 namespace lsd {
@@ -97,6 +99,50 @@ int energy_run(Ctx& c, int which, double* scratch, double* E, double* El, double
     if (Em) LSD_HIP(hipMemcpyAsync(Em, d_out + 1 + c.L, sizeof(double) * c.M, hipMemcpyDeviceToHost, c.stream));
     LSD_HIP(hipStreamSynchronize(c.stream));
     return LSD_OK;
+}
+
+// Scratch of lsd_energy_noalloc: the per-harmonic parts [3][H] and the
+// sums [1 + L + M], one copy per GPU for the life of the process.
+constexpr int ENERGY_MAX_H = 65536;
+constexpr int ENERGY_MAX_LM = 2048;
+constexpr size_t ENERGY_SCRATCH = (size_t)3 * ENERGY_MAX_H + 1 + ENERGY_MAX_LM;
+static __device__ __align__(256) double scratch[ENERGY_SCRATCH];
+
+constexpr int ENERGY_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[ENERGY_MAX_DEVICES] = {};
+
+// Device address of this device's copy of `scratch`; called once per device
+// from lsd_create with that device current.
+int energy_noalloc_init(int device) {
+    if (device < 0 || device >= ENERGY_MAX_DEVICES) {
+        set_error("lsd_create: device index beyond the static scratch table");
+        return LSD_E_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return LSD_OK;
+    double* ptr = nullptr;
+    LSD_HIP(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = ptr;
+    return LSD_OK;
+}
+
+int energy_noalloc(Ctx& c, int which, double* E, double* El, double* Em) {
+    if (c.H > ENERGY_MAX_H || c.L + c.M > ENERGY_MAX_LM) {
+        set_error("lsd_energy_noalloc: H = " + std::to_string(c.H) + ", L + M = " + std::to_string(c.L + c.M) +
+                  " exceed the static scratch (H <= 65536, L + M <= 2048); use lsd_energy");
+        return LSD_E_ARG;
+    }
+    double* ptr = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        ptr = (c.device >= 0 && c.device < ENERGY_MAX_DEVICES) ? g_scratch_ptr[c.device] : nullptr;
+    }
+    if (!ptr) {
+        set_error("lsd_energy_noalloc: static scratch not set up on this device");
+        return LSD_E_ARG;
+    }
+    return energy_run(c, which, ptr, E, El, Em);
 }
 
 int energy(Ctx& c, int which, double* E, double* El, double* Em) {

@@ -130,6 +130,8 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
     *out = c;
     LSD_HIP(hipSetDevice(device));
     LSD_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    rc = energy_noalloc_init(device);
+    if (rc) return rc;
 
     // host index bookkeeping
     harmonics(c->L, c->M, c->Mp, c->hl, c->hm);
@@ -1077,6 +1079,19 @@ int lsd_energy(lsd_ctx* ctx, int which, double* E, double* El, double* Em) {
         return LSD_E_ARG;
     }
     return lsd::energy(c, which, E, El, Em);
+}
+
+int lsd_energy_noalloc(lsd_ctx* ctx, int which, double* E, double* El, double* Em) {
+    CTX_OR_FAIL(ctx);
+    if ((which != 0 && which != 1) || !E) {
+        lsd::set_error("lsd_energy_noalloc: which must be 0 (velocity) or 1 (magnetic), E non-null");
+        return LSD_E_ARG;
+    }
+    if (which == 1 && !c.cfg.mag) {
+        lsd::set_error("lsd_energy_noalloc: the magnetic field is not time-stepped");
+        return LSD_E_ARG;
+    }
+    return lsd::energy_noalloc(c, which, E, El, Em);
 }
 
 int lsd_step_many(lsd_ctx** ctxs, int n, int nsteps, lsd_step_stats* stats) {

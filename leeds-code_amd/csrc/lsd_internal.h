@@ -285,6 +285,8 @@ int launch_nonlinear(Ctx& c);
 int energy(Ctx& c, int which, double* E, double* El, double* Em);
 size_t energy_scratch_doubles(const Ctx& c);
 int energy_run(Ctx& c, int which, double* scratch, double* E, double* El, double* Em);
+int energy_noalloc_init(int device);
+int energy_noalloc(Ctx& c, int which, double* E, double* El, double* Em);
 
 // state_io.hip: coll = [6][2][H][N] (uT, uP, C, Comp, BT, BP; Re, Im)
 int state_read(const char* path, const lsd_config& cfg, double* coll, double* t, double* dt);

@@ -28,7 +28,8 @@ EXPORTS = (
     "lsd_get_phys", "lsd_get_table", "lsd_get_mesh", "lsd_transform", "lsd_nonlinear", "lsd_predictor",
     "lsd_corrector", "lsd_step", "lsd_set_dt", "lsd_get_dt",
     "lsd_read_state", "lsd_write_state", "lsd_load_state", "lsd_save_state", "lsd_tra_spec2phys", "lsd_tra_phys2spec", "lsd_tra_grad",
-    "lsd_tra_qst2rtp", "lsd_tra_rtp2qst", "lsd_sync", "lsd_profile_begin", "lsd_profile_end", "lsd_set_graph", "lsd_energy", "lsd_step_many",
+    "lsd_tra_qst2rtp", "lsd_tra_rtp2qst", "lsd_sync", "lsd_profile_begin", "lsd_profile_end", "lsd_set_graph", "lsd_energy", "lsd_energy_noalloc",
+    "lsd_step_many",
     "lsd_mlp_create", "lsd_mlp_destroy", "lsd_mlp_num_params", "lsd_mlp_get_params", "lsd_mlp_set_params",
     "lsd_mlp_forward", "lsd_mlp_grad", "lsd_mlp_train",
 )
@@ -81,6 +82,7 @@ def load(path=LIB_PATH):
     lib.lsd_get_dt.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_double)] * 3
     lib.lsd_set_graph.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.lsd_energy.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp]
+    lib.lsd_energy_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp]
     lib.lsd_step_many.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int,
                                   ctypes.POINTER(StepStats)]
     _fp = ctypes.POINTER(ctypes.c_float)
@@ -247,6 +249,15 @@ class Context:
         El = np.zeros(self.cfg.L)
         Em = np.zeros(self.cfg.M)
         self._chk(self.lib.lsd_energy(self.h, which, ctypes.byref(E), _p(El), _p(Em)), "lsd_energy")
+        return E.value, El, Em
+
+    def energy_noalloc(self, which):
+        """As energy(), through lsd_energy_noalloc (no device allocation)."""
+        E = ctypes.c_double()
+        El = np.zeros(self.cfg.L)
+        Em = np.zeros(self.cfg.M)
+        self._chk(self.lib.lsd_energy_noalloc(self.h, which, ctypes.byref(E), _p(El), _p(Em)),
+                  "lsd_energy_noalloc")
         return E.value, El, Em
 
     def set_graph(self, on):
