@@ -196,9 +196,10 @@ int lsd_nonlinear(lsd_ctx* ctx);
 int lsd_predictor(lsd_ctx* ctx);
 int lsd_corrector(lsd_ctx* ctx, double* dterr);
 
-/* nsteps full timesteps of program/main.F90:100-164 with fixed dt: one
- * predictor and correctors until tim_check_cgce (modules/timestep.F90:791-827,
- * i_time_step_control = 2) accepts. */
+/* nsteps full timesteps of program/main.F90:100-164: one predictor and
+ * correctors until tim_check_cgce (modules/timestep.F90:744-830, control
+ * cfg->step_control) accepts; with cfg->dt < 0 a new dt is chosen at the
+ * start of every step (see lsd_get_dt). */
 int lsd_step(lsd_ctx* ctx, int nsteps, lsd_step_stats* stats);
 
 /* Ensembles: nsteps timesteps of each of n independent contexts (any mix of
@@ -268,9 +269,50 @@ int lsd_tra_rtp2qst(lsd_ctx* ctx, const double* r, const double* th, const doubl
                     double* q_re, double* q_im, double* s_re, double* s_im,
                     double* t_re, double* t_im);
 
+/* Synthesis -> analysis round trip of the Legendre transforms in the
+ * context's precision (FP64, FP32 or BF16x3), at the Fourier level, where the
+ * phi FFTs of a round trip cancel: kind 0 takes one scalar field (tra_spec2phys
+ * then tra_phys2spec), kind 1 the (q, s, t) of a vector field (tra_qst2rtp
+ * then tra_rtp2qst).  in / out: [nfield][2][H][N], Re then Im, coll layout.
+ * For measuring the transform kernels in isolation; no counterpart in the
+ * reference. */
+int lsd_sht_roundtrip(lsd_ctx* ctx, int kind, const double* in, double* out);
+
 /* Wait for all work queued on the context's stream (get_time bracket,
  * modules/dyn_mpi.F90:81-92). */
 int lsd_sync(lsd_ctx* ctx);
+
+/* Kernel variants (no counterpart in the reference): launch shapes and
+ * kernel forms that compute the same step, chosen per context.  Defaults
+ * come from the tuning cache (lsd_tune) when one is found, else the values
+ * given here.  Every variant gives the same results, bit for bit for the
+ * synthesis shapes and to rounding for the analysis and solve forms.
+ *  SYN_*     FP32 synthesis: m per workgroup, 4 (both hemispheres staged
+ *            per pass) or 8 (one hemisphere per pass); defaults 4, 8, 8
+ *  ANA       FP32 analysis: 0 wide-load (even Th/2 only; default), 1 LDS-staged
+ *  SOLVE     banded solves: 0 two-pass register-column FP64 (N in {32, 48,
+ *            64, 96}; default there), 1 LDS-column (the mixed forms: FP32
+ *            factors with one FP64-residual refinement sweep) */
+#define LSD_V_SYN_SCALAR 0
+#define LSD_V_SYN_VECTOR 1
+#define LSD_V_SYN_GRAD   2
+#define LSD_V_ANA        3
+#define LSD_V_SOLVE      4
+#define LSD_V_COUNT      5
+int lsd_set_variant(lsd_ctx* ctx, int which, int value);
+int lsd_get_variant(lsd_ctx* ctx, int which, int* value);
+
+/* Autotuner: times every variant of this context's sizes on the GPU (the
+ * synthesis and analysis launches on the current buffers, reps times each;
+ * the solve forms over whole timesteps from a saved copy of the state,
+ * which is restored afterwards), selects the fastest of each, and, if
+ * cache_path is non-NULL, records the choice under this context's sizes
+ * and precision in that cache file (created if absent).  lsd_create reads
+ * the cache named by the environment variable LSD_TUNING_CACHE, else
+ * tuning_cache.jsonl beside the library, when present.  times_us[LSD_V_COUNT
+ * * 2] (may be NULL) receives the time of variant 0 and 1 of each group
+ * (0 where a variant does not apply). */
+int lsd_tune(lsd_ctx* ctx, int reps, const char* cache_path, double* times_us);
 
 /* Kernel classes for per-launch timing. */
 #define LSD_K_SYNTH    0   /* Legendre synthesis                                */

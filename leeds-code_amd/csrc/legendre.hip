@@ -330,6 +330,7 @@ int launch_synthesis_group(Ctx& c, int kind, int njob, const void* const* in0, c
             set_error("synthesis: 1..6 transforms per launch");
             return LSD_E_ARG;
         }
+        if (c.cfg.precision == LSD_MIXED_BF16X3) return synb_launch(c, kind, njob, in0, in1, out0, out1);
         const size_t HTp = (size_t)c.Hp * c.Th2p;
         const size_t p0 = (kind == SYN_SCALAR ? 0 : kind == SYN_VECTOR ? 3 : 1) * HTp;
         const size_t p1 = (kind == SYN_VECTOR ? 4 : 2) * HTp;
@@ -370,17 +371,16 @@ constexpr int S4_TH = 64;                  // theta points per wave tile
 constexpr int S4_SH = 16;                  // shells per wave tile
 constexpr int S4_MG = 16;                  // m per block group (one 128-byte line of a Fourier row)
 // consecutive m per workgroup (one per wave) and hemispheres staged in LDS
-// per pass: the scalar kind 4 and both, the vector and gradient kinds 8 and
-// one (66 KB of LDS either way)
-template <int KIND>
-constexpr int S4_MQ = KIND == 0 ? 4 : 8;
-template <int KIND>
-constexpr int S4_NMIR = KIND == 0 ? 2 : 1;
+// per pass: 4 m and both hemispheres, or 8 m and one (66 KB of LDS either
+// way); a kernel variant (LSD_V_SYN_*), by default 4 for the scalar kind and
+// 8 for the vector and gradient kinds
+template <int MQ>
+constexpr int S4_NMIR = MQ == 4 ? 2 : 1;
 // operand buffers (one step of 8 harmonics each); with the 128 accumulators
 // of the vector and gradient kinds no more fit at two waves per SIMD
 constexpr int S4_NB = 4;
-template <int KIND>
-constexpr int S4_SHS = S4_TH * S4_MQ<KIND> + 2;  // LDS float2 per (hemisphere, shell), padded
+template <int MQ>
+constexpr int S4_SHS = S4_TH * MQ + 2;  // LDS float2 per (hemisphere, shell), padded
 
 struct Syn4Args {
     int N, Th, Th2, Th2p, M, MP, H;
@@ -406,12 +406,12 @@ __device__ __forceinline__ v4f mfma4(float a, float b, v4f c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-template <int KIND>
-__global__ void __launch_bounds__(64 * S4_MQ<KIND>) __attribute__((amdgpu_waves_per_eu(2)))
+template <int KIND, int MQ>
+__global__ void __launch_bounds__(64 * MQ) __attribute__((amdgpu_waves_per_eu(2)))
 k_syn4(Syn4Args a) {
     constexpr int NOUT = KIND == 0 ? 1 : 2;
     constexpr int NB = S4_NB;
-    constexpr int MQ = S4_MQ<KIND>, NMIR = S4_NMIR<KIND>, SHS = S4_SHS<KIND>;
+    constexpr int NMIR = S4_NMIR<MQ>, SHS = S4_SHS<MQ>;
     __shared__ float2 St[NMIR * S4_SH * SHS];
     // groups (16 m, job, shell block); members (m quad, theta tile), the
     // theta tiles of one m quad consecutive: they read the same coefficients
@@ -587,12 +587,17 @@ static int syn4_launch(Ctx& c, int kind, int njob, const float* tA, const float*
     a.nmo = (c.M + S4_MG - 1) / S4_MG;
     a.njob = njob;
     auto grid = [&](int mq) { return dim3(grid_blocks(a.nmo * njob * a.nsb, (S4_MG / mq) * a.nth)); };
-    if (kind == SYN_SCALAR)
-        hipLaunchKernelGGL(k_syn4<0>, grid(S4_MQ<0>), dim3(64 * S4_MQ<0>), 0, c.stream, a);
-    else if (kind == SYN_VECTOR)
-        hipLaunchKernelGGL(k_syn4<1>, grid(S4_MQ<1>), dim3(64 * S4_MQ<1>), 0, c.stream, a);
-    else
-        hipLaunchKernelGGL(k_syn4<2>, grid(S4_MQ<2>), dim3(64 * S4_MQ<2>), 0, c.stream, a);
+    const int mq = c.var[LSD_V_SYN_SCALAR + kind] == 4 ? 4 : 8;
+    if (kind == SYN_SCALAR) {
+        if (mq == 4) hipLaunchKernelGGL((k_syn4<0, 4>), grid(4), dim3(64 * 4), 0, c.stream, a);
+        else hipLaunchKernelGGL((k_syn4<0, 8>), grid(8), dim3(64 * 8), 0, c.stream, a);
+    } else if (kind == SYN_VECTOR) {
+        if (mq == 4) hipLaunchKernelGGL((k_syn4<1, 4>), grid(4), dim3(64 * 4), 0, c.stream, a);
+        else hipLaunchKernelGGL((k_syn4<1, 8>), grid(8), dim3(64 * 8), 0, c.stream, a);
+    } else {
+        if (mq == 4) hipLaunchKernelGGL((k_syn4<2, 4>), grid(4), dim3(64 * 4), 0, c.stream, a);
+        else hipLaunchKernelGGL((k_syn4<2, 8>), grid(8), dim3(64 * 8), 0, c.stream, a);
+    }
     LSD_HIP(hipGetLastError());
     return LSD_OK;
 }
@@ -1054,7 +1059,8 @@ int launch_analysis_group(Ctx& c, int kind, int njob, const void* const* g0, con
     size_t HT = (size_t)c.H * c.Th2;
     const double unit = 4.0 * (double)c.H * c.Th2 * c.N;
     ProfScope ps(c, LSD_K_ANAL, unit * (kind == ANA_SCALAR ? 1 : 5) * njob);
-    if (f32 && c.d_atab4) {
+    if (f32 && c.cfg.precision == LSD_MIXED_BF16X3) return anab_launch(c, kind, njob, g0, g1, g2, o0, o1, o2);
+    if (f32 && c.d_atab4 && c.var[LSD_V_ANA] == 0) {
         if (njob < 1 || njob > LG_MAXJ) {
             set_error("analysis: 1..6 transforms per launch");
             return LSD_E_ARG;

@@ -1,6 +1,10 @@
 // C-ABI entry points (include/lsd.h): context life cycle, field I/O and the
 // phases of the reference's main loop (program/main.F90:100-164).
+#include <dlfcn.h>
+
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -37,8 +41,8 @@ static int check_config(const lsd_config* c, int* H, int* Th, int* Ph) {
         set_error("i_time_step_control must be 1 or 2 (0 selects 2)");
         return LSD_E_CONFIG;
     }
-    if (c->precision != LSD_PLAIN_F64 && c->precision != LSD_MIXED_F32) {
-        set_error("precision must be LSD_PLAIN_F64 or LSD_MIXED_F32");
+    if (c->precision != LSD_PLAIN_F64 && c->precision != LSD_MIXED_F32 && c->precision != LSD_MIXED_BF16X3) {
+        set_error("precision must be LSD_PLAIN_F64, LSD_MIXED_F32 or LSD_MIXED_BF16X3");
         return LSD_E_CONFIG;
     }
     if (c->precision != LSD_PLAIN_F64 && ph > 1024) {
@@ -85,10 +89,11 @@ static void release(Ctx* c) {
     void* ptrs[] = {c->d_hl, c->d_hm, c->d_pidx, c->d_apos, c->d_slots, c->d_slot_order, c->d_mslot, c->d_gp, c->d_gw,
                     c->d_sin, c->d_tab, c->d_atab, c->d_r, c->d_dr1, c->d_dr2, c->d_radlap, c->d_r1dr,
                     c->d_intr2dr, c->d_ll, c->d_lu, c->d_piv, c->d_pivpk, c->d_y, c->d_gre, c->d_invg, c->d_info,
-                    c->d_lut32, c->d_xband, c->d_sblocks, c->d_sorder, c->d_tab32,
+                    c->d_lut32, c->d_xband, c->d_sblk[0], c->d_sblk[1], c->d_sord[0], c->d_sord[1], c->d_tab32,
                     c->d_atab32, c->d_tw32, c->d_slot_t0, c->d_row_h, c->d_atab4, c->d_ana_base, c->d_acol,
                     c->d_spec_in, c->d_spec_out, c->d_fourier, c->d_folded, c->d_phys, c->d_nlphys,
-                    c->d_tw, c->d_partial, c->d_cfl_d, c->d_cfl_part, c->d_cfl};
+                    c->d_tw, c->d_partial, c->d_cfl_d, c->d_cfl_part, c->d_cfl, c->d_tabb, c->d_atabb,
+                    c->d_bx_sbase};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (int k = 0; k < LSD_NUM_COLL; k++)
@@ -104,6 +109,81 @@ static void release(Ctx* c) {
     drop_graphs(*c);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+}
+
+// ------------------------------------------------------- tuning cache
+//
+// One JSON object per line, written by lsd_tune:
+//   {"L": 170, "M": 170, "Mp": 1, "N": 96, "Th": 256, "Ph": 512,
+//    "precision": 1, "syn_scalar": 4, "syn_vector": 8, "syn_grad": 8,
+//    "ana": 0, "solve": 0}
+// The last line matching a context's sizes and precision applies.
+
+static std::string tuning_path() {
+    if (const char* e = std::getenv("LSD_TUNING_CACHE")) return e;
+    Dl_info info;
+    if (dladdr(reinterpret_cast<void*>(&tuning_path), &info) && info.dli_fname) {
+        std::string p = info.dli_fname;
+        const size_t k = p.rfind('/');
+        return (k == std::string::npos ? std::string(".") : p.substr(0, k)) + "/tuning_cache.jsonl";
+    }
+    return "tuning_cache.jsonl";
+}
+
+struct TuneEntry {
+    int L, M, Mp, N, Th, Ph, prec, v[LSD_V_COUNT];
+};
+
+static bool parse_tune_line(const char* line, TuneEntry& e) {
+    return std::sscanf(line,
+                       " {\"L\": %d, \"M\": %d, \"Mp\": %d, \"N\": %d, \"Th\": %d, \"Ph\": %d, "
+                       "\"precision\": %d, \"syn_scalar\": %d, \"syn_vector\": %d, \"syn_grad\": %d, "
+                       "\"ana\": %d, \"solve\": %d",
+                       &e.L, &e.M, &e.Mp, &e.N, &e.Th, &e.Ph, &e.prec, &e.v[0], &e.v[1], &e.v[2], &e.v[3],
+                       &e.v[4]) == 12;
+}
+
+static void select_solve_form(Ctx& c);
+
+// Whether a variant value applies to this context.
+static bool variant_ok(const Ctx& c, int which, int value) {
+    switch (which) {
+        case LSD_V_SYN_SCALAR:
+        case LSD_V_SYN_VECTOR:
+        case LSD_V_SYN_GRAD: return value == 4 || value == 8;
+        case LSD_V_ANA: return value == 1 || (value == 0 && c.ana_wide_ok);
+        case LSD_V_SOLVE: return (value == 0 && c.solve_direct_ok) || (value == 1 && c.hb_form[1] >= 1);
+        default: return false;
+    }
+}
+
+static void load_tuning(Ctx& c) {
+    if (!c.ana_wide_ok) c.var[LSD_V_ANA] = 1;
+    FILE* f = std::fopen(tuning_path().c_str(), "r");
+    if (!f) return;
+    char line[1024];
+    TuneEntry e, hit;
+    bool found = false;
+    while (std::fgets(line, sizeof line, f))
+        if (parse_tune_line(line, e) && e.L == c.L && e.M == c.M && e.Mp == c.Mp && e.N == c.N && e.Th == c.Th &&
+            e.Ph == c.Ph && e.prec == c.cfg.precision) {
+            hit = e;
+            found = true;
+        }
+    std::fclose(f);
+    if (!found) return;
+    for (int k = 0; k < LSD_V_COUNT; k++)
+        if (variant_ok(c, k, hit.v[k])) c.var[k] = hit.v[k];
+    select_solve_form(c);
+}
+
+// Points the solve kernels at the block set of the selected form.
+static void select_solve_form(Ctx& c) {
+    const int form = (c.var[LSD_V_SOLVE] == 0 && c.solve_direct_ok) ? 0 : 1;
+    c.d_sblocks = c.d_sblk[form];
+    c.d_sorder = c.d_sord[form];
+    c.n_partial_blocks = c.nblk_form[form];
+    c.solve_hb = c.hb_form[form];
 }
 
 static int create(const lsd_config* cfg, int device, Ctx** out) {
@@ -182,37 +262,51 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
             if (dalloc(&c->d_rhs[k], HN)) return LSD_E_HIP;
         if (dalloc(&c->d_rhs2, HN)) return LSD_E_HIP;
     }
-    // solve blocks: harmonics grouped by degree, at most solve_hb per block
-    c->solve_hb = direct ? 32 : solve_block_harmonics(N, c->mixed);
-    if (c->solve_hb < 1) {
+    // solve blocks of both forms: harmonics grouped by degree, at most 32
+    // per block for the two-pass kernels, at most what the LDS-column
+    // kernels' LDS holds for those
+    c->solve_direct_ok = direct;
+    c->hb_form[0] = 32;
+    c->hb_form[1] = solve_block_harmonics(N, c->mixed);
+    if (c->hb_form[1] < 1 && !direct) {
         set_error("N too large for the banded-solve kernels");
         return LSD_E_CONFIG;
     }
-    std::vector<int> sorder;
-    std::vector<int4> sblocks;
+    std::vector<int> sorder[2];
+    std::vector<int4> sblocks[2];
     {
         std::vector<std::vector<int>> byl(L);
         for (int h = 0; h < H; h++) byl[c->hl[h]].push_back(h);
-        for (int l = 0; l < L; l++)
-            for (size_t k = 0; k < byl[l].size(); k += c->solve_hb) {
-                int cnt = (int)std::min(byl[l].size() - k, (size_t)c->solve_hb);
-                sblocks.push_back(make_int4(l, (int)sorder.size(), cnt, 0));
-                for (int q = 0; q < cnt; q++) sorder.push_back(byl[l][k + q]);
-            }
+        for (int form = 0; form < 2; form++) {
+            const int hb = c->hb_form[form];
+            if (hb < 1 || (form == 0 && !direct)) continue;
+            for (int l = 0; l < L; l++)
+                for (size_t k = 0; k < byl[l].size(); k += hb) {
+                    int cnt = (int)std::min(byl[l].size() - k, (size_t)hb);
+                    sblocks[form].push_back(make_int4(l, (int)sorder[form].size(), cnt, 0));
+                    for (int q = 0; q < cnt; q++) sorder[form].push_back(byl[l][k + q]);
+                }
+            if (dalloc(&c->d_sblk[form], sblocks[form].size()) || dalloc(&c->d_sord[form], H)) return LSD_E_HIP;
+            c->nblk_form[form] = (int)sblocks[form].size();
+        }
     }
-    if (dalloc(&c->d_sblocks, sblocks.size())) return LSD_E_HIP;
-    c->n_partial_blocks = (int)sblocks.size();
-    if (dalloc(&c->d_partial, (size_t)8 * c->n_partial_blocks)) return LSD_E_HIP;
-    LSD_HIP(hipHostMalloc((void**)&c->h_partial, sizeof(double) * 8 * c->n_partial_blocks, hipHostMallocDefault));
+    c->var[LSD_V_SOLVE] = direct ? 0 : 1;
+    select_solve_form(*c);
+    const int nblk_max = std::max(c->nblk_form[0], c->nblk_form[1]);
+    if (dalloc(&c->d_partial, (size_t)8 * nblk_max)) return LSD_E_HIP;
+    LSD_HIP(hipHostMalloc((void**)&c->h_partial, sizeof(double) * 8 * nblk_max, hipHostMallocDefault));
 
     hipStream_t st = c->stream;
     LSD_HIP(hipMemcpyAsync(c->d_hl, c->hl.data(), sizeof(int) * H, hipMemcpyHostToDevice, st));
     LSD_HIP(hipMemcpyAsync(c->d_hm, c->hm.data(), sizeof(int) * H, hipMemcpyHostToDevice, st));
     LSD_HIP(hipMemcpyAsync(c->d_pidx, pidx.data(), sizeof(int) * pidx.size(), hipMemcpyHostToDevice, st));
     LSD_HIP(hipMemcpyAsync(c->d_apos, apos.data(), sizeof(int) * H, hipMemcpyHostToDevice, st));
-    LSD_HIP(hipMemcpyAsync(c->d_sorder, sorder.data(), sizeof(int) * H, hipMemcpyHostToDevice, st));
-    LSD_HIP(hipMemcpyAsync(c->d_sblocks, sblocks.data(), sizeof(int4) * sblocks.size(), hipMemcpyHostToDevice,
-                           st));
+    for (int form = 0; form < 2; form++) {
+        if (!c->d_sblk[form]) continue;
+        LSD_HIP(hipMemcpyAsync(c->d_sord[form], sorder[form].data(), sizeof(int) * H, hipMemcpyHostToDevice, st));
+        LSD_HIP(hipMemcpyAsync(c->d_sblk[form], sblocks[form].data(), sizeof(int4) * sblocks[form].size(),
+                               hipMemcpyHostToDevice, st));
+    }
     LSD_HIP(hipMemcpyAsync(c->d_slots, c->slots.data(), sizeof(Slot) * c->nslot, hipMemcpyHostToDevice, st));
     LSD_HIP(hipMemcpyAsync(c->d_slot_order, c->slot_order.data(), sizeof(int) * c->nslot, hipMemcpyHostToDevice, st));
     {
@@ -293,8 +387,11 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
             if ((rc = tables_ana4(*c))) return rc;
         }
     }
+    if (cfg->precision == LSD_MIXED_BF16X3 && (rc = tables_bf16(*c))) return rc;
+    c->ana_wide_ok = c->d_atab4 != nullptr;
     if ((rc = build_operators(*c))) return rc;
     LSD_HIP(hipStreamSynchronize(st));
+    load_tuning(*c);
     return LSD_OK;
 }
 
@@ -827,6 +924,120 @@ static int step_many(std::vector<Ctx*>& cs, int nsteps, lsd_step_stats* st) {
     return first;
 }
 
+// ------------------------------------------------------------ autotuner
+
+// Average device time in microseconds of `reps` runs of body on the
+// context's stream (after one untimed run).
+template <class F>
+static int time_us(Ctx& c, int reps, F&& body, double* out) {
+    RC(body());
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    LSD_HIP(hipEventCreate(&e0));
+    hipError_t he = hipEventCreate(&e1);
+    if (he != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        LSD_HIP(he);
+    }
+    int rc = LSD_OK;
+    if (hipEventRecord(e0, c.stream) != hipSuccess) rc = LSD_E_HIP;
+    for (int k = 0; k < reps && rc == LSD_OK; k++) rc = body();
+    float ms = 0.f;
+    if (rc == LSD_OK && (hipEventRecord(e1, c.stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                         hipEventElapsedTime(&ms, e0, e1) != hipSuccess)) {
+        set_error("lsd_tune: event timing failed");
+        rc = LSD_E_HIP;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    *out = 1e3 * (double)ms / reps;
+    return LSD_OK;
+}
+
+static int tune(Ctx& c, int reps, const char* cache, double* tus) {
+    if (c.it != 0 || c.prof_class >= 0) {
+        set_error("lsd_tune: only between timesteps and without profiling");
+        return LSD_E_ARG;
+    }
+    reps = std::max(reps, 1);
+    double t[LSD_V_COUNT][2] = {};
+    drop_graphs(c);
+    // FP32 synthesis shapes and analysis forms: the launches of one
+    // evaluation, on the current buffers (they only write derived buffers)
+    if (c.mixed) {
+        for (int k = LSD_V_SYN_SCALAR; k <= LSD_V_SYN_GRAD; k++) {
+            for (int v = 0; v < 2; v++) {
+                c.var[k] = v ? 8 : 4;
+                RC(time_us(c, reps, [&]() { return synthesise(c, true, false); }, &t[k][v]));
+            }
+            c.var[k] = t[k][0] <= t[k][1] ? 4 : 8;
+        }
+        if (c.ana_wide_ok) {
+            for (int v = 0; v < 2; v++) {
+                c.var[LSD_V_ANA] = v;
+                RC(time_us(c, reps, [&]() { return analyse(c, true); }, &t[LSD_V_ANA][v]));
+            }
+            c.var[LSD_V_ANA] = t[LSD_V_ANA][0] <= t[LSD_V_ANA][1] ? 0 : 1;
+        }
+    }
+    // solve forms: predictor + corrector on the current state, which is
+    // saved first and restored afterwards
+    if (c.solve_direct_ok && c.hb_form[1] >= 1) {
+        const size_t HN = c.coll_elems();
+        double2* save = nullptr;
+        AsyncScratch sv(c.stream);
+        LSD_HIP(sv.alloc(&save, sizeof(double2) * 12 * HN));
+        for (int k = 0; k < 6; k++) {
+            LSD_HIP(hipMemcpyAsync(save + k * HN, c.d_f[k], sizeof(double2) * HN, hipMemcpyDeviceToDevice, c.stream));
+            LSD_HIP(hipMemcpyAsync(save + (6 + k) * HN, c.d_hist[k], sizeof(double2) * HN, hipMemcpyDeviceToDevice,
+                                   c.stream));
+        }
+        for (int v = 0; v < 2; v++) {
+            c.var[LSD_V_SOLVE] = v;
+            select_solve_form(c);
+            RC(time_us(
+                c, reps,
+                [&]() {
+                    RC(launch_predictor(c));
+                    return launch_corrector_async(c);
+                },
+                &t[LSD_V_SOLVE][v]));
+        }
+        c.var[LSD_V_SOLVE] = t[LSD_V_SOLVE][0] <= t[LSD_V_SOLVE][1] ? 0 : 1;
+        select_solve_form(c);
+        for (int k = 0; k < 6; k++) {
+            LSD_HIP(hipMemcpyAsync(c.d_f[k], save + k * HN, sizeof(double2) * HN, hipMemcpyDeviceToDevice, c.stream));
+            LSD_HIP(hipMemcpyAsync(c.d_hist[k], save + (6 + k) * HN, sizeof(double2) * HN, hipMemcpyDeviceToDevice,
+                                   c.stream));
+        }
+        LSD_HIP(hipStreamSynchronize(c.stream));
+    }
+    if (tus)
+        for (int k = 0; k < LSD_V_COUNT; k++) {
+            tus[2 * k] = t[k][0];
+            tus[2 * k + 1] = t[k][1];
+        }
+    if (cache) {
+        FILE* f = std::fopen(cache, "a");
+        if (!f) {
+            set_error(std::string("lsd_tune: cannot open ") + cache);
+            return LSD_E_ARG;
+        }
+        std::fprintf(f,
+                     "{\"L\": %d, \"M\": %d, \"Mp\": %d, \"N\": %d, \"Th\": %d, \"Ph\": %d, \"precision\": %d, "
+                     "\"syn_scalar\": %d, \"syn_vector\": %d, \"syn_grad\": %d, \"ana\": %d, \"solve\": %d, "
+                     "\"us\": [%.1f, %.1f, %.1f, %.1f, %.1f, %.1f, %.1f, %.1f, %.1f, %.1f]}\n",
+                     c.L, c.M, c.Mp, c.N, c.Th, c.Ph, c.cfg.precision, c.var[0], c.var[1], c.var[2], c.var[3],
+                     c.var[4], t[0][0], t[0][1], t[1][0], t[1][1], t[2][0], t[2][1], t[3][0], t[3][1], t[4][0],
+                     t[4][1]);
+        if (std::fclose(f) != 0) {
+            set_error(std::string("lsd_tune: write failed for ") + cache);
+            return LSD_E_ARG;
+        }
+    }
+    return LSD_OK;
+}
+
 }  // namespace lsd
 
 using lsd::Ctx;
@@ -1165,6 +1376,38 @@ int lsd_profile_end(lsd_ctx* ctx, double* total_ms, int* launches, double* work)
     return LSD_OK;
 }
 
+int lsd_set_variant(lsd_ctx* ctx, int which, int value) {
+    CTX_OR_FAIL(ctx);
+    if (c.it != 0) {
+        lsd::set_error("lsd_set_variant: only between timesteps (tim_it = 0)");
+        return LSD_E_ARG;
+    }
+    if (!lsd::variant_ok(c, which, value)) {
+        lsd::set_error("lsd_set_variant: variant " + std::to_string(which) + " = " + std::to_string(value) +
+                       " does not apply to this context");
+        return LSD_E_ARG;
+    }
+    c.var[which] = value;
+    lsd::drop_graphs(c);
+    lsd::select_solve_form(c);
+    return LSD_OK;
+}
+
+int lsd_get_variant(lsd_ctx* ctx, int which, int* value) {
+    CTX_OR_FAIL(ctx);
+    if (which < 0 || which >= LSD_V_COUNT || !value) {
+        lsd::set_error("lsd_get_variant: bad variant or null output");
+        return LSD_E_ARG;
+    }
+    *value = c.var[which];
+    return LSD_OK;
+}
+
+int lsd_tune(lsd_ctx* ctx, int reps, const char* cache_path, double* times_us) {
+    CTX_OR_FAIL(ctx);
+    return lsd::tune(c, reps, cache_path, times_us);
+}
+
 int lsd_get_dt(lsd_ctx* ctx, double* dt, double* cfl_dt, double* corr_dt) {
     CTX_OR_FAIL(ctx);
     if (dt) *dt = c.dt;
@@ -1211,6 +1454,64 @@ static int download_spec(Ctx& c, const double2* src, double* re, double* im) {
 
 static int download_phys(Ctx& c, int k, double* p) {
     LSD_HIP(hipMemcpyAsync(p, lsd::phys(c, k), sizeof(double) * c.phys_elems(), hipMemcpyDeviceToHost, c.stream));
+    return LSD_OK;
+}
+
+// Synthesis -> analysis round trip at the Fourier level in the context's
+// precision (FP64, FP32 or BF16x3 Legendre kernels).
+int lsd_sht_roundtrip(lsd_ctx* ctx, int kind, const double* in, double* out) {
+    CTX_OR_FAIL(ctx);
+    if ((kind != 0 && kind != 1) || !in || !out) {
+        lsd::set_error("lsd_sht_roundtrip: kind must be 0 (scalar) or 1 (vector), non-null arrays");
+        return LSD_E_ARG;
+    }
+    const bool f = c.mixed;
+    const int nf = kind ? 3 : 1;
+    const size_t HN = c.coll_elems();
+    double2* stage = c.d_f[LSD_N_VEL_TOR];  // staging: the N fields are recomputed by the next evaluation
+    std::vector<double2> tmp(HN);
+    for (int k = 0; k < nf; k++) {
+        const double* re = in + (size_t)(2 * k) * HN;
+        const double* im = in + (size_t)(2 * k + 1) * HN;
+        for (size_t q = 0; q < HN; q++) tmp[q] = make_double2(re[q], im[q]);
+        LSD_HIP(hipMemcpyAsync(stage, tmp.data(), sizeof(double2) * HN, hipMemcpyHostToDevice, c.stream));
+        int rc = lsd::launch_cast_coll(c, stage, lsd::spec_in_p(c, k, f), f, false);
+        if (rc) return rc;
+        LSD_HIP(hipStreamSynchronize(c.stream));
+    }
+    int rc;
+    if (kind == 0) {
+        if ((rc = lsd::launch_synthesis_p(c, lsd::SYN_SCALAR, lsd::spec_in_p(c, 0, f), nullptr,
+                                          lsd::fourier_p(c, 0, f), nullptr, f)))
+            return rc;
+        if ((rc = lsd::launch_fold(c, lsd::fourier_p(c, 0, f), lsd::folded_p(c, 0, f), f))) return rc;
+        if ((rc = lsd::launch_analysis_p(c, lsd::ANA_SCALAR, lsd::folded_p(c, 0, f), nullptr, nullptr,
+                                         lsd::spec_out_p(c, 0, f), nullptr, nullptr, f)))
+            return rc;
+    } else {
+        if ((rc = lsd::launch_synthesis_p(c, lsd::SYN_SCALAR, lsd::spec_in_p(c, 0, f), nullptr,
+                                          lsd::fourier_p(c, 0, f), nullptr, f)) ||
+            (rc = lsd::launch_synthesis_p(c, lsd::SYN_VECTOR, lsd::spec_in_p(c, 1, f), lsd::spec_in_p(c, 2, f),
+                                          lsd::fourier_p(c, 1, f), lsd::fourier_p(c, 2, f), f)))
+            return rc;
+        for (int k = 0; k < 3; k++)
+            if ((rc = lsd::launch_fold(c, lsd::fourier_p(c, k, f), lsd::folded_p(c, k, f), f))) return rc;
+        if ((rc = lsd::launch_analysis_p(c, lsd::ANA_VECTOR, lsd::folded_p(c, 0, f), lsd::folded_p(c, 1, f),
+                                         lsd::folded_p(c, 2, f), lsd::spec_out_p(c, 0, f), lsd::spec_out_p(c, 1, f),
+                                         lsd::spec_out_p(c, 2, f), f)))
+            return rc;
+    }
+    for (int k = 0; k < nf; k++) {
+        if ((rc = lsd::launch_cast_coll(c, lsd::spec_out_p(c, k, f), stage, false, f))) return rc;
+        LSD_HIP(hipMemcpyAsync(tmp.data(), stage, sizeof(double2) * HN, hipMemcpyDeviceToHost, c.stream));
+        LSD_HIP(hipStreamSynchronize(c.stream));
+        double* re = out + (size_t)(2 * k) * HN;
+        double* im = out + (size_t)(2 * k + 1) * HN;
+        for (size_t q = 0; q < HN; q++) {
+            re[q] = tmp[q].x;
+            im[q] = tmp[q].y;
+        }
+    }
     return LSD_OK;
 }
 

@@ -150,6 +150,11 @@ struct Ctx {
     int Th2a = 0, Hpa = 0, ana_maxhb = 0;
     int* d_ana_base = nullptr;     // [2 nslot]
     int* d_acol = nullptr;         // [Hpa] column of the FP64 analysis tables, -1 for padding
+    // BF16x3 split tables (LSD_MIXED_BF16X3), layouts in legendre_bf16.hip
+    __bf16* d_tabb = nullptr;      // synthesis [5][class blocks of 32][theta blocks of 64][4096]
+    __bf16* d_atabb = nullptr;     // analysis [3][class blocks of 64][theta blocks of 32][4096]
+    int* d_bx_sbase = nullptr;     // [2 nslot] first synthesis block, then [2 nslot] first analysis block
+    int bx_nsb = 0, bx_nab = 0, bx_nthb = 0, bx_njb = 0, bx_maxsb = 0, bx_maxab = 0;
     // device: radial mesh
     double* d_r = nullptr;         // [7][N]  r^p, p = -3..3
     double* d_dr1 = nullptr;       // [N][7]
@@ -188,6 +193,16 @@ struct Ctx {
     double* h_partial = nullptr;       // pinned copy
     int n_partial_blocks = 0;          // solve blocks per field
     int solve_hb = 0;                  // harmonics per solve block
+
+    // kernel variants (LSD_V_*), see lsd_set_variant
+    int var[LSD_V_COUNT] = {4, 8, 8, 0, 0};
+    bool ana_wide_ok = false;          // the wide-load FP32 analysis applies (even Th2)
+    bool solve_direct_ok = false;      // the two-pass solves apply (N in {32, 48, 64, 96})
+    // solve blocks of the two forms: [0] two-pass (32 harmonics), [1] LDS-column
+    int4* d_sblk[2] = {nullptr, nullptr};
+    int* d_sord[2] = {nullptr, nullptr};
+    int nblk_form[2] = {0, 0};
+    int hb_form[2] = {0, 0};
 
     // captured step graphs: [evaluate, predictor, evaluate, corrector] and
     // [evaluate, corrector], each ending with the copy of the norm partials
@@ -291,6 +306,17 @@ int energy_noalloc(Ctx& c, int which, double* E, double* El, double* Em);
 // state_io.hip: coll = [6][2][H][N] (uT, uP, C, Comp, BT, BP; Re, Im)
 int state_read(const char* path, const lsd_config& cfg, double* coll, double* t, double* dt);
 int state_write(const char* path, const lsd_config& cfg, const double* coll, double t, double dt);
+
+// legendre_bf16.hip
+int tables_bf16(Ctx& c);
+int synb_launch(Ctx& c, int kind, int njob, const void* const* in0, const void* const* in1, void* const* out0,
+                void* const* out1);
+int anab_launch(Ctx& c, int kind, int njob, const void* const* g0, const void* const* g1, const void* const* g2,
+                void* const* o0, void* const* o1, void* const* o2);
+
+// roundtrip.hip
+int launch_cast_coll(Ctx& c, const void* src, void* dst, bool to_f32, bool from_f32);
+int launch_fold(Ctx& c, const void* fourier, void* folded, bool f32);
 
 // cfl.hip
 extern const double COURANT;

@@ -401,8 +401,11 @@ static int active_fields(const Ctx& c) {
 
 static size_t solve_lds(const Ctx& c) { return solve_lds_bytes(c); }
 
+// the two-pass kernels when they apply and are selected (LSD_V_SOLVE = 0)
+static bool use_direct(const Ctx& c) { return c.solve_direct_ok && c.var[LSD_V_SOLVE] == 0; }
+
 int launch_predictor(Ctx& c) {
-    if (solve_direct_supported(c.N)) return launch_solve_direct(c, true);
+    if (use_direct(c)) return launch_solve_direct(c, true);
     SolveArgs a = make_args(c);
     // per field: read state and N, write state and saved N
     ProfScope ps(c, LSD_K_SOLVE, 4.0 * 16.0 * c.coll_elems() * active_fields(c));
@@ -420,7 +423,7 @@ int launch_corrector_async(Ctx& c) {
     SolveArgs a = make_args(c);
     const int nb = c.n_partial_blocks;
     dim3 grid(nb, 6);
-    if (solve_direct_supported(c.N)) {
+    if (use_direct(c)) {
         int rc = launch_solve_direct(c, false);
         if (rc) return rc;
     } else {

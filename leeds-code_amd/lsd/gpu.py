@@ -27,12 +27,17 @@ EXPORTS = (
     "lsd_last_error", "lsd_config_sizes", "lsd_create", "lsd_destroy", "lsd_set_coll", "lsd_get_coll",
     "lsd_get_phys", "lsd_get_table", "lsd_get_mesh", "lsd_transform", "lsd_nonlinear", "lsd_predictor",
     "lsd_corrector", "lsd_step", "lsd_set_dt", "lsd_get_dt",
-    "lsd_read_state", "lsd_write_state", "lsd_load_state", "lsd_save_state", "lsd_tra_spec2phys", "lsd_tra_phys2spec", "lsd_tra_grad",
+    "lsd_read_state", "lsd_write_state", "lsd_load_state", "lsd_save_state",
+    "lsd_set_variant", "lsd_get_variant", "lsd_tune", "lsd_sht_roundtrip", "lsd_tra_spec2phys", "lsd_tra_phys2spec", "lsd_tra_grad",
     "lsd_tra_qst2rtp", "lsd_tra_rtp2qst", "lsd_sync", "lsd_profile_begin", "lsd_profile_end", "lsd_set_graph", "lsd_energy", "lsd_energy_noalloc",
     "lsd_step_many",
     "lsd_mlp_create", "lsd_mlp_destroy", "lsd_mlp_num_params", "lsd_mlp_get_params", "lsd_mlp_set_params",
     "lsd_mlp_forward", "lsd_mlp_grad", "lsd_mlp_train",
 )
+
+# kernel variants (lsd_set_variant)
+V_SYN_SCALAR, V_SYN_VECTOR, V_SYN_GRAD, V_ANA, V_SOLVE = range(5)
+V_NAMES = ("syn_scalar", "syn_vector", "syn_grad", "ana", "solve")
 
 # kernel classes for lsd_profile_begin
 K_SYNTH, K_ANAL, K_FFT_INV, K_FFT_FWD, K_NONLIN, K_SPEC, K_SOLVE, K_PHYS = range(8)
@@ -78,6 +83,10 @@ def load(path=LIB_PATH):
     lib.lsd_write_state.argtypes = [ctypes.c_char_p, ctypes.POINTER(LsdConfig), _dp, ctypes.c_double,
                                     ctypes.c_double]
     lib.lsd_load_state.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+    lib.lsd_set_variant.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    lib.lsd_get_variant.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    lib.lsd_tune.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, _dp]
+    lib.lsd_sht_roundtrip.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
     lib.lsd_save_state.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
     lib.lsd_get_dt.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_double)] * 3
     lib.lsd_set_graph.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -236,6 +245,36 @@ class Context:
     def save_state(self, path):
         """io_save_state of the current fields, tim_t and tim_dt."""
         self._chk(self.lib.lsd_save_state(self.h, os.fsencode(path)), "lsd_save_state")
+
+    def set_variant(self, which, value):
+        self._chk(self.lib.lsd_set_variant(self.h, which, value), "lsd_set_variant")
+
+    def get_variant(self, which):
+        v = ctypes.c_int()
+        self._chk(self.lib.lsd_get_variant(self.h, which, ctypes.byref(v)), "lsd_get_variant")
+        return v.value
+
+    def variants(self):
+        return {n: self.get_variant(k) for k, n in enumerate(V_NAMES)}
+
+    def tune(self, reps=10, cache_path=None):
+        """lsd_tune: times each variant, keeps the fastest; returns
+        {variant name: (time of variant 0 or shape 4, time of 1 or 8)} in us."""
+        t = np.zeros(2 * len(V_NAMES))
+        self._chk(self.lib.lsd_tune(self.h, reps, os.fsencode(cache_path) if cache_path else None, _p(t)),
+                  "lsd_tune")
+        return {n: (t[2 * k], t[2 * k + 1]) for k, n in enumerate(V_NAMES)}
+
+    def sht_roundtrip(self, kind, fields):
+        """lsd_sht_roundtrip: synthesis then analysis in the context's
+        precision; fields: list of (re, im) [H][N] (1 scalar or q, s, t)."""
+        nf = 3 if kind else 1
+        buf = np.zeros((nf, 2, self.H, self.N))
+        for k, (re, im) in enumerate(fields[:nf]):
+            buf[k, 0], buf[k, 1] = re, im
+        out = np.zeros_like(buf)
+        self._chk(self.lib.lsd_sht_roundtrip(self.h, kind, _p(buf), _p(out)), "lsd_sht_roundtrip")
+        return [(out[k, 0], out[k, 1]) for k in range(nf)]
 
     def get_dt(self):
         """(tim_dt, tim_cfl_dt, tim_corr_dt)"""
