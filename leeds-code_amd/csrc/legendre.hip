@@ -376,9 +376,10 @@ constexpr int S4_MG = 16;                  // m per block group (one 128-byte li
 // 8 for the vector and gradient kinds
 template <int MQ>
 constexpr int S4_NMIR = MQ == 4 ? 2 : 1;
-// operand buffers (one step of 8 harmonics each); with the 128 accumulators
-// of the vector and gradient kinds no more fit at two waves per SIMD
-constexpr int S4_NB = 4;
+// operand buffers (one step of 8 harmonics each), each refilled three steps
+// ahead: at L = 170, N = 96 three run the scalar kind in 105 us against
+// 114-117 us with four or six, and six spill the vector and gradient kinds
+constexpr int S4_NB = 3;
 template <int MQ>
 constexpr int S4_SHS = S4_TH * MQ + 2;  // LDS float2 per (hemisphere, shell), padded
 
@@ -825,7 +826,7 @@ k_ana(AnaArgs<T> a) {
 // 16 lanes write one 128-byte run of a coll row.
 constexpr int A4_SH = 16;  // shells per wave
 constexpr int A4_W = 2;    // waves (shell blocks) per workgroup
-constexpr int A4_NB = 2;   // operand buffers
+constexpr int A4_NB = 2;   // operand buffers (one or three measured 7-12 % slower)
 
 struct Ana4Args {
     int N, Th2, Th2a, M, Hpa;

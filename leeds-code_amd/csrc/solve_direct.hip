@@ -1,20 +1,23 @@
-// Predictor / corrector solves in two passes, for the radial resolutions
-// the library is built for (N fixed at compile time).
+// Predictor / corrector solves for the radial resolutions the library is
+// built for (N fixed at compile time).
 //
-// Pass 1 forms each field's right-hand side in FP64 -- Y f + N with the
-// boundary rows of the *_setbc routines (k_rhs_pred, one workgroup per
-// block of harmonics of one degree, with the Y band and the state columns
-// staged in LDS), c (N* - N) (k_rhs_corr, one work-item per point) -- and
-// saves N for the next corrector.
-//
-// Pass 2, k_sdir: one 64-lane wave per block of up to 32 harmonics of one
-// degree l (lane = harmonic, Re/Im part), so every lane needs the same
-// factors, which are staged once in LDS and read at lane-uniform
-// addresses.  Each lane loads its whole right-hand-side column straight
-// into registers -- every row's load in flight at once -- runs dgbtrs on
-// it, and writes the result back; no LDS transposition and no barrier
-// between the loads and the sweeps, so the forward sweep starts on the
-// first rows while the later ones are still arriving.
+// One 64-lane wave per block of up to 32 harmonics of one degree l (lane =
+// harmonic, Re/Im part), so every lane needs the same factors, which are
+// staged once in LDS and read at lane-uniform addresses.  Each lane loads a
+// whole column straight into registers -- every row's load in flight at
+// once -- runs dgbtrs on it, and writes the result back; no LDS
+// transposition and no barrier between the loads and the sweeps, so the
+// forward sweep starts on the first rows while the later ones are still
+// arriving.
+//  * Predictor, two passes: k_rhs_pred forms Y f + N with the boundary rows
+//    of the *_setbc routines (one workgroup per block of harmonics of one
+//    degree, with the Y band and the state columns staged in LDS) and saves
+//    N; k_spred solves.  (Forming Y f in the solve kernel would hold the
+//    state column beside the right-hand side's, past the 256 VGPRs of two
+//    waves per SIMD at N = 96.)
+//  * Corrector, one pass: k_scorr forms c (N* - N) in the register column
+//    (N* loaded, the saved N streamed and replaced) and solves, so that
+//    right-hand side never reaches HBM.
 //
 // The reference's predictor x* = X^-1 (Y x + N) and corrector
 // x += X^-1 c (N* - N), with the poloidal velocity's XGre / XPol /
@@ -46,7 +49,7 @@ struct DirArgs {
     double2* x[6];       // state
     const double2* nn[6];  // current N
     double2* hist[6];      // saved N
-    double2* rhs[6];     // [H][N] right-hand sides
+    double2* rhs[6];     // [H][N] predictor right-hand sides
     double2* rhs2;       // poloidal velocity: the Y x part of the XPol right-hand side
     double* partial;     // [8][gridDim.x]
 };
@@ -79,29 +82,6 @@ __device__ __forceinline__ double2 band_acc(double2 acc, const double* __restric
         acc.y = fma(a, xj.y, acc.y);
     }
     return acc;
-}
-
-// Corrector right-hand sides d = c (N* - N), with the saved N advanced;
-// one work-item per (harmonic, radial point), coalesced.
-__global__ void __launch_bounds__(256) k_rhs_corr(DirArgs a) {
-    const int f = blockIdx.y;
-    if (!a.on[f]) return;
-    const int N = a.N;
-    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (idx >= (size_t)a.H * N) return;
-    const int h = (int)(idx / N), n = (int)(idx - (size_t)h * N);
-    const int l = a.hl[h];
-    // boundary rows as a factor, not a select between double2 values
-    // (which the compiler would route through scratch memory)
-    const double keep = (n == 0 || n == N - 1) ? 0.0 : 1.0;
-    const double2 nv = pick(a.nn, f)[idx];
-    double2* hist = pick(a.hist, f);
-    const double2 hv = hist[idx];
-    hist[idx] = nv;
-    double2 d = make_double2(keep * (a.cimp * (nv.x - hv.x)), keep * (a.cimp * (nv.y - hv.y)));
-    // vel_setbc_Tor for (1,0) once tim_it /= 0
-    if (f == 0 && l == 1 && a.hm[h] == 0 && n == 0) d.x = 0.0 - pick(a.x, f)[(size_t)h * N].x;
-    pick(a.rhs, f)[idx] = d;
 }
 
 // Predictor right-hand sides, one workgroup per solve block (harmonics of one
@@ -285,17 +265,16 @@ __device__ __forceinline__ const unsigned* pivots(const DirArgs& a, int kind, in
 // VGPRs of two waves per SIMD at N = 96.
 constexpr int DIR_BATCH = 16;
 
-// At most 256 VGPRs, so two waves share each SIMD.
-template <int N, bool PRED>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_sdir(DirArgs a) {
+// Predictor solves: x* = X^-1 b for the right-hand sides of k_rhs_pred.  At
+// most 256 VGPRs, so two waves share each SIMD.
+template <int N>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_spred(DirArgs a) {
     static_assert(N % DIR_BATCH == 0, "N must be a multiple of the batch");
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    __shared__ double red[2][64];
     DirSmem<N>& s = *reinterpret_cast<DirSmem<N>*>(smem_raw);
     constexpr int BT = DIR_BATCH;
     const int f = blockIdx.y;
     const int t = threadIdx.x;
-    double e2 = 0.0, eic = 0.0;
     if (a.on[f]) {
         const int4 blk = a.blocks[blockIdx.x];
         const int l = blk.x, first = blk.y, nh = blk.z;
@@ -318,69 +297,132 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
         if (solve) gbtrs_reg<N, double>(b, s.ab, s.rd, pivots(a, lu_kind(f), l));
         double* xs = reinterpret_cast<double*>(pick(a.x, f) + (size_t)h * N) + part;
         if (f == 1 && solve) {
-            if constexpr (PRED) {
-                // XPol right-hand side: XGre^-1 N + Y x inside, zero on the
-                // boundary rows (where the second part is zero)
-                const double* src2 = reinterpret_cast<const double*>(a.rhs2 + (size_t)h * N) + part;
+            // XPol right-hand side: XGre^-1 N + Y x inside, zero on the
+            // boundary rows (where the second part is zero)
+            const double* src2 = reinterpret_cast<const double*>(a.rhs2 + (size_t)h * N) + part;
 #pragma unroll
-                for (int c = 0; c < N / BT; c++) {
-                    double r2[BT];
+            for (int c = 0; c < N / BT; c++) {
+                double r2[BT];
 #pragma unroll
-                    for (int r = 0; r < BT; r++) r2[r] = src2[2 * (c * BT + r)];
+                for (int r = 0; r < BT; r++) r2[r] = src2[2 * (c * BT + r)];
 #pragma unroll
-                    for (int r = 0; r < BT; r++) {
-                        const int n = c * BT + r;
-                        b[n] = (n == 0 || n == N - 1) ? r2[r] : b[n] + r2[r];
-                        pin(b[n]);
-                    }
-                    BATCH_FENCE();
+                for (int r = 0; r < BT; r++) {
+                    const int n = c * BT + r;
+                    b[n] = (n == 0 || n == N - 1) ? r2[r] : b[n] + r2[r];
+                    pin(b[n]);
                 }
-            } else {
-                b[0] = 0.0;
-                b[N - 1] = 0.0;
+                BATCH_FENCE();
             }
             __syncthreads();
             stage_factors<N>(a, s, 1, l);
             __syncthreads();
             gbtrs_reg<N, double>(b, s.ab, s.rd, pivots(a, 1, l));
-            // vel_gre_adjust (velocity.F90:445-465), on x* (predictor) or on
-            // the correction (corrector)
+            // vel_gre_adjust (velocity.F90:445-465) on x*
             const double* ig = a.invg + l * 4;
             const double bi = b[0], bo = b[N - 1];
             const double ca = ig[0] * bi + ig[2] * bo;
             const double cb = ig[1] * bi + ig[3] * bo;
-            if constexpr (PRED) {
 #pragma unroll
-                for (int c = 0; c < N / BT; c++) {
+            for (int c = 0; c < N / BT; c++) {
 #pragma unroll
-                    for (int r = 0; r < BT; r++) {
-                        const int n = c * BT + r;
-                        if (act) xs[2 * n] = b[n] - ca * s.g[n] - cb * s.g[N + n];
-                    }
-                    BATCH_FENCE();
+                for (int r = 0; r < BT; r++) {
+                    const int n = c * BT + r;
+                    if (act) xs[2 * n] = b[n] - ca * s.g[n] - cb * s.g[N + n];
                 }
-            } else {
-#pragma unroll
-                for (int c = 0; c < N / BT; c++) {
-                    double xo[BT];
-#pragma unroll
-                    for (int r = 0; r < BT; r++) xo[r] = xs[2 * (c * BT + r)];
-#pragma unroll
-                    for (int r = 0; r < BT; r++) {
-                        const int n = c * BT + r;
-                        const double d = b[n] - ca * s.g[n] - cb * s.g[N + n];
-                        if (act) {
-                            xs[2 * n] = xo[r] + d;
-                            e2 = e2 + d * d;
-                        }
-                    }
-                    BATCH_FENCE();
-                }
+                BATCH_FENCE();
             }
-        } else if constexpr (PRED) {
+        } else {
 #pragma unroll
             for (int n = 0; n < N; n++)
                 if (act) xs[2 * n] = b[n];
+        }
+    }
+}
+
+// Corrector: b = c (N* - N) formed in registers (N* in the column, the saved
+// N streamed and replaced), x += X^-1 b, with the tim_addcorr norms; one
+// wave per (solve block, field).
+template <int N>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_scorr(DirArgs a) {
+    static_assert(N % DIR_BATCH == 0, "N must be a multiple of the batch");
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ double red[2][64];
+    DirSmem<N>& s = *reinterpret_cast<DirSmem<N>*>(smem_raw);
+    constexpr int BT = DIR_BATCH;
+    const int f = blockIdx.y;
+    const int t = threadIdx.x;
+    double e2 = 0.0, eic = 0.0;
+    if (a.on[f]) {
+        const int4 blk = a.blocks[blockIdx.x];
+        const int l = blk.x, first = blk.y, nh = blk.z;
+        const int hh = t >> 1, part = t & 1;
+        const bool act = hh < nh;
+        const int h = a.order[first + (act ? hh : 0)];
+        // X is solved except for the l = 0 modes of the velocity and the
+        // magnetic field
+        const bool solve = f == 2 || f == 3 || l != 0;
+        const double* xsrc = reinterpret_cast<const double*>(pick(a.x, f) + (size_t)h * N) + part;
+        const double* nsrc = reinterpret_cast<const double*>(pick(a.nn, f) + (size_t)h * N) + part;
+        double* hsto = reinterpret_cast<double*>(pick(a.hist, f) + (size_t)h * N) + part;
+        // N*, every row at once, issued ahead of the factor staging
+        double b[N];
+#pragma unroll
+        for (int n = 0; n < N; n++) b[n] = nsrc[2 * n];
+        if (solve) {
+            stage_factors<N>(a, s, lu_kind(f), l);
+            if (f == 1)
+                for (int i = t; i < 2 * N; i += 64) s.g[i] = a.gre[(size_t)l * 2 * N + i];
+        }
+        __syncthreads();
+        // b = c (N* - N) inside, zero on the boundary rows; N* saved
+        const double cimp = a.cimp;
+#pragma unroll
+        for (int c = 0; c < N / BT; c++) {
+            double hv[BT];
+#pragma unroll
+            for (int r = 0; r < BT; r++) hv[r] = hsto[2 * (c * BT + r)];
+#pragma unroll
+            for (int r = 0; r < BT; r++) {
+                const int n = c * BT + r;
+                if (act) hsto[2 * n] = b[n];
+                const double keep = (n == 0 || n == N - 1) ? 0.0 : 1.0;
+                b[n] = keep * (cimp * (b[n] - hv[r]));
+                pin(b[n]);
+            }
+            BATCH_FENCE();
+        }
+        // vel_setbc_Tor for (1, 0) once tim_it /= 0
+        if (f == 0 && l == 1 && a.hm[h] == 0 && part == 0) b[0] = 0.0 - xsrc[0];
+        if (solve) gbtrs_reg<N, double>(b, s.ab, s.rd, pivots(a, lu_kind(f), l));
+        double* xs = reinterpret_cast<double*>(pick(a.x, f) + (size_t)h * N) + part;
+        if (f == 1 && solve) {
+            b[0] = 0.0;
+            b[N - 1] = 0.0;
+            __syncthreads();
+            stage_factors<N>(a, s, 1, l);
+            __syncthreads();
+            gbtrs_reg<N, double>(b, s.ab, s.rd, pivots(a, 1, l));
+            // vel_gre_adjust (velocity.F90:445-465) on the correction
+            const double* ig = a.invg + l * 4;
+            const double bi = b[0], bo = b[N - 1];
+            const double ca = ig[0] * bi + ig[2] * bo;
+            const double cb = ig[1] * bi + ig[3] * bo;
+#pragma unroll
+            for (int c = 0; c < N / BT; c++) {
+                double xo[BT];
+#pragma unroll
+                for (int r = 0; r < BT; r++) xo[r] = xs[2 * (c * BT + r)];
+#pragma unroll
+                for (int r = 0; r < BT; r++) {
+                    const int n = c * BT + r;
+                    const double d = b[n] - ca * s.g[n] - cb * s.g[N + n];
+                    if (act) {
+                        xs[2 * n] = xo[r] + d;
+                        e2 = e2 + d * d;
+                    }
+                }
+                BATCH_FENCE();
+            }
         } else {
             // x += d, with the correction norms of tim_addcorr
 #pragma unroll
@@ -402,20 +444,18 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
             }
         }
     }
-    if constexpr (!PRED) {
-        red[0][t] = e2;
-        red[1][t] = eic;
-        __syncthreads();
-        if (t == 0) {
-            double sum = 0.0, si = 0.0;
-            for (int k = 0; k < 64; k++) {
-                sum += red[0][k];
-                si += red[1][k];
-            }
-            const int slot = f < 4 ? f : (f == 4 ? 4 : 6);
-            a.partial[(size_t)slot * gridDim.x + blockIdx.x] = sum;
-            if (f >= 4) a.partial[(size_t)(slot + 1) * gridDim.x + blockIdx.x] = si;
+    red[0][t] = e2;
+    red[1][t] = eic;
+    __syncthreads();
+    if (t == 0) {
+        double sum = 0.0, si = 0.0;
+        for (int k = 0; k < 64; k++) {
+            sum += red[0][k];
+            si += red[1][k];
         }
+        const int slot = f < 4 ? f : (f == 4 ? 4 : 6);
+        a.partial[(size_t)slot * gridDim.x + blockIdx.x] = sum;
+        if (f >= 4) a.partial[(size_t)(slot + 1) * gridDim.x + blockIdx.x] = si;
     }
 }
 
@@ -458,7 +498,9 @@ static DirArgs dir_args(Ctx& c) {
 
 template <int N, bool PRED>
 static void launch_sdir(Ctx& c, const DirArgs& a) {
-    hipLaunchKernelGGL((k_sdir<N, PRED>), dim3(c.n_partial_blocks, 6), dim3(64), sizeof(DirSmem<N>), c.stream, a);
+    const dim3 grid(c.n_partial_blocks, 6);
+    if (PRED) hipLaunchKernelGGL(k_spred<N>, grid, dim3(64), sizeof(DirSmem<N>), c.stream, a);
+    else hipLaunchKernelGGL(k_scorr<N>, grid, dim3(64), sizeof(DirSmem<N>), c.stream, a);
 }
 
 template <bool PRED>
@@ -474,24 +516,21 @@ static int launch_sdir_n(Ctx& c, const DirArgs& a) {
     return LSD_OK;
 }
 
-// Both passes of the predictor (pred) or the corrector on the context's
-// stream.  Work (bytes) per field and coll element: pass 1 reads N and the
-// state (predictor) or N and the saved N (corrector) and writes the saved
-// N and the right-hand side; pass 2 reads the right-hand side and writes
-// the state, reading it too in the corrector.
+// The predictor (pred: both passes) or the corrector on the context's
+// stream.  Work (bytes) per field and coll element: the predictor reads the
+// state and N, writes the saved N and the right-hand side, reads that and
+// writes the state; the corrector reads N, the saved N and the state and
+// writes the saved N and the state.
 int launch_solve_direct(Ctx& c, bool pred) {
     const DirArgs a = dir_args(c);
     const int nf = 2 + (c.cfg.cod ? 1 : 0) + (c.cfg.comp ? 1 : 0) + (c.cfg.mag ? 2 : 0);
-    ProfScope ps(c, LSD_K_SOLVE, (4.0 * 16.0 + (pred ? 2.0 : 3.0) * 16.0) * c.coll_elems() * nf);
+    ProfScope ps(c, LSD_K_SOLVE, (pred ? 6.0 : 5.0) * 16.0 * c.coll_elems() * nf);
     if (pred) {
         const size_t lds = sizeof(double2) * (size_t)c.solve_hb * c.N + sizeof(double) * 7 * (size_t)c.N;
         hipLaunchKernelGGL(k_rhs_pred, dim3(c.n_partial_blocks, 6), dim3(RHS_BLK), lds, c.stream, a,
                            1.0f / (float)c.N);
-    } else {
-        const dim3 g1((unsigned)((c.coll_elems() + 255) / 256), 6);
-        hipLaunchKernelGGL(k_rhs_corr, g1, dim3(256), 0, c.stream, a);
+        LSD_HIP(hipGetLastError());
     }
-    LSD_HIP(hipGetLastError());
     return pred ? launch_sdir_n<true>(c, a) : launch_sdir_n<false>(c, a);
 }
 
