@@ -287,8 +287,9 @@ int lsd_sync(lsd_ctx* ctx);
  * come from the tuning cache (lsd_tune) when one is found, else the values
  * given here.  Every variant gives the same results, bit for bit for the
  * synthesis shapes and to rounding for the analysis and solve forms.
- *  SYN_*     FP32 synthesis: m per workgroup, 4 (both hemispheres staged
- *            per pass) or 8 (one hemisphere per pass); defaults 4, 8, 8
+ *  SYN_*     FP32 synthesis: m per workgroup, 4 or 8 (one wave per m, one
+ *            hemisphere of one output staged in LDS per pass); defaults
+ *            4, 8, 8
  *  ANA       FP32 analysis: 0 wide-load (even Th/2 only; default), 1 LDS-staged
  *  SOLVE     banded solves: 0 two-pass register-column FP64 (N in {32, 48,
  *            64, 96}; default there), 1 LDS-column (the mixed forms: FP32

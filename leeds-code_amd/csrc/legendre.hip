@@ -370,18 +370,22 @@ int launch_synthesis(Ctx& c, int kind, const double2* in0, const double2* in1, d
 constexpr int S4_TH = 64;                  // theta points per wave tile
 constexpr int S4_SH = 16;                  // shells per wave tile
 constexpr int S4_MG = 16;                  // m per block group (one 128-byte line of a Fourier row)
-// consecutive m per workgroup (one per wave) and hemispheres staged in LDS
-// per pass: 4 m and both hemispheres, or 8 m and one (66 KB of LDS either
-// way); a kernel variant (LSD_V_SYN_*), by default 4 for the scalar kind and
-// 8 for the vector and gradient kinds
+// consecutive m per workgroup (one per wave), 4 or 8: a kernel variant
+// (LSD_V_SYN_*); one hemisphere of one output is staged in LDS per pass
 template <int MQ>
-constexpr int S4_NMIR = MQ == 4 ? 2 : 1;
+constexpr int S4_NMIR = 1;
 // operand buffers (one step of 8 harmonics each), each refilled three steps
 // ahead: at L = 170, N = 96 three run the scalar kind in 105 us against
 // 114-117 us with four or six, and six spill the vector and gradient kinds
 constexpr int S4_NB = 3;
+// LDS staging row of one shell: 64 theta x (MQ + 1) float2, the m of one
+// theta padded by one slot, so the 16 theta of a lane group's writes (a
+// stride of 2 MQ + 2 banks) land in 16 distinct bank pairs; plus one slot
+// between shells
 template <int MQ>
-constexpr int S4_SHS = S4_TH * MQ + 2;  // LDS float2 per (hemisphere, shell), padded
+constexpr int S4_MQP = MQ + 1;
+template <int MQ>
+constexpr int S4_SHS = S4_TH * S4_MQP<MQ> + 1;
 
 struct Syn4Args {
     int N, Th, Th2, Th2p, M, MP, H;
@@ -536,7 +540,7 @@ k_syn4(Syn4Args a) {
                 const float y0i = Y[o][0][1][t][r], y1i = Y[o][1][1][t][r];
 #pragma unroll
                 for (int mm = 0; mm < NMIR; mm++)
-                    St[(mm * S4_SH + sh) * SHS + th * MQ + wave] =
+                    St[(mm * S4_SH + sh) * SHS + th * S4_MQP<MQ> + wave] =
                         (mir0 + mm) ? make_float2(y0r - y1r, y0i - y1i) : make_float2(y0r + y1r, y0i + y1i);
             }
         __syncthreads();
@@ -549,7 +553,7 @@ k_syn4(Syn4Args a) {
             if (n >= a.N || j >= a.Th2) continue;
             const int jrow = mir ? a.Th - 1 - j : j;
             if (mir && jrow < a.Th2) continue;
-            const float2* sp = St + (mm * S4_SH + sh) * SHS + th * MQ + 2 * qq;
+            const float2* sp = St + (mm * S4_SH + sh) * SHS + th * S4_MQP<MQ> + 2 * qq;
             float4 v = make_float4(sp[0].x, sp[0].y, sp[1].x, sp[1].y);
             if constexpr (KIND == 2) {
                 const float f = (float)a.rinv[n];

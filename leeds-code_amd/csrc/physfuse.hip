@@ -101,8 +101,10 @@ __device__ __forceinline__ void load_batch(Batch<V>& bt, const V* __restrict__ f
 
 // Packs a loaded batch into Hermitian-extended complex sequences (row jp
 // real part, row jm imaginary part); the DC imaginary parts are dropped as a
-// c2r transform does.  The caller synchronises before the transform.
-template <int NT, typename V>
+// c2r transform does.  Unless the first transform stage skips the zero
+// band k = M .. Ph-M (PRUNED), it is written with zeros.  The caller
+// synchronises before the transform.
+template <int NT, bool PRUNED, typename V>
 __device__ __forceinline__ void pack_batch(V* Z, const Batch<V>& bt, int nb, int M, int Ph, float inv_m,
                                            float inv_w) {
 #pragma unroll
@@ -120,10 +122,12 @@ __device__ __forceinline__ void pack_batch(V* Z, const Batch<V>& bt, int nb, int
             }
         }
     }
-    const int W = Ph - 2 * M + 1;  // zero band k = M .. Ph-M
-    for (int idx = threadIdx.x; idx < nb * W; idx += NT) {
-        const int b = quot(idx, inv_w), k = M + (idx - b * W);
-        Z[lidx<true>(b * Ph + k)] = cx<V>(0, 0);
+    if constexpr (!PRUNED) {
+        const int W = Ph - 2 * M + 1;  // zero band k = M .. Ph-M
+        for (int idx = threadIdx.x; idx < nb * W; idx += NT) {
+            const int b = quot(idx, inv_w), k = M + (idx - b * W);
+            Z[lidx<true>(b * Ph + k)] = cx<V>(0, 0);
+        }
     }
 }
 
@@ -192,6 +196,104 @@ __device__ __forceinline__ void stage512(V* Z, int nf, const V* tw) {
     __syncthreads();
 }
 
+// The first stage of the inverse length-512 transform when 128 < M <= 192,
+// without reading the zero band k = M .. 512-M: input k = j + 64 r is zero
+// for r = 3, 4, for r = 2 when j >= M - 128 and for r = 5 when j <= 192 - M,
+// so the band need not be written and the butterfly drops the terms of
+// inputs 3 and 4.  Same results as stage512<1, 1> on a zero-filled band
+// (up to the sign of exact zeros).
+template <typename V>
+__device__ __forceinline__ void stage512_inv_first(V* Z, int nf, int M) {
+    typedef decltype(V().x) S;
+    const int b = threadIdx.x;
+    const bool on = b < nf * 64;
+    const int p = b >> 6, j = b & 63;
+    V v[8];
+    if (on) {
+        const int rb = lidx<true>(p * 512 + j);
+        const V zero = cx<V>((S)0, (S)0);
+        const V x0 = Z[rb], x1 = Z[rb + 72], x2r = Z[rb + 144], x5r = Z[rb + 360], x6 = Z[rb + 432],
+                x7 = Z[rb + 504];
+        const V x2 = j < M - 128 ? x2r : zero, x5 = j > 192 - M ? x5r : zero;
+        // dft<8> with inputs 3 and 4 zero: even part (x0, x2, 0, x6), odd
+        // part (x1, 0, x5, x7); w = exp(2 pi i / 8)
+        const S c = (S)0.70710678118654752440;
+        const V t2 = caddv(x2, x6), t3 = mul_si(csubv(x2, x6), (S)1);
+        const V e0 = caddv(x0, t2), e2 = csubv(x0, t2), e1 = caddv(x0, t3), e3 = csubv(x0, t3);
+        const V u0 = caddv(x1, x5), u1 = csubv(x1, x5), u3 = mul_si(cx<V>(-x7.x, -x7.y), (S)1);
+        const V o0 = caddv(u0, x7), o2 = csubv(u0, x7), o1 = caddv(u1, u3), o3 = csubv(u1, u3);
+        const V w1 = cx<V>(c * (o1.x - o1.y), c * (o1.y + o1.x));
+        const V w2 = mul_si(o2, (S)1);
+        const V w3 = cx<V>(c * (-o3.x - o3.y), c * (-o3.y + o3.x));
+        v[0] = caddv(e0, o0);
+        v[4] = csubv(e0, o0);
+        v[1] = caddv(e1, w1);
+        v[5] = csubv(e1, w1);
+        v[2] = caddv(e2, w2);
+        v[6] = csubv(e2, w2);
+        v[3] = caddv(e3, w3);
+        v[7] = csubv(e3, w3);
+    }
+    __syncthreads();
+    if (on) {
+        const int wb = lidx<true>(p * 512 + j * 8);
+#pragma unroll
+        for (int r = 0; r < 8; r++) Z[wb + r] = v[r];
+    }
+    __syncthreads();
+}
+
+// The last stage of the forward length-512 transform when M <= 192: outputs
+// k = jm + 64 r for r = 3, 4 (k = 192 .. 319) are never read, since only
+// k < M and k > 512 - M are gathered, so they are neither formed nor
+// written.
+template <typename V>
+__device__ __forceinline__ void stage512_fwd_last(V* Z, int nf, const V* tw) {
+    typedef decltype(V().x) S;
+    constexpr S sg = (S)-1;
+    const int b = threadIdx.x;
+    const bool on = b < nf * 64;
+    const int p = b >> 6, j = b & 63;
+    V v[8];
+    if (on) {
+        const int rb = lidx<true>(p * 512 + j);
+#pragma unroll
+        for (int r = 0; r < 8; r++) v[r] = Z[rb + 72 * r];
+        const V* t = tw + 56 + j;
+#pragma unroll
+        for (int r = 1; r < 8; r++) {
+            V w = t[(r - 1) * 64];
+            w.y *= sg;
+            v[r] = cmulv(v[r], w);
+        }
+        const S c = (S)0.70710678118654752440;
+        V e[4] = {v[0], v[2], v[4], v[6]};
+        V o[4] = {v[1], v[3], v[5], v[7]};
+        dft<4>(e, sg);
+        dft<4>(o, sg);
+        const V w1 = cx<V>(c * (o[1].x - sg * o[1].y), c * (o[1].y + sg * o[1].x));
+        const V w2 = mul_si(o[2], sg);
+        const V w3 = cx<V>(c * (-o[3].x - sg * o[3].y), c * (-o[3].y + sg * o[3].x));
+        v[0] = caddv(e[0], o[0]);
+        v[1] = caddv(e[1], w1);
+        v[5] = csubv(e[1], w1);
+        v[2] = caddv(e[2], w2);
+        v[6] = csubv(e[2], w2);
+        v[7] = csubv(e[3], w3);
+    }
+    __syncthreads();
+    if (on) {
+        const int wb = lidx<true>(p * 512 + j);
+        Z[wb] = v[0];
+        Z[wb + 72] = v[1];
+        Z[wb + 144] = v[2];
+        Z[wb + 360] = v[5];
+        Z[wb + 432] = v[6];
+        Z[wb + 504] = v[7];
+    }
+    __syncthreads();
+}
+
 // Twiddle tables of the two twiddled stages of a length-512 transform,
 // from the full table exp(2 pi i k / 512): stage NS = 8 at [0, 56),
 // entry (r-1)*8 + jm = w^(8 jm r); stage NS = 64 at [56, 504), entry
@@ -211,12 +313,16 @@ __device__ void stage_twiddles(V* twl, const V* __restrict__ tw) {
     }
 }
 
-template <int SG, bool R512, int NT, typename V>
-__device__ __forceinline__ void fft_pf(V* Z, const FftPlan& pl, int nf, const V* tw) {
+// PRUNED (length 512 with 128 < M <= 192): the inverse transform skips the
+// zero band of its input, the forward one the outputs that are not gathered.
+template <int SG, bool R512, bool PRUNED, int NT, typename V>
+__device__ __forceinline__ void fft_pf(V* Z, const FftPlan& pl, int nf, const V* tw, int M) {
     if constexpr (R512) {
-        stage512<1, SG>(Z, nf, tw);
+        if constexpr (PRUNED && SG == 1) stage512_inv_first(Z, nf, M);
+        else stage512<1, SG>(Z, nf, tw);
         stage512<8, SG>(Z, nf, tw);
-        stage512<64, SG>(Z, nf, tw);
+        if constexpr (PRUNED && SG == -1) stage512_fwd_last(Z, nf, tw);
+        else stage512<64, SG>(Z, nf, tw);
     } else {
         fft_lds<V, NT, PF_EPT, true>(Z, pl, nf, tw, (decltype(V().x))SG);
     }
@@ -240,10 +346,12 @@ size_t physfuse_lds(int Ph, bool r512) {
 }
 
 // CFL: also form the CFL partial maxima (a separate instantiation, so the
-// usual pass carries no registers for them).
-template <typename R, bool R512, int NT, bool CFL>
+// usual pass carries no registers for them).  PRUNED (length 512 with
+// 128 < M <= 192): the transforms skip the zero band (fft_pf).
+template <typename R, bool R512, int NT, bool CFL, bool PRUNED>
 __global__ void __launch_bounds__(NT) k_physfuse(PhysArgs<R> a) {
     static_assert(!R512 || NT == PF_THREADS, "length-512 stages take one butterfly per thread of 512");
+    static_assert(R512 || !PRUNED, "only the length-512 stages are pruned");
     constexpr int PP = 1;  // phi points per thread
     typedef typename Cx<R>::t V;
     extern __shared__ unsigned char lds_raw[];
@@ -272,7 +380,7 @@ __global__ void __launch_bounds__(NT) k_physfuse(PhysArgs<R> a) {
         load_batch<NT>(inb, a.fourier, gf0, LSD_P_COMP_GR, ng, sh, N, M, a.MP, Th, jp, jm, a.inv_m);
     };
     auto pack = [&](int f0, int f1, int nb) {
-        if constexpr (AHEAD) pack_batch<NT>(Z, inb, nb, M, Ph, a.inv_m, a.inv_w);
+        if constexpr (AHEAD) pack_batch<NT, PRUNED>(Z, inb, nb, M, Ph, a.inv_m, a.inv_w);
         else pack_direct<NT>(Z, a.fourier, f0, f1, nb, sh, N, M, a.MP, Th, Ph, jp, jm, a.inv_m, a.inv_w);
     };
     if constexpr (AHEAD)
@@ -298,7 +406,7 @@ __global__ void __launch_bounds__(NT) k_physfuse(PhysArgs<R> a) {
             const int nb = a.mag ? 6 : ng;
             load_batch<NT>(inb, a.fourier, f0, f1, nb, sh, N, M, a.MP, Th, jp, jm, a.inv_m);
         }
-        fft_pf<1, R512, NT>(Z, a.pl, 6, twl);
+        fft_pf<1, R512, PRUNED, NT>(Z, a.pl, 6, twl, M);
 #pragma unroll
         for (int p = 0; p < PP; p++) {
             const int x = t + p * NT;
@@ -339,7 +447,7 @@ __global__ void __launch_bounds__(NT) k_physfuse(PhysArgs<R> a) {
         if constexpr (AHEAD) {
             if (ng) load_grad();
         }
-        fft_pf<1, R512, NT>(Z, a.pl, 6, twl);
+        fft_pf<1, R512, PRUNED, NT>(Z, a.pl, 6, twl, M);
 #pragma unroll
         for (int p = 0; p < PP; p++) {
             const int x = t + p * NT;
@@ -393,7 +501,7 @@ __global__ void __launch_bounds__(NT) k_physfuse(PhysArgs<R> a) {
     if (ng) {
         pack(gf0, LSD_P_COMP_GR, ng);
         __syncthreads();
-        fft_pf<1, R512, NT>(Z, a.pl, ng, twl);
+        fft_pf<1, R512, PRUNED, NT>(Z, a.pl, ng, twl, M);
         const int oc = 0, ox = a.cod ? 3 : 0;
 #pragma unroll
         for (int p = 0; p < PP; p++) {
@@ -438,7 +546,7 @@ __global__ void __launch_bounds__(NT) k_physfuse(PhysArgs<R> a) {
         }
     }
     __syncthreads();
-    fft_pf<-1, R512, NT>(Z, a.pl, nout, twl);
+    fft_pf<-1, R512, PRUNED, NT>(Z, a.pl, nout, twl, M);
     // separate the two rows and fold: G0 = X_j + X_mirror, G1 = X_j - X_mirror
     const size_t fstride = (size_t)N * M * 2 * a.Th2;
     for (int idx = t; idx < nout * M * 2; idx += NT) {
@@ -504,21 +612,25 @@ static int physfuse_launch(Ctx& c, const void* fourier, void* folded, const void
     // algorithmic bytes: Fourier fields in, folded fields out
     ProfScope ps(c, LSD_K_PHYS,
                  (double)sizeof(V) * c.N * c.M * ((double)nin * c.Th + (double)nout * 2 * c.Th2));
+    const bool pruned = r512 && c.M > 128 && c.M <= 192;
+    const dim3 g(nblk);
     if (c.cfl_on) {
-        if (r512)
-            hipLaunchKernelGGL((k_physfuse<R, true, PF_THREADS, true>), dim3(nblk), dim3(PF_THREADS), lds, c.stream, a);
+        if (pruned) hipLaunchKernelGGL((k_physfuse<R, true, PF_THREADS, true, true>), g, dim3(PF_THREADS), lds, c.stream, a);
+        else if (r512)
+            hipLaunchKernelGGL((k_physfuse<R, true, PF_THREADS, true, false>), g, dim3(PF_THREADS), lds, c.stream, a);
         else if (c.Ph <= PF_THREADS)
-            hipLaunchKernelGGL((k_physfuse<R, false, PF_THREADS, true>), dim3(nblk), dim3(PF_THREADS), lds, c.stream,
-                               a);
+            hipLaunchKernelGGL((k_physfuse<R, false, PF_THREADS, true, false>), g, dim3(PF_THREADS), lds, c.stream, a);
         else
-            hipLaunchKernelGGL((k_physfuse<R, false, PF_THREADS_MAX, true>), dim3(nblk), dim3(PF_THREADS_MAX), lds,
+            hipLaunchKernelGGL((k_physfuse<R, false, PF_THREADS_MAX, true, false>), g, dim3(PF_THREADS_MAX), lds,
                                c.stream, a);
+    } else if (pruned) {
+        hipLaunchKernelGGL((k_physfuse<R, true, PF_THREADS, false, true>), g, dim3(PF_THREADS), lds, c.stream, a);
     } else if (r512) {
-        hipLaunchKernelGGL((k_physfuse<R, true, PF_THREADS, false>), dim3(nblk), dim3(PF_THREADS), lds, c.stream, a);
+        hipLaunchKernelGGL((k_physfuse<R, true, PF_THREADS, false, false>), g, dim3(PF_THREADS), lds, c.stream, a);
     } else if (c.Ph <= PF_THREADS) {
-        hipLaunchKernelGGL((k_physfuse<R, false, PF_THREADS, false>), dim3(nblk), dim3(PF_THREADS), lds, c.stream, a);
+        hipLaunchKernelGGL((k_physfuse<R, false, PF_THREADS, false, false>), g, dim3(PF_THREADS), lds, c.stream, a);
     } else {
-        hipLaunchKernelGGL((k_physfuse<R, false, PF_THREADS_MAX, false>), dim3(nblk), dim3(PF_THREADS_MAX), lds,
+        hipLaunchKernelGGL((k_physfuse<R, false, PF_THREADS_MAX, false, false>), g, dim3(PF_THREADS_MAX), lds,
                            c.stream, a);
     }
     LSD_HIP(hipGetLastError());
