@@ -17,7 +17,9 @@
                                 LSD_MAG_TOR = 4, LSD_MAG_POL = 5, LSD_N_VEL_TOR = 6,          &
                                 LSD_N_VEL_POL = 7, LSD_N_COD = 8, LSD_N_COMP = 9,             &
                                 LSD_N_MAG_TOR = 10, LSD_N_MAG_POL = 11
-   integer(c_int), parameter :: LSD_T_GP = 0, LSD_T_GW = 1
+   integer(c_int), parameter :: LSD_T_GP = 0, LSD_T_GW = 1, LSD_T_P = 2, LSD_T_P_ = 3, LSD_T_SIN1MP = 4, &
+                                LSD_T_LLP_ = 5, LSD_T_LLSIN1MP = 6, LSD_T_GWP = 7, LSD_T_GWLLP_ = 8,   &
+                                LSD_T_GWLLSIN1MP = 9
    integer(c_int), parameter :: LSD_M_R = 0, LSD_M_INTR2DR = 5
 
    !> lsd_config: run parameters (Scripts/parameters.F90_template names)

@@ -4,14 +4,11 @@
 !! for the serial build (_Nth = _Nr = 1): the same public procedures with
 !! the same dummy arguments, the transforms running on the MI355X through
 !! the C-ABI (module lsd_c).  The reference's modules parameters and
-!! variables provide the sizes and the spec/phys types; the context is
-!! created by tra_precompute and kept for the life of the program.
+!! variables provide the sizes and the spec/phys types; the library context
+!! is the one shared with the drop-in legendre module (module lsd_dropin).
 !*************************************************************************
  module transform
-   use parameters, only : i_L, i_M, i_Mp, i_N, i_Th, i_Ph, d_rratio, d_Ra, d_Ra_comp, d_Pr, d_Sc, &
-                          d_Pm, d_E, d_timestep, d_implicit, d_dterr, d_source, d_source_comp,      &
-                          b_cod_tstep, b_comp_tstep, b_mag_tstep, i_vel_bc, i_cod_bc, i_comp_bc, &
-                          i_time_step_control
+   use parameters, only : i_Th, i_Ph
    use lsd_c
    implicit none
    save
@@ -22,32 +19,17 @@
 
 !------------------------------------------------------------------------
 !> modules/transform.fftw3.F90:65-98.  priority (FFTW planning effort) has
-!! no counterpart: the GPU transforms need no plans.
+!! no counterpart: the GPU transforms need no plans.  The context is the
+!! one the drop-in modules share (created by leg_precompute if that came
+!! first).
 !------------------------------------------------------------------------
    subroutine tra_precompute(priority)
+      use lsd_dropin, only : lsd_dropin_context
       integer, intent(in) :: priority
-      type(lsd_config) :: cfg
       if (c_associated(tra_ctx)) return
-      cfg%L = i_L; cfg%M = i_M; cfg%Mp = i_Mp; cfg%N = i_N; cfg%Th = i_Th; cfg%Ph = i_Ph
-      cfg%vel = 1
-      cfg%cod = merge(1, 0, b_cod_tstep)
-      cfg%comp = merge(1, 0, b_comp_tstep)
-      cfg%mag = merge(1, 0, b_mag_tstep)
-      cfg%vel_bc = i_vel_bc; cfg%cod_bc = i_cod_bc; cfg%comp_bc = i_comp_bc
-      cfg%precision = LSD_PLAIN_F64
-      cfg%rratio = d_rratio; cfg%Ra = d_Ra; cfg%Ra_comp = d_Ra_comp; cfg%Pr = d_Pr; cfg%Sc = d_Sc
-      cfg%Pm = d_Pm; cfg%E = d_E; cfg%dt = abs(d_timestep); cfg%implicit = d_implicit
-      cfg%dterr = d_dterr; cfg%source = d_source; cfg%source_comp = d_source_comp
-      cfg%step_control = i_time_step_control
-      call lsd_check(lsd_create(cfg, 0_c_int, tra_ctx), 'tra_precompute')
+      tra_ctx = lsd_dropin_context()
       call lsd_check(lsd_get_table(tra_ctx, LSD_T_GW, tra_gw), 'tra_precompute')
    end subroutine tra_precompute
-
-!> The library context, for the drop-in nonlinear and timestep modules.
-   function tra_context() result(ctx)
-      type(c_ptr) :: ctx
-      ctx = tra_ctx
-   end function tra_context
 
 !------------------------------------------------------------------------
 !> modules/transform.fftw3.F90:472-520
@@ -79,6 +61,21 @@
       call lsd_check(lsd_tra_qst2rtp(tra_ctx, q%Re, q%Im, s%Re, s%Im, t%Re, t%Im, r%Re, th%Re, ph%Re), &
                      'tra_qst2rtp')
    end subroutine tra_qst2rtp
+
+!------------------------------------------------------------------------
+!> modules/transform.fftw3.F90:611-717, the anelastic branch of
+!! vel_transform (modules/velocity.F90:336-339), which this drop-in does
+!! not provide; present so that the reference's velocity module compiles
+!! against it unchanged.  The Boussinesq branch calls tra_qst2rtp.
+!------------------------------------------------------------------------
+   subroutine tra_qst2rtp_vel(mes_oc, q, s, t, qr, sr, tr, r, th, ph, dur, qv1, dutr, dupr, qv2)
+      use meshs, only : rdom
+      use variables, only : spec, phys
+      type(rdom), intent(in)  :: mes_oc
+      type(spec), intent(in)  :: q, s, t, qr, sr, tr
+      type(phys), intent(out) :: r, th, ph, dur, qv1, dutr, dupr, qv2
+      stop 'tra_qst2rtp_vel: the anelastic branch is not part of the MI355X drop-in'
+   end subroutine tra_qst2rtp_vel
 
 !------------------------------------------------------------------------
 !> modules/transform.fftw3.F90:825-949

@@ -26,7 +26,15 @@ other machines, see ref_io.exe); the edited template, objects and module
 files are deleted after the link.  Nothing of the reference is copied into
 the repository.
 
-Usage: python oracle/ref_build.py [case ...]      (default: every case)
+With --dropin it builds instead oracle/_ref/<case>/dropin_step: the same
+driver and the same reference modules, except that modules/legendre.F90 and
+modules/transform.fftw3.F90 are replaced by the repository's drop-in
+modules (leeds-code_amd/fortran/legendre.hip.F90, transform.hip.F90, with
+lsd_c.F90 and lsd_dropin.F90), linked against leeds-code_amd/liblsdgpu.so:
+the reference's own physics modules stepping with the Legendre tables and
+transforms on the GPU.  Run it on a GPU machine.
+
+Usage: python oracle/ref_build.py [--dropin] [case ...]      (default: every case)
 """
 import os
 import gzip
@@ -104,8 +112,12 @@ def run(cmd, cwd):
     subprocess.run(cmd, cwd=cwd, check=True)
 
 
-def build(case, cfg):
-    obj = os.path.join(HERE, "_ref", "_obj", case)
+DROPIN = ["lsd_c", "lsd_dropin", "legendre.hip", "transform.hip"]
+LIBDIR = os.path.join(ROOT, "leeds-code_amd")
+
+
+def build(case, cfg, dropin=False):
+    obj = os.path.join(HERE, "_ref", "_obj", case + ("_dropin" if dropin else ""))
     out = os.path.join(HERE, "_ref", case)
     shutil.rmtree(obj, ignore_errors=True)
     os.makedirs(os.path.join(obj, "inc", "sub"))
@@ -116,7 +128,16 @@ def build(case, cfg):
     big = ["-mcmodel=medium"] if cfg.sizes()[1] * cfg.sizes()[2] * cfg.N > 1 << 22 else []
     flags = ["-cpp", "-O2", "-ffp-contract=off", "-I" + os.path.join(obj, "inc", "sub"),
              "-I" + os.path.join(obj, "inc")] + big
-    srcs = [os.path.join(obj, "parameters.F90")] + [os.path.join(REF, "modules", m + ".F90") for m in MODULES]
+    srcs = [os.path.join(obj, "parameters.F90")]
+    for m in MODULES:
+        if dropin and m == "legendre":
+            # the binding and the shared context first, then the drop-ins in
+            # the places of the modules they replace
+            srcs += [os.path.join(LIBDIR, "fortran", d + ".F90") for d in DROPIN[:3]]
+        elif dropin and m == "transform.fftw3":
+            srcs.append(os.path.join(LIBDIR, "fortran", "transform.hip.F90"))
+        else:
+            srcs.append(os.path.join(REF, "modules", m + ".F90"))
     objs = []
     for s in srcs:
         o = os.path.join(obj, os.path.basename(s).replace(".F90", ".o"))
@@ -128,8 +149,12 @@ def build(case, cfg):
         objs.append(o)
     drv = os.path.join(obj, "ref_step.o")
     run([FC] + flags + ["-c", os.path.join(HERE, "ref_step.F90"), "-o", drv], obj)
-    exe = os.path.join(out, "ref_step")
-    run([FC] + big + ["-o", exe, drv] + objs + ["-L" + MKL, "-lmkl_rt", "-Wl,-rpath," + MKL], obj)
+    exe = os.path.join(out, "dropin_step" if dropin else "ref_step")
+    libs = ["-L" + MKL, "-lmkl_rt", "-Wl,-rpath," + MKL]
+    if dropin:
+        # liblsdgpu.so found relative to the executable, oracle/_ref/<case>/
+        libs += ["-L" + LIBDIR, "-llsdgpu", "-Wl,-rpath,$ORIGIN/../../../leeds-code_amd"]
+    run([FC] + big + ["-o", exe, drv] + objs + libs, obj)
     shutil.rmtree(obj)
     with open(exe, "rb") as f, gzip.open(exe + ".gz", "wb", compresslevel=6) as g:
         shutil.copyfileobj(f, g, 1 << 24)
@@ -140,10 +165,12 @@ def main(argv):
     if not os.path.isdir(os.path.join(REF, "modules")):
         print("reference sources not present; nothing to build")
         return 0
+    dropin = "--dropin" in argv
+    argv = [a for a in argv if a != "--dropin"]
     allc = cases()
     names = argv or list(allc)
     for n in names:
-        build(n, allc[n])
+        build(n, allc[n], dropin)
     shutil.rmtree(os.path.join(HERE, "_ref", "_obj"), ignore_errors=True)
     return 0
 

@@ -21,14 +21,15 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 FIELDS = ("uT", "uP", "C", "X", "BT", "BP")
 
 
-def exe(case):
-    """Path of the case's executable; where only the gzip copy is present
-    (it is what travels to other machines) it is unpacked once into the
-    system's temporary folder."""
-    p = os.path.join(HERE, "_ref", case, "ref_step")
+def exe(case, name="ref_step"):
+    """Path of the case's executable (name: ref_step, or dropin_step for the
+    build with the repository's drop-in legendre and transform modules);
+    where only the gzip copy is present (it is what travels to other
+    machines) it is unpacked once into the system's temporary folder."""
+    p = os.path.join(HERE, "_ref", case, name)
     if os.access(p, os.X_OK) or not os.path.exists(p + ".gz"):
         return p
-    out = os.path.join(tempfile.gettempdir(), "lsd_ref_%s_%d" % (case, os.getuid()), "ref_step")
+    out = os.path.join(tempfile.gettempdir(), "lsd_ref_%s_%d" % (case, os.getuid()), name)
     if not os.access(out, os.X_OK) or os.path.getmtime(out) < os.path.getmtime(p + ".gz"):
         os.makedirs(os.path.dirname(out), exist_ok=True)
         tmp = out + ".part"
@@ -39,8 +40,8 @@ def exe(case):
     return out
 
 
-def available(case):
-    p = os.path.join(HERE, "_ref", case, "ref_step")
+def available(case, name="ref_step"):
+    p = os.path.join(HERE, "_ref", case, name)
     return os.access(p, os.X_OK) or os.path.exists(p + ".gz")
 
 
@@ -104,20 +105,25 @@ def _unlimited_stack():
     resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
 
 
-def run(case, cfg, state, nsteps, every=1, timing=False, threads=1):
-    """Runs ref_step; returns (records, seconds per step or None)."""
+def run(case, cfg, state, nsteps, every=1, timing=False, threads=1, name="ref_step"):
+    """Runs ref_step (or dropin_step); returns (records, seconds per step or
+    None)."""
     env = dict(os.environ)
     env["MKL_NUM_THREADS"] = str(threads)
     env["OMP_NUM_THREADS"] = str(threads)
+    # dropin_step links leeds-code_amd/liblsdgpu.so (also when unpacked
+    # elsewhere)
+    lib = os.path.join(os.path.dirname(HERE), "leeds-code_amd")
+    env["LD_LIBRARY_PATH"] = lib + (":" + env["LD_LIBRARY_PATH"] if env.get("LD_LIBRARY_PATH") else "")
     with tempfile.TemporaryDirectory() as d:
         sin = os.path.join(d, "state.in")
         sout = os.path.join(d, "dump.out")
         write_state(sin, cfg, state)
-        cmd = [exe(case), sin, sout, str(nsteps), str(every)] + (["time"] if timing else [])
+        cmd = [exe(case, name), sin, sout, str(nsteps), str(every)] + (["time"] if timing else [])
         # the reference keeps large automatic arrays on the stack
         p = subprocess.run(cmd, cwd=d, env=env, capture_output=True, text=True, preexec_fn=_unlimited_stack)
         if p.returncode != 0:
-            raise RuntimeError("ref_step failed (%d): %s %s" % (p.returncode, p.stdout[-2000:], p.stderr[-2000:]))
+            raise RuntimeError("%s failed (%d): %s %s" % (name, p.returncode, p.stdout[-2000:], p.stderr[-2000:]))
         sec = None
         for line in p.stdout.splitlines():
             if line.startswith("sec/step"):
