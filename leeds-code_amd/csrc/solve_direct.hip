@@ -66,6 +66,33 @@ __device__ __forceinline__ P pick(const P (&v)[6], int f) {
     return r;
 }
 
+// Column access of a lane pair.  Lanes 2 hh and 2 hh + 1 hold the Re and Im
+// columns of one harmonic, whose coll column stores (Re, Im) of each row as
+// 16 bytes.  Each lane moves the 16 bytes of one row of a pair of rows (lane
+// part: row n + part) and the pair exchanges halves, so each lane gets (or
+// gives) its part of both rows with one request per row pair instead of one
+// 8-byte request per row and lane.  The exchange is a DPP swap within the
+// lane quad; every lane of the wave takes part in it.
+__device__ __forceinline__ double swap_pair(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, 0xB1, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0xB1, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// v0, v1: this lane's part of rows n, n + 1 (n even).
+__device__ __forceinline__ void load_pair(const double2* col, int n, int part, double& v0, double& v1) {
+    const double2 v = col[n + part];
+    const double recv = swap_pair(part ? v.x : v.y);
+    v0 = part ? recv : v.x;
+    v1 = part ? v.y : recv;
+}
+
+__device__ __forceinline__ void store_pair(double2* col, int n, int part, double v0, double v1, bool act) {
+    const double recv = swap_pair(part ? v0 : v1);
+    if (act) col[n + part] = part ? make_double2(recv, v1) : make_double2(v0, recv);
+}
+
 // LU kind of the (main) solve of field f; the poloidal velocity's XGre solve.
 __host__ __device__ inline int lu_kind(int f) { return f == 0 ? 0 : (f == 1 ? 2 : f + 1); }
 
@@ -282,8 +309,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
         const bool act = hh < nh;
         const int h = a.order[first + (act ? hh : 0)];
         const bool solve = f == 2 || f == 3 || l != 0;
-        // this lane's real column of the right-hand side, every row at once,
-        // issued ahead of the factor staging so both are in flight together
+        // this lane's part of the right-hand side, every row at once, issued
+        // ahead of the factor staging so both are in flight together
+        // (8-byte loads into the column: pair loads would hold 16 bytes per
+        // row until every row has arrived)
         const double* src = reinterpret_cast<const double*>(pick(a.rhs, f) + (size_t)h * N) + part;
         double b[N];
 #pragma unroll
@@ -295,16 +324,16 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
         }
         __syncthreads();
         if (solve) gbtrs_reg<N, double>(b, s.ab, s.rd, pivots(a, lu_kind(f), l));
-        double* xs = reinterpret_cast<double*>(pick(a.x, f) + (size_t)h * N) + part;
+        double2* xcol = pick(a.x, f) + (size_t)h * N;
         if (f == 1 && solve) {
             // XPol right-hand side: XGre^-1 N + Y x inside, zero on the
             // boundary rows (where the second part is zero)
-            const double* src2 = reinterpret_cast<const double*>(a.rhs2 + (size_t)h * N) + part;
+            const double2* r2col = a.rhs2 + (size_t)h * N;
 #pragma unroll
             for (int c = 0; c < N / BT; c++) {
                 double r2[BT];
 #pragma unroll
-                for (int r = 0; r < BT; r++) r2[r] = src2[2 * (c * BT + r)];
+                for (int r = 0; r < BT; r += 2) load_pair(r2col, c * BT + r, part, r2[r], r2[r + 1]);
 #pragma unroll
                 for (int r = 0; r < BT; r++) {
                     const int n = c * BT + r;
@@ -325,16 +354,16 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
 #pragma unroll
             for (int c = 0; c < N / BT; c++) {
 #pragma unroll
-                for (int r = 0; r < BT; r++) {
+                for (int r = 0; r < BT; r += 2) {
                     const int n = c * BT + r;
-                    if (act) xs[2 * n] = b[n] - ca * s.g[n] - cb * s.g[N + n];
+                    store_pair(xcol, n, part, b[n] - ca * s.g[n] - cb * s.g[N + n],
+                               b[n + 1] - ca * s.g[n + 1] - cb * s.g[N + n + 1], act);
                 }
                 BATCH_FENCE();
             }
         } else {
 #pragma unroll
-            for (int n = 0; n < N; n++)
-                if (act) xs[2 * n] = b[n];
+            for (int n = 0; n < N; n += 2) store_pair(xcol, n, part, b[n], b[n + 1], act);
         }
     }
 }
@@ -361,10 +390,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
         // X is solved except for the l = 0 modes of the velocity and the
         // magnetic field
         const bool solve = f == 2 || f == 3 || l != 0;
-        const double* xsrc = reinterpret_cast<const double*>(pick(a.x, f) + (size_t)h * N) + part;
-        const double* nsrc = reinterpret_cast<const double*>(pick(a.nn, f) + (size_t)h * N) + part;
-        double* hsto = reinterpret_cast<double*>(pick(a.hist, f) + (size_t)h * N) + part;
-        // N*, every row at once, issued ahead of the factor staging
+        double2* xcol = pick(a.x, f) + (size_t)h * N;
+        const double2* ncol = pick(a.nn, f) + (size_t)h * N;
+        double2* hcol = pick(a.hist, f) + (size_t)h * N;
+        // N*, every row at once, issued ahead of the factor staging (8-byte
+        // loads into the column; the streamed columns below move by pairs)
+        const double* nsrc = reinterpret_cast<const double*>(ncol) + part;
         double b[N];
 #pragma unroll
         for (int n = 0; n < N; n++) b[n] = nsrc[2 * n];
@@ -380,11 +411,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
         for (int c = 0; c < N / BT; c++) {
             double hv[BT];
 #pragma unroll
-            for (int r = 0; r < BT; r++) hv[r] = hsto[2 * (c * BT + r)];
+            for (int r = 0; r < BT; r += 2) load_pair(hcol, c * BT + r, part, hv[r], hv[r + 1]);
+#pragma unroll
+            for (int r = 0; r < BT; r += 2) store_pair(hcol, c * BT + r, part, b[c * BT + r], b[c * BT + r + 1], act);
 #pragma unroll
             for (int r = 0; r < BT; r++) {
                 const int n = c * BT + r;
-                if (act) hsto[2 * n] = b[n];
                 const double keep = (n == 0 || n == N - 1) ? 0.0 : 1.0;
                 b[n] = keep * (cimp * (b[n] - hv[r]));
                 pin(b[n]);
@@ -392,9 +424,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
             BATCH_FENCE();
         }
         // vel_setbc_Tor for (1, 0) once tim_it /= 0
-        if (f == 0 && l == 1 && a.hm[h] == 0 && part == 0) b[0] = 0.0 - xsrc[0];
+        if (f == 0 && l == 1 && a.hm[h] == 0 && part == 0) b[0] = 0.0 - xcol[0].x;
         if (solve) gbtrs_reg<N, double>(b, s.ab, s.rd, pivots(a, lu_kind(f), l));
-        double* xs = reinterpret_cast<double*>(pick(a.x, f) + (size_t)h * N) + part;
         if (f == 1 && solve) {
             b[0] = 0.0;
             b[N - 1] = 0.0;
@@ -410,6 +441,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
 #pragma unroll
             for (int c = 0; c < N / BT; c++) {
                 double xo[BT];
+                double* xs = reinterpret_cast<double*>(xcol) + part;
 #pragma unroll
                 for (int r = 0; r < BT; r++) xo[r] = xs[2 * (c * BT + r)];
 #pragma unroll
@@ -428,18 +460,21 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
 #pragma unroll
             for (int c = 0; c < N / BT; c++) {
                 double xo[BT];
+                double* xs = reinterpret_cast<double*>(xcol) + part;
 #pragma unroll
                 for (int r = 0; r < BT; r++) xo[r] = xs[2 * (c * BT + r)];
 #pragma unroll
                 for (int r = 0; r < BT; r++) {
                     const int n = c * BT + r;
                     const double d = b[n];
+                    xo[r] = xo[r] + d;
                     if (act) {
-                        xs[2 * n] = xo[r] + d;
                         e2 = e2 + d * d;
                         if (f >= 4 && n == 0 && l != 0) eic = eic + d * d;
                     }
                 }
+#pragma unroll
+                for (int r = 0; r < BT; r += 2) store_pair(xcol, c * BT + r, part, xo[r], xo[r + 1], act);
                 BATCH_FENCE();
             }
         }
