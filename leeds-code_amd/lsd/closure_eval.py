@@ -1,6 +1,7 @@
 """Evaluation of the learned closure at the reference's config-5 scale: a
 plain resolved run at L = M = 128 (maximal degree 127; M must be even), N = 64,
-E = 3e-4, Ra = 5 x the onset estimate, Pm = 2, Pr = 1, without composition,
+E = 3e-4, Ra = 5 x the E^(-1/3) onset estimate (Ra_mod = 417.6, 6.6 x the
+measured onset lsd.config.ra_crit), Pm = 2, Pr = 1, without composition,
 and the truncated model at L = M = 42.
 
   1. The resolved run starts from a seeded state band-limited to l < 42 and
@@ -123,7 +124,7 @@ def main(argv=None):
     fb = {k: float(np.mean([r[k] for r in res[False]])) for k in res[False][0]}
     cr = {k: float(np.mean([r[k] for r in res[True]])) for k in res[True][0]}
     rec = {
-        "workload": "closure L=M=%d -> %d, N=%d, E=3e-4, Ra=5 Ra_c, Pm=2, dt=%g" % (ch.L, cl.L, ch.N, args.dt),
+        "workload": "closure L=M=%d -> %d, N=%d, E=3e-4, Ra_mod=417.6, Pm=2, dt=%g" % (ch.L, cl.L, ch.N, args.dt),
         "snapshots": args.snapshots, "every": args.every, "spinup": args.spinup,
         "train_steps": args.train_steps, "loss_first": float(np.mean(loss[:10])), "loss_last": float(np.mean(loss[-10:])),
         "heldout_skill": float(skill), "heldout_skill_uT": float(skill_ut),

@@ -2,7 +2,7 @@
 runs with L = M drawn from {32, 48, 64, 86} (the maximal degrees 31, 47, 63,
 85, rounded up to even M as par_precompute requires) and N from {32, 48, 64},
 each with its own Ekman number (1e-4 .. 1e-3), Rayleigh number (2 to 10 times
-the onset estimate) and Pm (1 .. 5), mixed precision, stepped together by
+the measured onset, lsd.config.ra_crit) and Pm (1 .. 5), mixed precision, stepped together by
 lsd_step_many for a fixed number of timesteps from seeded states.
 
 Reports the aggregate rate (member timesteps per second of wall time, over
@@ -20,14 +20,14 @@ import numpy as np
 
 
 def members(n, seed):
-    from .config import make_config, grid_for, ra_crit_estimate, MIXED_F32
+    from .config import make_config, grid_for, ra_crit, MIXED_F32
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(n):
         L = int(rng.choice([32, 48, 64, 86]))
         N = int(rng.choice([32, 48, 64]))
         E = float(10 ** rng.uniform(-4, -3))
-        ra = float(rng.uniform(2, 10)) * ra_crit_estimate(E)
+        ra = float(rng.uniform(2, 10)) * ra_crit(E)
         pm = float(rng.uniform(1, 5))
         th, ph = grid_for(L, L)
         out.append(make_config(L=L, M=L, N=N, Th=th, Ph=ph, comp=0, Ra=ra, Pm=pm, E=E, dt=2e-6,

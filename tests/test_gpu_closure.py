@@ -4,7 +4,8 @@ part of the nonlinear terms on held-out snapshots and, applied in time
 stepping, bring the truncated run closer to the resolved one than the
 truncated model alone (the fallback).
 
-Workload: E = 3e-4, Ra = 5 x the critical estimate, Pm = 2, Pr = 1, no
+Workload: E = 3e-4, Ra = 5 x the E^(-1/3) onset estimate (Ra_mod = 417.6,
+6.6 x the measured onset lsd.config.ra_crit), Pm = 2, Pr = 1, no
 composition, dt = 3e-5, N = 32 shells.  The resolved run (L = M = 48) starts
 from a seeded state band-limited to l < 24 and spins up 6000 steps; 200
 snapshots 20 steps apart give (truncated state, truncated resolved N,

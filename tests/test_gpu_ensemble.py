@@ -11,7 +11,7 @@ the next even value here (32, 48, 64, 86), on the grids of lsd.grid_for.
 import numpy as np
 import pytest
 
-from lsd import make_config, seeded_state, MIXED_F32, PLAIN_F64, ra_crit_estimate, grid_for
+from lsd import make_config, seeded_state, MIXED_F32, PLAIN_F64, ra_crit, grid_for
 from lsd.gpu import Context, step_many
 
 pytestmark = pytest.mark.gpu
@@ -21,7 +21,7 @@ def member(k, rng):
     L = int(rng.choice([32, 48, 64, 86]))
     N = int(rng.choice([32, 48, 64]))
     E = float(10 ** rng.uniform(-4, -3))
-    ra = float(rng.uniform(2, 10)) * ra_crit_estimate(E)
+    ra = float(rng.uniform(2, 10)) * ra_crit(E)
     pm = float(rng.uniform(1, 5))
     th, ph = grid_for(L, L)
     return make_config(L=L, M=L, N=N, Th=th, Ph=ph, comp=0, Ra=ra, Pm=pm, E=E, dt=2e-6, precision=MIXED_F32)

@@ -132,6 +132,25 @@ def test_steps_config1():
     assert worst < 1e-10
 
 
+@pytest.mark.parametrize("comp", [0, 1])
+def test_steps_non_magnetic(comp):
+    """Convection without the magnetic field (mag = 0: the fields, products
+    and solves of the induction equation drop out), the form the onset scan
+    steps: 20 plain steps against the oracle, same corrector counts."""
+    cfg = config_small(mag=0, comp=comp)
+    st = seeded_state(cfg, 20261207, 1e-2)
+    g = Context(cfg)
+    o = O.Oracle(cfg)
+    g.set_state(st)
+    o.set_state(st)
+    sg = g.step(20)
+    so = o.step(20)
+    assert sg.evaluations == so.evaluations
+    for k in range(6):
+        assert rel_l2(g.get_coll(k), o.get_coll(k)) < 1e-11, k
+    g.close()
+
+
 def test_one_step_config3():
     cfg = config3()
     st = seeded_state(cfg, 20261203, 1e-2)
