@@ -65,6 +65,9 @@ typedef struct lsd_config {
     double source;  /* d_source                                                 */
     double source_comp; /* d_source_comp                                        */
     int step_control;   /* i_time_step_control: 2 (or 0, the default) or 1     */
+    int Nic;            /* i_Nic: inner-core radial points; 1 (or 0) = insulating
+                           inner core, > 1 = conducting inner core            */
+    int rot;            /* b_rot_tstep: time-step the inner-core rotation rate */
 } lsd_config;
 
 /* Coll-field identifiers. */

@@ -30,6 +30,7 @@
       integer(c_int) :: precision
       real(c_double) :: rratio, Ra, Ra_comp, Pr, Sc, Pm, E, dt, implicit, dterr, source, source_comp
       integer(c_int) :: step_control
+      integer(c_int) :: Nic, rot
    end type lsd_config
 
    type, bind(C) :: lsd_step_stats

@@ -33,6 +33,7 @@
          cfg%Pm = d_Pm; cfg%E = d_E; cfg%dt = abs(d_timestep); cfg%implicit = d_implicit
          cfg%dterr = d_dterr; cfg%source = d_source; cfg%source_comp = d_source_comp
          cfg%step_control = i_time_step_control
+         cfg%Nic = 1; cfg%rot = 0
          call lsd_check(lsd_create(cfg, 0_c_int, ctx), 'lsd_dropin_context')
       end if
       c = ctx

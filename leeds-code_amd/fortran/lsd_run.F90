@@ -50,6 +50,7 @@
    cfg%rratio = rratio; cfg%Ra = Ra; cfg%Ra_comp = Ra_comp; cfg%Pr = Pr; cfg%Sc = Sc
    cfg%Pm = Pm; cfg%E = E; cfg%dt = dt; cfg%implicit = implicit; cfg%dterr = dterr
    cfg%source = source; cfg%source_comp = source_comp; cfg%step_control = 2
+   cfg%Nic = 1; cfg%rot = 0
 
    call lsd_check(lsd_config_sizes(cfg, H, iTh, iPh), 'par_precompute')
    call lsd_check(lsd_create(cfg, 0_c_int, ctx), 'initialise')

@@ -25,6 +25,7 @@ class LsdConfig(ctypes.Structure):
         ("E", ctypes.c_double), ("dt", ctypes.c_double), ("implicit", ctypes.c_double),
         ("dterr", ctypes.c_double), ("source", ctypes.c_double), ("source_comp", ctypes.c_double),
         ("step_control", ctypes.c_int),
+        ("Nic", ctypes.c_int), ("rot", ctypes.c_int),
     ]
 
     def sizes(self):
@@ -45,7 +46,8 @@ def make_config(**kw):
     base = dict(L=32, M=32, Mp=1, N=32, Th=0, Ph=0, vel=1, cod=1, comp=0, mag=1,
                 vel_bc=1, cod_bc=1, comp_bc=1, precision=PLAIN_F64,
                 rratio=0.35, Ra=6.5e4, Ra_comp=1.0, Pr=1.0, Sc=1.0, Pm=5.0, E=1.0e-3,
-                dt=1.0e-4, implicit=0.5, dterr=0.025, source=0.0, source_comp=0.0)
+                dt=1.0e-4, implicit=0.5, dterr=0.025, source=0.0, source_comp=0.0,
+                Nic=1, rot=0)
     base.update(kw)
     c = LsdConfig()
     for k, v in base.items():
@@ -135,6 +137,21 @@ def christensen_case1(**kw):
     file's dt."""
     base = dict(L=64, M=16, Mp=4, N=64, Th=96, Ph=48, comp=0, Ra=6.5e4, Pr=1.0, Pm=5.0, E=1e-3,
                 dt=-6.106330848710143e-05, implicit=0.5, step_control=1)
+    base.update(kw)
+    return make_config(**base)
+
+
+def christensen_case2(**kw):
+    """The Christensen case-2 benchmark of the reference
+    (Benchmark_files/Boussinesq/Christensen_case2_benchmark/parameters.F90,
+    INFO): the case-1 dynamo with a conducting inner core of i_Nic = 16
+    points that rotates freely under the viscous and magnetic torques
+    (b_rot_tstep), Ra = 7.15e4 (Ra_mod = 110), N = 64, L = 64, M = 16,
+    Mp = 4, E = 1e-3, Pr = 1, Pm = 5, adaptive step from dt = 5e-5 under
+    i_time_step_control = 1 (the benchmark directory's
+    i_time_step_control, d_dterr and d_timestep)."""
+    base = dict(L=64, M=16, Mp=4, N=64, Th=96, Ph=48, comp=0, Ra=7.15e4, Pr=1.0, Pm=5.0, E=1e-3,
+                dt=-5e-5, implicit=0.5, step_control=1, Nic=16, rot=1)
     base.update(kw)
     return make_config(**base)
 

@@ -82,5 +82,64 @@ def seeded_state(cfg, seed, amp):
     return out
 
 
+def ic_radii(cfg):
+    """Inner-core points r = r_i cos(pi (2 Nic - n) / (2 Nic - 1)),
+    n = Nic+1 .. 2 Nic (modules/meshs.F90:165-169)."""
+    Ni = max(1, cfg.Nic)
+    ri = cfg.rratio / (1.0 - cfg.rratio)
+    N_ = 2 * Ni
+    n = np.arange(Ni + 1, N_ + 1)
+    return ri * np.cos(np.pi * (N_ - n) / (N_ - 1))
+
+
+def seeded_ic(cfg, seed, amp):
+    """Seeded inner-core fields {icBT, icBP: (re[H,Nic], im[H,Nic])}: smooth
+    in r, zero at r_i (where the seeded outer-core field is zero too) and
+    at l = 0, with the (l+1)^-2 spectrum of seeded_state."""
+    H = cfg.sizes()[0]
+    Ni = max(1, cfg.Nic)
+    l, m = harmonics(cfg.L, cfg.M, cfg.Mp)
+    ri = cfg.rratio / (1.0 - cfg.rratio)
+    r = ic_radii(cfg)
+    shape = np.sin(np.pi * r / ri) * (r / ri) ** 2
+    scale = amp * (l.astype(np.float64) + 1.0) ** -2
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = {}
+    for name in ("icBT", "icBP"):
+        z = rng.standard_normal((2, H))
+        re = (z[0] * scale)[:, None] * shape[None, :]
+        im = (z[1] * scale)[:, None] * shape[None, :]
+        im[m == 0, :] = 0.0
+        re[l == 0, :] = 0.0
+        im[l == 0, :] = 0.0
+        out[name] = (np.ascontiguousarray(re), np.ascontiguousarray(im))
+    return out
+
+
+def ic_extend(cfg, field, r, dr1):
+    """io_ic_extend (program/io.F90:1438-1477): an inner-core field a r^3 +
+    b r^2 that meets the outer-core field (re[H,N], im[H,N]) at r_i in value
+    and radial slope, the slope from the first row of the outer-core weights
+    dr1 (the library's or the reference's mesh array dr(1), [N, 7] band
+    storage).  Returns (re[H,Nic], im[H,Nic])."""
+    rc = ic_radii(cfg)
+    r3 = rc * (rc * rc)
+    r2 = rc * rc
+    ri = float(r[0])
+    ri2 = ri * ri
+    ri3 = ri * ri2
+    w = [float(dr1[j, 3 - j]) for j in range(4)]  # dr(1)%M(KL+1+1-j, j), j = 1..4
+    out = []
+    for part in field:
+        f = part[:, 0]
+        f1 = np.zeros_like(f)
+        for j in range(4):
+            f1 = f1 + w[j] * part[:, j]
+        a = (f1 * ri - 2.0 * f) / ri3
+        b = (f - a * ri3) / ri2
+        out.append(np.ascontiguousarray(a[:, None] * r3[None, :] + b[:, None] * r2[None, :]))
+    return out[0], out[1]
+
+
 SEEDS = {1: 20261201, 2: 20261202, 3: 20261203, 4: 20261204, 5: 20261205}
 AMPS = {1: 1e-3, 2: 1.0, 3: 1e-2, 4: 1e-3}

@@ -506,6 +506,18 @@ struct orc_sim {
     double dterr, lasterr, corr_dt;
     int adaptive;          /* d_timestep < 0: CFL / corrector controlled dt */
     double cfl_dt;         /* tim_cfl_dt of the last non_maxtstep */
+    /* conducting inner core (i_Nic > 1) and the inner-core rotation */
+    int Nic, NT;                      /* inner-core points; coupled length Nic + N - 1 */
+    double* icrp;                     /* inner-core r(n,p) = icrp[(p+3)*Nic + n] */
+    double *icdr1, *icdr2, *icdr3, *icradlap;
+    double* iclu[2];                  /* coupled X of BT, BP [l*NT*LDAB] */
+    int* icpiv[2];                    /* [l*NT] */
+    double* icy;                      /* inner-core Y [l*Nic*MBW] */
+    coll_t icf[2], icn[2], ichist[2]; /* icBT, icBP, their N, saved N: [h*Nic + n] */
+    double* gwp0;                     /* leg_gwP0 [Th] */
+    double omega, inertia;            /* rot_omega, rot_inertia */
+    double vtorq, mtorq;              /* rot_velTorq, rot_magTorq of the last evaluation */
+    double vtorq_s, mtorq_s;          /* velTorq, magTorq saved by rot_predictor/corrector */
 };
 
 /* d_courant = 0.8 is a default-real literal in the reference
@@ -520,6 +532,54 @@ static coll_t coll_new(int n) {
     return c;
 }
 static void coll_free(coll_t* c) { free(c->re); free(c->im); }
+
+/* FD weights dr(1..3) of one radial domain of N points, r(n,p) =
+ * rp[(p+3)*N + n], and if w != NULL its r^2 dr integration weights
+ * (mes_rdom_init, modules/meshs.F90:292-336). */
+static int rdom_weights(int N, const double* rp, double* drs[3], double* w) {
+    const double* r1 = rp + 4 * N;
+    const double* r2 = rp + 5 * N;
+    for (int n = 1; n <= N; n++) {
+        int l = n - KL > 1 ? n - KL : 1;
+        int r = n + KL < N ? n + KL : N;
+        int nn = r - l + 1;
+        double A[MBW * MBW];
+        for (int i = 1; i <= nn; i++) A[(i - 1)] = 1.0;
+        for (int j = 2; j <= nn; j++)
+            for (int i = 1; i <= nn; i++)
+                A[(j - 1) * MBW + (i - 1)] =
+                    A[(j - 2) * MBW + (i - 1)] * (r1[l + i - 2] - r1[n - 1]) / (double)(j - 1);
+        if (mat_invert(nn, A, MBW)) { set_err("matrix inversion error 1"); return LSD_E_SINGULAR; }
+        for (int i = 1; i <= KL; i++)
+            for (int j = l; j <= r; j++)
+                drs[i - 1][(j - 1) * MBW + (KL + n - j)] = A[(j - l) * MBW + i];
+        if (!w) continue;
+        /* integration weights r^2 dr (modules/meshs.F90:311-336) */
+        double c = 1.0, e = 1.0;
+        int np = n + 1 < N ? n + 1 : N, nm = n - 1 > 1 ? n - 1 : 1;
+        for (int i = 1; i <= nn; i++) {
+            c = c * (r1[np - 1] - r1[n - 1]) / (double)i;
+            e = e * (r1[nm - 1] - r1[n - 1]) / (double)i;
+            for (int j = l; j <= r; j++) {
+                double wt = 0.5 * (c - e) * A[(j - l) * MBW + (i - 1)] * r2[n - 1];
+                w[j - 1] += wt;
+            }
+        }
+    }
+    return LSD_OK;
+}
+
+/* radLap = dr2 + dr1 * 2/r (modules/meshs.F90:339-347). */
+static void rdom_radlap(int N, const double* rp, const double* dr1, const double* dr2, double* radlap) {
+    const double* rm1 = rp + 2 * N;
+    for (int j = 1; j <= N; j++) {
+        int l = j - KL > 1 ? j - KL : 1, r = j + KL < N ? j + KL : N;
+        for (int n = l; n <= r; n++) {
+            int q = (j - 1) * MBW + (KL + n - j);
+            radlap[q] = dr2[q] + dr1[q] * 2.0 * rm1[n - 1];
+        }
+    }
+}
 
 /* mes_precompute / mes_rdom_init (modules/meshs.F90:116-219, 228-419),
  * Boussinesq: rho = Temp0 = 0, eta = 1 (program/io.F90:547-550), f-layer
@@ -542,45 +602,36 @@ static int mesh_init(orc_sim* s) {
     s->r1dr = (double*)calloc((size_t)MBW * N, sizeof(double));
     s->intr2dr = (double*)calloc((size_t)N, sizeof(double));
     double* drs[3] = {s->dr1, s->dr2, s->dr3};
-    const double* r2 = s->rp + 5 * N;
-    for (int n = 1; n <= N; n++) {
-        int l = n - KL > 1 ? n - KL : 1;
-        int r = n + KL < N ? n + KL : N;
-        int nn = r - l + 1;
-        double A[MBW * MBW];
-        for (int i = 1; i <= nn; i++) A[(i - 1)] = 1.0;
-        for (int j = 2; j <= nn; j++)
-            for (int i = 1; i <= nn; i++)
-                A[(j - 1) * MBW + (i - 1)] =
-                    A[(j - 2) * MBW + (i - 1)] * (r1[l + i - 2] - r1[n - 1]) / (double)(j - 1);
-        if (mat_invert(nn, A, MBW)) { set_err("matrix inversion error 1"); return LSD_E_SINGULAR; }
-        for (int i = 1; i <= KL; i++)
-            for (int j = l; j <= r; j++)
-                drs[i - 1][(j - 1) * MBW + (KL + n - j)] = A[(j - l) * MBW + i];
-        /* integration weights r^2 dr (modules/meshs.F90:311-336) */
-        double c = 1.0, e = 1.0;
-        int np = n + 1 < N ? n + 1 : N, nm = n - 1 > 1 ? n - 1 : 1;
-        for (int i = 1; i <= nn; i++) {
-            c = c * (r1[np - 1] - r1[n - 1]) / (double)i;
-            e = e * (r1[nm - 1] - r1[n - 1]) / (double)i;
-            for (int j = l; j <= r; j++) {
-                double w = 0.5 * (c - e) * A[(j - l) * MBW + (i - 1)] * r2[n - 1];
-                s->intr2dr[j - 1] += w;
-            }
-        }
-    }
-    /* radLap = dr2 + dr1 * 2/r (modules/meshs.F90:339-347) */
-    const double* rm1 = s->rp + 2 * N;
-    for (int j = 1; j <= N; j++) {
-        int l = j - KL > 1 ? j - KL : 1, r = j + KL < N ? j + KL : N;
-        for (int n = l; n <= r; n++) {
-            int q = (j - 1) * MBW + (KL + n - j);
-            s->radlap[q] = s->dr2[q] + s->dr1[q] * 2.0 * rm1[n - 1];
-        }
-    }
+    if (rdom_weights(N, s->rp, drs, s->intr2dr)) return LSD_E_SINGULAR;
+    rdom_radlap(N, s->rp, s->dr1, s->dr2, s->radlap);
     /* r1dr = dr1 + 1/r on the diagonal (modules/meshs.F90:408-409) */
+    const double* rm1 = s->rp + 2 * N;
     memcpy(s->r1dr, s->dr1, sizeof(double) * MBW * N);
     for (int j = 0; j < N; j++) s->r1dr[j * MBW + KL] += rm1[j];
+    return LSD_OK;
+}
+
+/* The inner-core domain of mes_precompute (modules/meshs.F90:165-169):
+ * Nic points r = r_i cos(pi (2 Nic - n) / (2 Nic - 1)), n = Nic+1 .. 2 Nic,
+ * the outer half of a Chebyshev grid over the diameter, with its own FD
+ * weights and radLap (mes_rdom_init). */
+static int ic_mesh_init(orc_sim* s) {
+    int Ni = s->Nic, N_ = 2 * Ni;
+    double ri = s->c.rratio / (1.0 - s->c.rratio);
+    s->icrp = (double*)calloc((size_t)7 * Ni, sizeof(double));
+    double* r1 = s->icrp + 4 * Ni;
+    for (int n = Ni + 1; n <= N_; n++) r1[n - Ni - 1] = ri * cos(D_PI * (N_ - n) / (double)(N_ - 1));
+    for (int p = -3; p <= 3; p++) {
+        if (p == 1) continue;
+        for (int n = 0; n < Ni; n++) s->icrp[(p + 3) * Ni + n] = ipow(r1[n], p);
+    }
+    s->icdr1 = (double*)calloc((size_t)MBW * Ni, sizeof(double));
+    s->icdr2 = (double*)calloc((size_t)MBW * Ni, sizeof(double));
+    s->icdr3 = (double*)calloc((size_t)MBW * Ni, sizeof(double));
+    s->icradlap = (double*)calloc((size_t)MBW * Ni, sizeof(double));
+    double* drs[3] = {s->icdr1, s->icdr2, s->icdr3};
+    if (rdom_weights(Ni, s->icrp, drs, NULL)) return LSD_E_SINGULAR;
+    rdom_radlap(Ni, s->icrp, s->icdr1, s->icdr2, s->icradlap);
     return LSD_OK;
 }
 
@@ -690,6 +741,73 @@ static int factor(double* ab, int* ipiv, int N, const char* who) {
     return LSD_OK;
 }
 
+/* tim_iclumesh_Tor_X / tim_iclumesh_Pol_X (modules/timestep.F90:218-344)
+ * for i_Nic > 1 with the BC rows of mag_bc_Tor / mag_bc_Pol
+ * (modules/magnetic.F90:137-282), Boussinesq without a diffusivity profile
+ * (MagTorLap = MagPolLap = radLap): one band over the inner-core points
+ * 1..Nic and the outer-core points Nic..NT (the inner-core boundary is point
+ * Nic of both), row 1 regular at the innermost point, row Nic the
+ * continuity of dB/dr across the inner-core boundary, row NT insulating. */
+static void build_icX(orc_sim* s, int l, int pol, double* ab) {
+    const int Ni = s->Nic, No = s->N, NT = s->NT;
+    const double c1_ = 1.0 / s->dt;
+    double c2_ = 1.0 * s->c.implicit;
+    if (s->c.implicit == 0.0) c2_ = 1.0;
+    const double ll1 = (double)(l * (l + 1));
+    const double* icrm2 = s->icrp + 1 * Ni;
+    const double* icrm1 = s->icrp + 2 * Ni;
+    const double* rm2 = s->rp + 1 * No;
+    const double* rm1 = s->rp + 2 * No;
+    memset(ab, 0, sizeof(double) * LDAB * NT);
+    for (int j = 0; j < Ni; j++)
+        for (int q = 0; q < MBW; q++) ab[j * LDAB + KL + q] = -c2_ * s->icradlap[j * MBW + q];
+    for (int j = 0; j < Ni; j++) {
+        double* d = &ab[j * LDAB + 2 * KL];
+        *d = (*d + c1_) + c2_ * ll1 * icrm2[j];
+    }
+    for (int j = 0; j < No; j++)
+        for (int q = 0; q < MBW; q++)
+            ab[(Ni - 1 + j) * LDAB + KL + q] = ab[(Ni - 1 + j) * LDAB + KL + q] - c2_ * s->radlap[j * MBW + q];
+    for (int j = 0; j < No; j++) {
+        double* d = &ab[(Ni - 1 + j) * LDAB + 2 * KL];
+        *d = (*d + c1_) + c2_ * ll1 * rm2[j];
+    }
+    for (int j = 1; j <= 1 + KL; j++) LUA(ab, 1, j) = 0.0;
+    for (int j = NT - KL; j <= NT; j++) LUA(ab, NT, j) = 0.0;
+    for (int j = Ni - KL; j <= Ni + KL; j++) LUA(ab, Ni, j) = 0.0;
+    if (!pol) {
+        LUA(ab, 1, 1) = 1.0;   /* BTor = 0 at the innermost point */
+        LUA(ab, NT, NT) = 1.0; /* insulating exterior */
+    } else {
+        /* (dr - l/r) BPol = 0 at the innermost point */
+        for (int j = 1; j <= 1 + KL; j++) LUA(ab, 1, j) = MSH(s->icdr1, 1, j);
+        LUA(ab, 1, 1) = LUA(ab, 1, 1) - icrm1[0] * l;
+        /* (dr + (l+1)/r) BPol = 0 at r_o */
+        for (int j = No - KL; j <= No; j++) LUA(ab, NT, j + Ni - 1) = MSH(s->dr1, No, j);
+        LUA(ab, NT, NT) = LUA(ab, NT, NT) + rm1[No - 1] * (l + 1);
+    }
+    /* dB/dr continuous across the inner-core boundary */
+    for (int j = 2; j <= 1 + KL; j++) LUA(ab, Ni, j + Ni - 1) = MSH(s->dr1, 1, j);
+    for (int j = Ni - KL; j <= Ni - 1; j++) LUA(ab, Ni, j) = -MSH(s->icdr1, Ni, j);
+    LUA(ab, Ni, Ni) = MSH(s->dr1, 1, 1) - MSH(s->icdr1, Ni, Ni);
+}
+
+/* tim_mesh_Y on the inner-core domain (modules/timestep.F90:421-449),
+ * c1 = c2 = 1: (7, Nic). */
+static void build_icY(orc_sim* s, int l, double* m) {
+    const int Ni = s->Nic;
+    const double c1_ = 1.0 / s->dt;
+    const double c2_ = 1.0 * (1.0 - s->c.implicit);
+    const double* icrm2 = s->icrp + 1 * Ni;
+    const double ll = (double)(l * (l + 1));
+    for (int j = 0; j < Ni; j++)
+        for (int q = 0; q < MBW; q++) m[j * MBW + q] = c2_ * s->icradlap[j * MBW + q];
+    for (int j = 0; j < Ni; j++) {
+        double* d = &m[j * MBW + KL];
+        *d = (*d + c1_) - c2_ * (ll * icrm2[j]);
+    }
+}
+
 /* vel_matrices, cod_matrices, comp_matrices, mag_matrices
  * (modules/velocity.F90:63-170, codensity.F90:46-75, composition.F90:43-72,
  * magnetic.F90:72-133) with the BC rows of vel_bc_Tor/Pol/Gre
@@ -738,6 +856,15 @@ static int matrices(orc_sim* s) {
             LUA(ab, N, N) = LUA(ab, N, N) + rm1[N - 1] * (l + 1);
             if (factor(ab, s->piv[6] + l * N, N, "mes_iclu_find")) return LSD_E_SINGULAR;
             build_Y(s, 1.0, 1.0, l, s->ym[5] + (size_t)l * N * MBW);
+            if (s->Nic > 1) {
+                /* conducting inner core: the coupled bands replace the two above */
+                for (int k = 0; k < 2; k++) {
+                    ab = s->iclu[k] + (size_t)l * s->NT * LDAB;
+                    build_icX(s, l, k, ab);
+                    if (factor(ab, s->icpiv[k] + l * s->NT, s->NT, "mes_iclu_find")) return LSD_E_SINGULAR;
+                }
+                build_icY(s, l, s->icy + (size_t)l * s->Nic * MBW);
+            }
         }
         /* codensity (l = 0 included) */
         ab = s->lu[3] + (size_t)l * N * LDAB;
@@ -793,6 +920,10 @@ orc_sim* orc_create(const lsd_config* cfg) {
         set_err("only i_vel_bc = i_cod_bc = i_comp_bc = 1 are supported");
         return NULL;
     }
+    if (cfg->Nic < 0 || (cfg->Nic > 1 && (!cfg->mag || cfg->Nic < 2 * KL + 2))) {
+        set_err("i_Nic > 1 needs the magnetic field and at least 8 inner-core points");
+        return NULL;
+    }
     orc_sim* s = (orc_sim*)calloc(1, sizeof(orc_sim));
     s->c = *cfg;
     s->L = cfg->L; s->M = cfg->M; s->Mp = cfg->Mp; s->N = cfg->N;
@@ -832,8 +963,30 @@ orc_sim* orc_create(const lsd_config* cfg) {
         s->rtll1[l] = sqrt(s->ll1[l]);
         s->rtll11[l] = 1.0 / s->rtll1[l];
     }
+    s->Nic = cfg->Nic > 1 ? cfg->Nic : 1;
+    s->NT = s->Nic + N - 1;
     if (mesh_init(s)) { orc_destroy(s); return NULL; }
+    if (s->Nic > 1 && ic_mesh_init(s)) { orc_destroy(s); return NULL; }
     leg_init(s);
+    /* leg_gwP0 (modules/legendre.F90:103-104) */
+    s->gwp0 = (double*)calloc((size_t)Th, sizeof(double));
+    for (int j = 0; j < s->Th2; j++) {
+        s->gwp0[j] = s->tab[0][j] * s->gw[j] * 0.5;
+        s->gwp0[Th - 1 - j] = s->gwp0[j];
+    }
+    /* rot_precompute (modules/rotation.F90:20-36) */
+    s->omega = 0.0;
+    s->inertia = (8.0 * D_PI / 15.0) * ipow(s->rp[4 * N], 5);
+    for (int k = 0; k < 2; k++) {
+        s->icf[k] = coll_new(H * s->Nic);
+        s->icn[k] = coll_new(H * s->Nic);
+        s->ichist[k] = coll_new(H * s->Nic);
+        if (s->Nic > 1) {
+            s->iclu[k] = (double*)calloc((size_t)L * s->NT * LDAB, sizeof(double));
+            s->icpiv[k] = (int*)calloc((size_t)L * s->NT, sizeof(int));
+        }
+    }
+    if (s->Nic > 1) s->icy = (double*)calloc((size_t)L * s->Nic * MBW, sizeof(double));
     for (int k = 0; k < 7; k++) {
         s->lu[k] = (double*)calloc((size_t)L * N * LDAB, sizeof(double));
         s->piv[k] = (int*)calloc((size_t)L * N, sizeof(int));
@@ -865,7 +1018,41 @@ void orc_destroy(orc_sim* s) {
     for (int k = 0; k < LSD_NUM_COLL; k++) coll_free(&s->f[k]);
     for (int k = 0; k < 6; k++) coll_free(&s->hist[k]);
     for (int k = 0; k < LSD_NUM_PHYS; k++) free(s->phys[k]);
+    free(s->icrp); free(s->icdr1); free(s->icdr2); free(s->icdr3); free(s->icradlap);
+    for (int k = 0; k < 2; k++) {
+        free(s->iclu[k]); free(s->icpiv[k]);
+        coll_free(&s->icf[k]); coll_free(&s->icn[k]); coll_free(&s->ichist[k]);
+    }
+    free(s->icy); free(s->gwp0);
     free(s);
+}
+
+/* Inner-core fields icBT (0) and icBP (1), re/im[h*Nic + n]. */
+int orc_set_ic(orc_sim* s, int k, const double* re, const double* im) {
+    if (k < 0 || k > 1) { set_err("inner-core field must be 0 or 1"); return LSD_E_ARG; }
+    memcpy(s->icf[k].re, re, sizeof(double) * s->H * s->Nic);
+    memcpy(s->icf[k].im, im, sizeof(double) * s->H * s->Nic);
+    return 0;
+}
+int orc_get_ic(orc_sim* s, int k, double* re, double* im) {
+    if (k < 0 || k > 1) { set_err("inner-core field must be 0 or 1"); return LSD_E_ARG; }
+    memcpy(re, s->icf[k].re, sizeof(double) * s->H * s->Nic);
+    memcpy(im, s->icf[k].im, sizeof(double) * s->H * s->Nic);
+    return 0;
+}
+/* out[0..2] = rot_omega, rot_velTorq, rot_magTorq. */
+int orc_get_rot(orc_sim* s, double* out) {
+    out[0] = s->omega; out[1] = s->vtorq; out[2] = s->mtorq;
+    return 0;
+}
+int orc_set_rot(orc_sim* s, double omega) {
+    s->omega = omega;
+    return 0;
+}
+/* Inner-core radii r(1..Nic). */
+int orc_get_icr(orc_sim* s, double* out) {
+    memcpy(out, s->icrp + 4 * s->Nic, sizeof(double) * s->Nic);
+    return 0;
 }
 
 int orc_set_dt(orc_sim* s, double dt) {
@@ -1372,9 +1559,80 @@ int orc_transform(orc_sim* s) {
 
 /* ---------------------------------------------------------- nonlinear */
 
+/* tra_surfintegral (modules/transform.fftw3.F90:1039-1062) of one shell
+ * p[k*Th + j]: (4 pi / Ph) sum_j gwP0(j) sum_k p(j, k). */
+static double surfintegral(const orc_sim* s, const double* p) {
+    const int Th = s->Th, Ph = s->Ph;
+    double S_ = 0.0;
+    for (int j = 0; j < Th; j++) {
+        double Y = 0.0;
+        for (int k = 0; k < Ph; k++) Y = Y + p[(size_t)k * Th + j];
+        S_ = S_ + Y * s->gwp0[j];
+    }
+    return S_ * 4.0 * D_PI / (double)Ph;
+}
+
+/* non_rotation (modules/nonlinear.F90:479-537), no-slip: the viscous
+ * torque r_i^3 Pm surf((-2/r_i u_phi - (curl u)_theta) sin) and the
+ * magnetic torque r_i^3 (2 Pm / E) surf(B_r B_phi sin) on the inner core,
+ * from the physical fields of the last transform at r_i. */
+static void rotation_torques(orc_sim* s) {
+    const int Th = s->Th, Ph = s->Ph, N = s->N;
+    const size_t plane = (size_t)Th * Ph;
+    const double ri = s->rp[4 * N], ri3 = s->rp[6 * N];
+    const double vfac = ri3 * s->c.Pm;
+    const double mfac = ri3 * 2 * s->c.Pm / s->c.E;
+    double* p = (double*)calloc(plane, sizeof(double));
+    const double S = -2.0 / ri;
+    const double* up = s->phys[LSD_P_VEL_P];
+    const double* wt = s->phys[LSD_P_VCURL_T];
+    for (int k = 0; k < Ph; k++)
+        for (int j = 0; j < Th; j++) {
+            const size_t q = (size_t)k * Th + j;
+            p[q] = (S * up[q] - wt[q]) * s->sn[j];
+        }
+    s->vtorq = surfintegral(s, p) * vfac;
+    if (s->Nic == 1) {
+        s->mtorq = 0.0;
+    } else {
+        const double* br = s->phys[LSD_P_MAG_R];
+        const double* bp = s->phys[LSD_P_MAG_P];
+        for (int k = 0; k < Ph; k++)
+            for (int j = 0; j < Th; j++) {
+                const size_t q = (size_t)k * Th + j;
+                p[q] = br[q] * bp[q] * s->sn[j];
+            }
+        s->mtorq = surfintegral(s, p) * mfac;
+    }
+    free(p);
+}
+
+/* The inner-core part of non_magnetic (modules/nonlinear.F90:450-471):
+ * induction by the solid-body rotation of the inner core,
+ * N = -rot_omega dB/dphi, and zero at (l, m) = (0, 0). */
+static void ic_nonlinear(orc_sim* s) {
+    const int Ni = s->Nic, H = s->H;
+    for (int k = 0; k < 2; k++) {
+        const coll_t* in = &s->icf[k];
+        coll_t* out = &s->icn[k];
+        for (int nh = 0; nh < H; nh++) {
+            const double m_ = (double)s->hm[nh];
+            for (int n = 0; n < Ni; n++) {
+                const size_t q = (size_t)nh * Ni + n;
+                const double a = in->re[q];
+                const double dre = -in->im[q] * m_, dim = a * m_;
+                out->re[q] = -(s->omega * dre);
+                out->im[q] = -(s->omega * dim);
+            }
+        }
+        for (int n = 0; n < Ni; n++) out->re[n] = out->im[n] = 0.0;
+    }
+}
+
 /* non_velocity, non_codensity, non_composition, non_magnetic, Boussinesq
  * branches (modules/nonlinear.F90:28-199, 205-289, 294-357, 363-473). */
 int orc_nonlinear(orc_sim* s) {
+    if (s->c.rot) rotation_torques(s);
     int N = s->N, Th = s->Th, Ph = s->Ph, H = s->H, HN = H * N;
     size_t np = (size_t)Th * Ph * N;
     const lsd_config* c = &s->c;
@@ -1499,6 +1757,7 @@ int orc_nonlinear(orc_sim* s) {
             }
         }
         for (int n = 0; n < N; n++) { MT->re[n] = MT->im[n] = MP->re[n] = MP->im[n] = 0.0; }
+        ic_nonlinear(s);
     }
     free(r); free(th); free(ph);
     spec_free(&a); spec_free(&b); spec_free(&d);
@@ -1534,6 +1793,18 @@ static double addcorr(const orc_sim* s, const coll_t* ac, coll_t* a) {
     return sqrt(e);
 }
 
+/* tim_addcorr on a field of n entries (the inner-core fields). */
+static double addcorr_n(const coll_t* ac, coll_t* a, size_t n) {
+    double e = 0.0;
+    for (size_t q = 0; q < n; q++) {
+        e = e + ac->re[q] * ac->re[q];
+        e = e + ac->im[q] * ac->im[q];
+        a->re[q] = a->re[q] + ac->re[q];
+        a->im[q] = a->im[q] + ac->im[q];
+    }
+    return sqrt(e);
+}
+
 /* vel_gre_adjust (modules/velocity.F90:445-465) on one harmonic vector. */
 static void gre_adjust(const orc_sim* s, int l, double* v) {
     int N = s->N;
@@ -1550,18 +1821,71 @@ static const double* LUp(const orc_sim* s, int k, int l) { return s->lu[k] + (si
 static const int* PVp(const orc_sim* s, int k, int l) { return s->piv[k] + l * s->N; }
 static const double* YMp(const orc_sim* s, int k, int l) { return s->ym[k] + (size_t)l * s->N * MBW; }
 
+/* tim_multY on the inner-core domain for one harmonic: out = c + A.b. */
+static void multY_ic(const orc_sim* s, const double* A, const double* b, const double* cc, double* out) {
+    const int Ni = s->Nic;
+    double tmp[1024];
+    for (int n = 0; n < Ni; n++) tmp[n] = cc[n];
+    for (int j = 1; j <= Ni; j++) {
+        int nl = j - KL > 1 ? j - KL : 1, nr = j + KL < Ni ? j + KL : Ni;
+        for (int n = nl; n <= nr; n++) tmp[n - 1] = tmp[n - 1] + MSH(A, n, j) * b[j - 1];
+    }
+    for (int n = 0; n < Ni; n++) out[n] = tmp[n];
+}
+
+/* mag_predictor for i_Nic > 1 (modules/magnetic.F90:370-411), one harmonic
+ * and part of field k (0 BT, 1 BP): Y B + N on each domain, zero BC rows
+ * (mag_setbc_Tor / mag_setbc_Pol with zero BC arrays, tim_zerobc on the
+ * inner core), then tim_icinvX (modules/timestep.F90:594-634). */
+static void ic_predict(orc_sim* s, int nh, int part, int k) {
+    const int N = s->N, Ni = s->Nic, NT = s->NT, l = s->hl[nh];
+    const size_t o = (size_t)nh * N, oi = (size_t)nh * Ni;
+    double* x = part ? s->f[LSD_MAG_TOR + k].im + o : s->f[LSD_MAG_TOR + k].re + o;
+    const double* nn = part ? s->f[LSD_N_MAG_TOR + k].im + o : s->f[LSD_N_MAG_TOR + k].re + o;
+    double* xi = part ? s->icf[k].im + oi : s->icf[k].re + oi;
+    const double* ni = part ? s->icn[k].im + oi : s->icn[k].re + oi;
+    double v[1024], vi[1024], b[2048];
+    if (l == 0) memcpy(v, nn, sizeof(double) * N);
+    else multY(s, YMp(s, 4 + k, l), x, nn, v);
+    if (l == 0) memcpy(vi, ni, sizeof(double) * Ni);
+    else multY_ic(s, s->icy + (size_t)l * Ni * MBW, xi, ni, vi);
+    vi[0] = 0.0; vi[Ni - 1] = 0.0;
+    v[0] = 0.0; v[N - 1] = 0.0;
+    if (l != 0) {
+        for (int n = 0; n < Ni - 1; n++) b[n] = vi[n];
+        for (int n = 0; n < N; n++) b[Ni - 1 + n] = v[n];
+        orc_gbtrs(NT, s->iclu[k] + (size_t)l * NT * LDAB, s->icpiv[k] + l * NT, b);
+        for (int n = 0; n < Ni; n++) vi[n] = b[n];
+        for (int n = 0; n < N; n++) v[n] = b[Ni - 1 + n];
+    }
+    memcpy(x, v, sizeof(double) * N);
+    memcpy(xi, vi, sizeof(double) * Ni);
+}
+
 /* vel_predictor, cod_predictor, comp_predictor, mag_predictor
  * (modules/velocity.F90:369-395, codensity.F90:224-236,
  * composition.F90:209-223, magnetic.F90:370-411). */
 int orc_predictor(orc_sim* s) {
     int N = s->N, H = s->H;
     size_t HN = (size_t)H * N;
+    /* rot_predictor (modules/rotation.F90:43-64) */
+    if (s->c.rot) {
+        s->omega = s->omega + (s->dt / s->inertia) * (s->vtorq + s->mtorq);
+        s->vtorq_s = s->vtorq;
+        s->mtorq_s = s->mtorq;
+    }
     /* save N at time t */
     int nmap[6] = {LSD_N_VEL_TOR, LSD_N_VEL_POL, LSD_N_COD, LSD_N_COMP, LSD_N_MAG_TOR, LSD_N_MAG_POL};
     for (int k = 0; k < 6; k++) {
         memcpy(s->hist[k].re, s->f[nmap[k]].re, sizeof(double) * HN);
         memcpy(s->hist[k].im, s->f[nmap[k]].im, sizeof(double) * HN);
     }
+    const int ic = s->Nic > 1 && s->c.mag;
+    if (ic)
+        for (int k = 0; k < 2; k++) {
+            memcpy(s->ichist[k].re, s->icn[k].re, sizeof(double) * H * s->Nic);
+            memcpy(s->ichist[k].im, s->icn[k].im, sizeof(double) * H * s->Nic);
+        }
 #pragma omp parallel for num_threads(g_threads) schedule(dynamic, 16)
     for (int nh = 0; nh < H; nh++) {
         int l = s->hl[nh], m_ = s->hm[nh];
@@ -1573,8 +1897,8 @@ int orc_predictor(orc_sim* s) {
             const double* nn = part ? s->f[LSD_N_VEL_TOR].im + o : s->f[LSD_N_VEL_TOR].re + o;
             if (l == 0) memcpy(v, nn, sizeof(double) * N);
             else multY(s, YMp(s, 0, l), x, nn, v);
-            v[0] = 0.0; v[N - 1] = 0.0; /* vel_setbc_Tor, rot_omega = 0 */
-            (void)m_;
+            v[0] = 0.0; v[N - 1] = 0.0; /* vel_setbc_Tor (velocity.F90:279-303) */
+            if (l == 1 && m_ == 0 && part == 0) v[0] = s->omega * s->rp[4 * N];
             if (l != 0) orc_gbtrs(N, LUp(s, 0, l), PVp(s, 0, l), v);
             memcpy(x, v, sizeof(double) * N);
             /* velocity poloidal */
@@ -1603,7 +1927,9 @@ int orc_predictor(orc_sim* s) {
                 orc_gbtrs(N, LUp(s, 3 + f, l), PVp(s, 3 + f, l), v);
                 memcpy(x, v, sizeof(double) * N);
             }
-            if (s->c.mag) {
+            if (ic) {
+                for (int k = 0; k < 2; k++) ic_predict(s, nh, part, k);
+            } else if (s->c.mag) {
                 /* magnetic toroidal: mag_setbc_Tor with zero BC arrays */
                 x = part ? s->f[LSD_MAG_TOR].im + o : s->f[LSD_MAG_TOR].re + o;
                 nn = part ? s->f[LSD_N_MAG_TOR].im + o : s->f[LSD_N_MAG_TOR].re + o;
@@ -1635,6 +1961,17 @@ int orc_corrector(orc_sim* s, double* dterr_out) {
     int N = s->N, H = s->H;
     size_t HN = (size_t)H * N;
     double cimp = s->c.implicit;
+    /* rot_corrector (modules/rotation.F90:70-88) */
+    if (s->c.rot) {
+        s->omega = s->omega + (s->dt * s->c.implicit / s->inertia) *
+                                  ((s->vtorq + s->mtorq) - (s->vtorq_s + s->mtorq_s));
+        s->vtorq_s = s->vtorq;
+        s->mtorq_s = s->mtorq;
+    }
+    const int ic = s->Nic > 1 && s->c.mag;
+    const int Ni = s->Nic, NT = s->NT;
+    coll_t iccorr[2];
+    for (int k = 0; k < 2; k++) iccorr[k] = coll_new(ic ? H * Ni : 1);
     /* correction fields: 0 vT, 1 vP, 2 C, 3 X, 4 bT, 5 bP */
     coll_t corr[6];
     for (int k = 0; k < 6; k++) corr[k] = coll_new((int)HN);
@@ -1653,10 +1990,29 @@ int orc_corrector(orc_sim* s, double* dterr_out) {
                 double* v = part ? corr[k].im + o : corr[k].re + o;
                 for (int n = 0; n < N; n++) v[n] = cimp * (nn[n] - np_[n]);
                 v[0] = 0.0; v[N - 1] = 0.0;
-                /* vel_setbc_Tor with tim_it /= 0 (velocity.F90:292-302),
-                 * rot_omega = 0: T_10 at r_i gets -vel_Tor there. */
+                /* vel_setbc_Tor with tim_it /= 0 (velocity.F90:292-302):
+                 * T_10 at r_i gets rot_omega r_i - vel_Tor there. */
                 if (k == 0 && part == 0 && l == 1 && s->hm[nh] == 0)
-                    v[0] = 0.0 * s->rp[4 * N] - s->f[LSD_VEL_TOR].re[o];
+                    v[0] = s->omega * s->rp[4 * N] - s->f[LSD_VEL_TOR].re[o];
+                if (ic && k >= 4) {
+                    /* tim_nlincorr on the inner core, tim_zerobc, tim_icinvX */
+                    const size_t oi = (size_t)nh * Ni;
+                    const coll_t* icn = &s->icn[k - 4];
+                    const coll_t* ich = &s->ichist[k - 4];
+                    double* vi = part ? iccorr[k - 4].im + oi : iccorr[k - 4].re + oi;
+                    for (int n = 0; n < Ni; n++)
+                        vi[n] = cimp * ((part ? icn->im : icn->re)[oi + n] - (part ? ich->im : ich->re)[oi + n]);
+                    vi[0] = 0.0; vi[Ni - 1] = 0.0;
+                    if (l != 0) {
+                        double b[2048];
+                        for (int n = 0; n < Ni - 1; n++) b[n] = vi[n];
+                        for (int n = 0; n < N; n++) b[Ni - 1 + n] = v[n];
+                        orc_gbtrs(NT, s->iclu[k - 4] + (size_t)l * NT * LDAB, s->icpiv[k - 4] + l * NT, b);
+                        for (int n = 0; n < Ni; n++) vi[n] = b[n];
+                        for (int n = 0; n < N; n++) v[n] = b[Ni - 1 + n];
+                    }
+                    continue;
+                }
                 switch (k) {
                 case 0: if (l != 0) orc_gbtrs(N, LUp(s, 0, l), PVp(s, 0, l), v); break;
                 case 1:
@@ -1691,7 +2047,16 @@ int orc_corrector(orc_sim* s, double* dterr_out) {
     dterr += addcorr(s, &corr[1], &s->f[LSD_VEL_POL]);
     if (s->c.cod) dterr += addcorr(s, &corr[2], &s->f[LSD_COD]);
     if (s->c.comp) dterr += addcorr(s, &corr[3], &s->f[LSD_COMP]);
-    if (s->c.mag) {
+    if (ic) {
+        for (int k = 0; k < 2; k++) {
+            memcpy(s->ichist[k].re, s->icn[k].re, sizeof(double) * H * Ni);
+            memcpy(s->ichist[k].im, s->icn[k].im, sizeof(double) * H * Ni);
+        }
+        dterr += addcorr(s, &corr[4], &s->f[LSD_MAG_TOR]);
+        dterr += addcorr_n(&iccorr[0], &s->icf[0], (size_t)H * Ni);
+        dterr += addcorr(s, &corr[5], &s->f[LSD_MAG_POL]);
+        dterr += addcorr_n(&iccorr[1], &s->icf[1], (size_t)H * Ni);
+    } else if (s->c.mag) {
         double e;
         dterr += addcorr(s, &corr[4], &s->f[LSD_MAG_TOR]);
         e = 0.0;
@@ -1703,6 +2068,7 @@ int orc_corrector(orc_sim* s, double* dterr_out) {
         dterr += sqrt(e);
     }
     for (int k = 0; k < 6; k++) coll_free(&corr[k]);
+    for (int k = 0; k < 2; k++) coll_free(&iccorr[k]);
     free(icT); free(icP);
     if (dterr_out) *dterr_out = dterr;
     return 0;

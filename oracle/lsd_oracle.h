@@ -65,6 +65,16 @@ int  orc_tra_rtp2qst(orc_sim* s, const double* r, const double* th, const double
                      double* qre, double* qim, double* sre, double* sim,
                      double* tre, double* tim);
 
+/* Conducting inner core (cfg.Nic > 1) and inner-core rotation (cfg.rot):
+ * the inner-core fields icBT (0) and icBP (1) as re/im[h*Nic + n], the
+ * rotation rate and the torques of the last evaluation
+ * (out[0..2] = rot_omega, rot_velTorq, rot_magTorq), the inner-core radii. */
+int  orc_set_ic(orc_sim* s, int k, const double* re, const double* im);
+int  orc_get_ic(orc_sim* s, int k, double* re, double* im);
+int  orc_get_rot(orc_sim* s, double* out);
+int  orc_set_rot(orc_sim* s, double omega);
+int  orc_get_icr(orc_sim* s, double* out);
+
 /* Energy diagnostics, var_coll_norm (modules/variables.F90:903-957):
  * out[0..3] = Eodd, Eeven, Em0le, Em0 of one coll field. */
 int  orc_coll_norm(orc_sim* s, int field, double* out);

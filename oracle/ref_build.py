@@ -58,10 +58,11 @@ AUX = ["gauwts", "schnla", "pmm", "pmm1", "plm", "dpmm", "dpmm1", "dplm"]
 def cases():
     """Case name -> lsd_config of the run (the same values the GPU path and
     the C oracle are given)."""
-    from lsd.config import config1, config3, config_small, christensen_case1
+    from lsd.config import config1, config3, config_small, christensen_case1, christensen_case2
     return {"cfg1": config1(), "small": config_small(), "cfg3": config3(),
             "adapt": config_small(dt=-1e-4), "adapt1": config_small(dt=-1e-4, step_control=1),
-            "case1": christensen_case1()}
+            "case1": christensen_case1(),
+            "ic": config_small(Nic=8, rot=1), "case2": christensen_case2()}
 
 
 def fdouble(x):
@@ -77,7 +78,7 @@ def flogical(b):
 def edits(cfg):
     """(parameter name, value text) pairs for the template's lines."""
     H, Th, Ph, Th2 = cfg.sizes()
-    ed = [("i_N", str(cfg.N)), ("i_Nic", "1"), ("i_L", str(cfg.L)), ("i_M", str(cfg.M)), ("i_Mp", str(cfg.Mp)),
+    ed = [("i_N", str(cfg.N)), ("i_Nic", str(max(1, cfg.Nic))), ("i_L", str(cfg.L)), ("i_M", str(cfg.M)), ("i_Mp", str(cfg.Mp)),
           ("d_Ra", fdouble(cfg.Ra)), ("d_Ra_comp", fdouble(cfg.Ra_comp)), ("d_Pr", fdouble(cfg.Pr)),
           ("d_Sc", fdouble(cfg.Sc)), ("d_Pm", fdouble(cfg.Pm)), ("d_E", fdouble(cfg.E)),
           ("d_rratio", fdouble(cfg.rratio)),
@@ -86,7 +87,8 @@ def edits(cfg):
           ("i_vel_bc", str(cfg.vel_bc)), ("i_cod_bc", str(cfg.cod_bc)), ("i_comp_bc", str(cfg.comp_bc)),
           ("d_source", fdouble(cfg.source)), ("d_source_comp", fdouble(cfg.source_comp)),
           ("i_time_step_control", "1" if cfg.step_control == 1 else "2"), ("d_dterr", fdouble(cfg.dterr)),
-          ("d_timestep", fdouble(cfg.dt)), ("d_implicit", fdouble(cfg.implicit))]
+          ("d_timestep", fdouble(cfg.dt)), ("d_implicit", fdouble(cfg.implicit)),
+          ("b_rot_tstep", flogical(cfg.rot))]
     # grids that differ from the template's 3/2 rule: the rule lines
     # themselves take the case's sizes (Ph = 2 i_Ma, so i_Ma = Ph/2)
     if Th != cfg.L * 3 // 2:

@@ -5,8 +5,9 @@ baseline of bench.py use it.
 
 File formats are those of oracle/ref_step.F90: the state file holds int32
 N, int32 H and then uT, uP, C, X, BT, BP as Re(N, 0:H-1), Im(N, 0:H-1) in
-float64 (the reference's coll layout, i.e. a C array [H][N]); the dump
-file holds one record per dumped step.
+float64 (the reference's coll layout, i.e. a C array [H][N]), and in the
+builds with a conducting inner core or the inner-core rotation icBT, icBP
+as [H][Nic] and rot_omega; the dump file holds one record per dumped step.
 """
 import gzip
 import os
@@ -45,14 +46,26 @@ def available(case, name="ref_step"):
     return os.access(p, os.X_OK) or os.path.exists(p + ".gz")
 
 
+def has_ic(cfg):
+    """The build carries the inner-core fields and the rotation rate."""
+    return cfg.Nic > 1 or bool(cfg.rot)
+
+
 def write_state(path, cfg, state):
     H = cfg.sizes()[0]
+    Ni = max(1, cfg.Nic)
     with open(path, "wb") as f:
         np.array([cfg.N, H], dtype=np.int32).tofile(f)
         for name in FIELDS:
             re, im = state[name]
             np.ascontiguousarray(re, dtype=np.float64).reshape(H, cfg.N).tofile(f)
             np.ascontiguousarray(im, dtype=np.float64).reshape(H, cfg.N).tofile(f)
+        if has_ic(cfg):
+            for name in ("icBT", "icBP"):
+                re, im = state.get(name, (np.zeros((H, Ni)), np.zeros((H, Ni))))
+                np.ascontiguousarray(re, dtype=np.float64).reshape(H, Ni).tofile(f)
+                np.ascontiguousarray(im, dtype=np.float64).reshape(H, Ni).tofile(f)
+            np.array([state.get("omega", 0.0)], dtype=np.float64).tofile(f)
 
 
 def _fields(buf, off, H, N):
@@ -81,18 +94,26 @@ def read_dump(path, cfg):
         off += 24
         fields, off = _fields(buf, off, H, N)
         rec = {"step": int(step), "evaluations": int(nev), "t": float(t), "dterr": float(err), "fields": fields}
+
+        def take(count, shape):
+            nonlocal off
+            a = np.frombuffer(buf, dtype=np.float64, count=count, offset=off).reshape(shape).copy()
+            off += 8 * count
+            return a
+        if has_ic(cfg):
+            Ni = max(1, cfg.Nic)
+            for name in ("icBT", "icBP"):
+                fields[name] = (take(H * Ni, (H, Ni)), take(H * Ni, (H, Ni)))
+            rec["omega"], rec["velTorq"], rec["magTorq"] = (float(v) for v in take(3, (3,)))
         if not recs:
-            def take(count, shape):
-                nonlocal off
-                a = np.frombuffer(buf, dtype=np.float64, count=count, offset=off).reshape(shape).copy()
-                off += 8 * count
-                return a
             mesh = {"r": take(N, (N,))}
             for k in (1, 2, 3):
                 mesh["dr%d" % k] = take(7 * N, (N, 7))
             mesh["radLap"] = take(7 * N, (N, 7))
             mesh["r1dr"] = take(7 * N, (N, 7))
             mesh["intr2dr"] = take(N, (N,))
+            if has_ic(cfg):
+                mesh["ric"] = take(max(1, cfg.Nic), (max(1, cfg.Nic),))
             tabs = {"gp": take(Th, (Th,)), "gw": take(Th, (Th,)), "P": take(Th2 * H, (H, Th2)),
                     "gwP": take(Th2 * H, (H, Th2))}
             rec["mesh"] = mesh

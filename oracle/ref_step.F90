@@ -10,13 +10,18 @@
 !
 ! Usage: ref_step <state.in> <dump.out> <nsteps> <dump_every> [time]
 !   state.in : int32 N, int32 H, then uT, uP, C, X, BT, BP as Re(N,0:H-1)
-!              followed by Im(N,0:H-1), float64 (the reference coll layout)
+!              followed by Im(N,0:H-1), float64 (the reference coll layout);
+!              with a conducting inner core (i_Nic > 1) or the inner-core
+!              rotation (b_rot_tstep) then icBT, icBP as Re(Nic,0:H-1),
+!              Im(Nic,0:H-1) and rot_omega
 !   dump.out : a record per dumped step: int32 step, int32 evaluations of
 !              the step, float64 tim_t, float64 tim_dterr of the accepted
-!              corrector, then the six fields as in state.in; step 0 also
-!              carries the mesh arrays r, dr(1..3), radLap, r1dr (7 x N
-!              band storage) and intr2dr, and the tables leg_gp, leg_gw,
-!              leg_P, leg_gwP.
+!              corrector, then the six fields as in state.in (and icBT,
+!              icBP, rot_omega, rot_velTorq, rot_magTorq in the inner-core
+!              builds); step 0 also carries the mesh arrays r, dr(1..3),
+!              radLap, r1dr (7 x N band storage) and intr2dr (and the
+!              inner-core radii), and the tables leg_gp, leg_gw, leg_P,
+!              leg_gwP.
 !   time     : if present, steps 2..nsteps are timed and the seconds per
 !              step printed as "sec/step <value>" (program/main.F90:414
 !              prints the same figure).
@@ -25,7 +30,8 @@
 ! replaced by the raw read above and io_load_radial_txtfiles by its
 ! Boussinesq branch (program/io.F90:508-550: eta = 1, d eta/dr = 0).
 ! With d_timestep < 0 each step starts with non_maxtstep, tim_new_tstep
-! and the *_matrices rebuild on a new dt (program/main.F90:124-135).
+! and the *_matrices rebuild on a new dt (program/main.F90:124-135).  The
+! inner-core rotation follows program/main.F90:108, 136, 148.
 !-------------------------------------------------------------------------
 #include "../parallel.h"
  program ref_step
@@ -42,8 +48,8 @@
                         vel_corrector
    use magnetic, only : mag_precompute, mag_transform, mag_matrices, mag_predictor, mag_corrector
    use nonlinear, only : non_precompute, non_codensity, non_composition, non_magnetic, non_velocity, &
-                         non_maxtstep
-   use rotation, only : rot_precompute
+                         non_maxtstep, non_rotation
+   use rotation, only : rot_precompute, rot_predictor, rot_corrector
    implicit none
 
    type(rdom) :: mes_oc, mes_ic
@@ -64,7 +70,7 @@
 
    character(len=512) :: fin, fout, arg
    integer :: nsteps, every, n, k, nev, uin, uout, nn, hh
-   logical :: timing
+   logical :: timing, icio
    integer(kind=8) :: c0, c1, crate
    double precision :: accepted_err
 
@@ -75,12 +81,19 @@
    call get_command_argument(4, arg)
    read(arg, *) every
    timing = command_argument_count() >= 5
+   icio = i_Nic > 1 .or. b_rot_tstep
 
    ! io_load_radial_txtfiles, Boussinesq without a diffusivity profile
    do n = 1, i_N
       mes_oc%eta(n,0) = 1d0
       mes_oc%eta(n,1) = 0d0
    end do
+   if (i_Nic /= 1) then
+      do n = 1, i_Nic
+         mes_ic%eta(n,0) = 1d0
+         mes_ic%eta(n,1) = 0d0
+      end do
+   end if
    call par_precompute()
    call mes_precompute(mes_oc, mes_ic)
    call leg_precompute()
@@ -105,6 +118,13 @@
    call read_coll(comp_C)
    call read_coll(mag_tor)
    call read_coll(mag_pol)
+   if (icio) then
+      read(uin) mag_icTor%Re(1:i_Nic, 0:i_H-1)
+      read(uin) mag_icTor%Im(1:i_Nic, 0:i_H-1)
+      read(uin) mag_icPol%Re(1:i_Nic, 0:i_H-1)
+      read(uin) mag_icPol%Im(1:i_Nic, 0:i_H-1)
+      read(uin) rot_omega
+   end if
    close(uin)
    ! with d_timestep < 0 (adaptive step) io_load_state takes the initial dt
    ! from the state file (program/io.F90:429); here its magnitude stands in
@@ -130,9 +150,10 @@
       write(uout) mes_oc%dr(k)%M
    end do
    write(uout) mes_oc%radLap%M, mes_oc%r1dr%M, mes_oc%intr2dr
+   if (icio) write(uout) mes_ic%r(1:i_Nic,1)
    write(uout) leg_gp, leg_gw, leg_P, leg_gwP
 
-   ! program/main.F90:100-164 (no rotation, no output files)
+   ! program/main.F90:100-164 (no output files)
    nev = 0
    accepted_err = 0d0
    call system_clock(c0, crate)
@@ -146,6 +167,8 @@
       if (b_comp_tstep) call comp_transform(mes_oc, comp_C, comp_gradr, comp_gradt, comp_gradp)
       if (b_mag_tstep) call mag_transform(mag_tor, mag_pol, mag_r, mag_t, mag_p, &
                                           mag_curlr, mag_curlt, mag_curlp)
+      if (b_rot_tstep) call non_rotation(mes_oc, mes_ic, vel_p, vel_curlt, mag_r, mag_p, &
+                                         rot_velTorq, rot_magTorq)
       if (b_vel_tstep) call non_velocity(mes_oc, vel_r, vel_t, vel_p, &
                                          vel_curlr, vel_curlt, vel_curlp, mag_r, mag_t, mag_p, &
                                          mag_curlr, mag_curlt, mag_curlp, cod_C, comp_C, vel_Ntor, vel_npol)
@@ -174,6 +197,7 @@
                call mag_matrices(mes_oc, mes_ic)
             end if
          end if
+         if (b_rot_tstep) call rot_predictor(rot_inertia, rot_velTorq, rot_magTorq, rot_omega)
          if (b_vel_tstep) call vel_predictor(mes_oc, vel_ntor, vel_npol, rot_omega, vel_tor, vel_pol)
          if (b_cod_tstep) call cod_predictor(cod_Nc, cod_bcRe, cod_bcIm, cod_C)
          if (b_comp_tstep) call comp_predictor(comp_NC, comp_bcRe, comp_bcIm, comp_C)
@@ -182,6 +206,7 @@
                                              mag_tor, mag_pol, mag_icTor, mag_icPol)
          tim_it = 1
       else
+         if (b_rot_tstep) call rot_corrector(rot_inertia, rot_velTorq, rot_magTorq, rot_omega)
          if (b_vel_tstep) call vel_corrector(mes_oc, vel_ntor, vel_npol, rot_omega, vel_tor, vel_pol)
          if (b_cod_tstep) call cod_corrector(cod_NC, cod_C)
          if (b_comp_tstep) call comp_corrector(comp_NC, comp_C)
@@ -223,6 +248,11 @@
       write(uout) comp_C%Re(1:i_N,0:i_H-1), comp_C%Im(1:i_N,0:i_H-1)
       write(uout) mag_tor%Re(1:i_N,0:i_H-1), mag_tor%Im(1:i_N,0:i_H-1)
       write(uout) mag_pol%Re(1:i_N,0:i_H-1), mag_pol%Im(1:i_N,0:i_H-1)
+      if (icio) then
+         write(uout) mag_icTor%Re(1:i_Nic,0:i_H-1), mag_icTor%Im(1:i_Nic,0:i_H-1)
+         write(uout) mag_icPol%Re(1:i_Nic,0:i_H-1), mag_icPol%Im(1:i_Nic,0:i_H-1)
+         write(uout) rot_omega, rot_velTorq, rot_magTorq
+      end if
    end subroutine write_fields
 
  end program ref_step

@@ -31,7 +31,7 @@ sys.path.insert(0, os.path.join(ROOT, "leeds-code_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 import ref_io  # noqa: E402
-from lsd.config import config1, config3, config_small, christensen_case1  # noqa: E402
+from lsd.config import config1, config3, config_small, christensen_case1, christensen_case2  # noqa: E402
 from lsd.initial import AMPS, SEEDS, harmonics, seeded_state  # noqa: E402
 
 CASES = {
@@ -49,13 +49,23 @@ CASES = {
     # reference's test state tests/state.cdf.in (tests/golden/state_case1.cdf.in),
     # adaptive step under control 1; seed -1 marks a state read from a file
     "case1": (christensen_case1, -1, 0.0, 30),
+    # a conducting inner core of 8 points that rotates (i_Nic = 8,
+    # b_rot_tstep): seeded inner-core fields (seed + 1) and rotation rate
+    # OMEGA0[case] at the start
+    "ic": (lambda: config_small(Nic=8, rot=1), 5, 1e-2, 20),
+    # the Christensen case-2 benchmark (conducting, rotating inner core of
+    # 16 points) from the case-1 test state, its field extended into the
+    # inner core as io_ic_extend does (program/io.F90:1438-1477)
+    "case2": (christensen_case2, -1, 0.0, 10),
 }
+OMEGA0 = {"ic": 0.3, "case2": 0.0}
 FULL = {"small": (1, 2, 20), "cfg1": (1,), "cfg3": (), "adapt": (1, 10, 40), "adapt1": (1, 20, 40),
-        "case1": ()}
-SAMPLED = {"small": (), "cfg1": (10, 100), "cfg3": (1, 2), "adapt": (), "adapt1": (), "case1": (1, 10, 30)}
-STATE_FILE = {"case1": "state_case1.cdf.in"}
+        "case1": (), "ic": (1, 2, 20), "case2": ()}
+SAMPLED = {"small": (), "cfg1": (10, 100), "cfg3": (1, 2), "adapt": (), "adapt1": (), "case1": (1, 10, 30),
+           "ic": (), "case2": (1, 10)}
+STATE_FILE = {"case1": "state_case1.cdf.in", "case2": "state_case1.cdf.in"}
 # cases whose fixtures also record the reference's own per-step spread
-SPREAD = ("adapt", "adapt1", "case1")
+SPREAD = ("adapt", "adapt1", "case1", "case2")
 NSAMPLE = 4096
 
 
@@ -72,14 +82,14 @@ def make(case):
     H, Th, Ph, Th2 = cfg.sizes()
     out = {"config": np.array(json.dumps(cfg.as_dict())), "seed": np.array(seed), "amp": np.array(amp),
            "nsteps": np.array(nsteps)}
+    sys.path.insert(0, os.path.dirname(HERE))
+    import ref_fixtures
     if case in STATE_FILE:
-        sys.path.insert(0, os.path.dirname(HERE))
-        import ref_fixtures
         out["state_file"] = np.array(STATE_FILE[case])
         out["b_scale"] = np.array(ref_fixtures.B_SCALE_OLD_STATE)
-        state = ref_fixtures.initial_state(out, cfg)
-    else:
-        state = seeded_state(cfg, seed, amp)
+    if case in OMEGA0:
+        out["omega0"] = np.array(OMEGA0[case])
+    state = ref_fixtures.initial_state(out, cfg)
     recs, _ = ref_io.run(case, cfg, state, nsteps, 1)
     if case in SPREAD:
         # the reference's own rounding spread: the same run on MKL's
@@ -105,12 +115,18 @@ def make(case):
             spread.append(worst)
         out["spread"] = np.array(spread)
         out["dterr_spread"] = np.array([abs(a["dterr"] - b["dterr"]) for a, b in zip(recs[1:], alt[1:])])
+        if ref_io.has_ic(cfg):
+            for key in ("omega", "velTorq", "magTorq"):
+                out[key + "_spread"] = np.array([abs(a[key] - b[key]) for a, b in zip(recs[1:], alt[1:])])
     l, m = harmonics(cfg.L, cfg.M, cfg.Mp)
     st = recs[1:]
     out["step"] = np.array([r["step"] for r in st], dtype=np.int32)
     out["evaluations"] = np.array([r["evaluations"] for r in st], dtype=np.int32)
     out["dterr"] = np.array([r["dterr"] for r in st])
     out["t"] = np.array([r["t"] for r in st])
+    if ref_io.has_ic(cfg):
+        for key in ("omega", "velTorq", "magTorq"):
+            out[key] = np.array([r[key] for r in st])
     r0 = recs[0]
     for k, v in r0["mesh"].items():
         out["mesh_" + k] = v
@@ -129,6 +145,11 @@ def make(case):
             out["s%d_%s_im" % (s, name)] = im
     for s in SAMPLED[case]:
         for name, (re, im) in by_step[s]["fields"].items():
+            if name.startswith("ic"):
+                # the inner-core fields are small: kept whole
+                out["s%d_%s_re" % (s, name)] = re
+                out["s%d_%s_im" % (s, name)] = im
+                continue
             out["q%d_%s_re" % (s, name)] = re[hs, ns]
             out["q%d_%s_im" % (s, name)] = im[hs, ns]
             p = (re ** 2 + im ** 2).sum(axis=1)

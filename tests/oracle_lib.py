@@ -59,6 +59,11 @@ def load():
     lib.orc_tra_rtp2qst.argtypes = [ctypes.c_void_p] + [_dp] * 9
     lib.orc_coll_norm.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
     lib.orc_energy.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp]
+    for name in ("orc_set_ic", "orc_get_ic"):
+        getattr(lib, name).argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+    lib.orc_get_rot.argtypes = [ctypes.c_void_p, _dp]
+    lib.orc_set_rot.argtypes = [ctypes.c_void_p, ctypes.c_double]
+    lib.orc_get_icr.argtypes = [ctypes.c_void_p, _dp]
     return lib
 
 
@@ -115,15 +120,57 @@ class Oracle:
         self._chk(self.lib.orc_get_coll(self.h, k, _p(re), _p(im)), "get_coll")
         return re, im
 
+    @property
+    def Nic(self):
+        return max(1, self.cfg.Nic)
+
+    def set_ic(self, k, re, im):
+        re = np.ascontiguousarray(re, dtype=np.float64)
+        im = np.ascontiguousarray(im, dtype=np.float64)
+        assert re.shape == (self.H, self.Nic)
+        self._chk(self.lib.orc_set_ic(self.h, k, _p(re), _p(im)), "set_ic")
+
+    def get_ic(self, k):
+        re = np.zeros((self.H, self.Nic))
+        im = np.zeros((self.H, self.Nic))
+        self._chk(self.lib.orc_get_ic(self.h, k, _p(re), _p(im)), "get_ic")
+        return re, im
+
+    def rotation(self):
+        out = np.zeros(3)
+        self.lib.orc_get_rot(self.h, _p(out))
+        return out
+
+    def set_rotation(self, omega):
+        self.lib.orc_set_rot(self.h, float(omega))
+
+    def ic_radii(self):
+        out = np.zeros(self.Nic)
+        self.lib.orc_get_icr(self.h, _p(out))
+        return out
+
     def get_phys(self, k):
         p = np.zeros((self.N, self.Ph, self.Th))
         self._chk(self.lib.orc_get_phys(self.h, k, _p(p)), "get_phys")
         return p
 
     def set_state(self, state):
+        """Fields by name (uT .. BP), and the inner-core fields icBT, icBP
+        and the rotation rate omega where the state has them."""
         from lsd.initial import FIELD_ID
-        for name, (re, im) in state.items():
-            self.set_coll(FIELD_ID[name], re, im)
+        for name, v in state.items():
+            if name == "omega":
+                self.set_rotation(v)
+            elif name in ("icBT", "icBP"):
+                self.set_ic(0 if name == "icBT" else 1, *v)
+            else:
+                self.set_coll(FIELD_ID[name], *v)
+
+    def get_named(self, name):
+        if name in ("icBT", "icBP"):
+            return self.get_ic(0 if name == "icBT" else 1)
+        from lsd.initial import FIELD_ID
+        return self.get_coll(FIELD_ID[name])
 
     def transform(self):
         self._chk(self.lib.orc_transform(self.h), "transform")

@@ -19,7 +19,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 FIELDS = ("uT", "uP", "C", "X", "BT", "BP")
-REL_STEP = {"small": 1e-12, "cfg1": 1e-12, "cfg3": 6e-11, "adapt": 1e-12, "adapt1": 1e-12, "case1": 1e-12}
+REL_STEP = {"small": 1e-12, "cfg1": 1e-12, "cfg3": 6e-11, "adapt": 1e-12, "adapt1": 1e-12, "case1": 1e-12,
+            "ic": 1e-12, "case2": 1e-12}
 
 # tim_dterr is the norm of the corrector's increment X^-1 c (N* - N).  At
 # dt below ~1e-6 (the first steps of an adaptive run, which start from
@@ -93,15 +94,31 @@ B_SCALE_OLD_STATE = 2.0 ** -0.5
 
 
 def initial_state(fx, cfg):
+    """The case's start: the state file (B scaled, and with a conducting
+    inner core its field extended into the inner core as io_ic_extend does,
+    program/io.F90:1438-1477, with the radial weights of the oracle's mesh)
+    or the seeded state (and seeded inner-core fields, seed + 1); the
+    rotation rate "omega0" where the fixture has one."""
+    from lsd.initial import seeded_state, seeded_ic, ic_extend
     if "state_file" in fx:
         from lsd.gpu import read_state
         st, _, _ = read_state(os.path.join(HERE, "golden", str(fx["state_file"])), cfg)
         b = float(fx["b_scale"])
         for name in ("BT", "BP"):
             st[name] = (st[name][0] * b, st[name][1] * b)
-        return st
-    from lsd.initial import seeded_state
-    return seeded_state(cfg, int(fx["seed"]), float(fx["amp"]))
+        if cfg.Nic > 1:
+            import oracle_lib
+            o = oracle_lib.Oracle(cfg, threads=1)
+            r, dr1 = o.mesh(0), o.mesh(1)
+            for name in ("BT", "BP"):
+                st["ic" + name] = ic_extend(cfg, st[name], r, dr1)
+    else:
+        st = seeded_state(cfg, int(fx["seed"]), float(fx["amp"]))
+        if cfg.Nic > 1:
+            st.update(seeded_ic(cfg, int(fx["seed"]) + 1, float(fx["amp"])))
+    if "omega0" in fx:
+        st["omega"] = float(fx["omega0"])
+    return st
 
 
 def rel_full(fx, step, name, re, im):
@@ -130,7 +147,8 @@ def compare(fx, case, step, get, nsteps_done):
     """dict field -> error of the state after `step` steps; `get(name)`
     returns (re, im)."""
     out = {}
-    for name in FIELDS:
+    names = FIELDS + (("icBT", "icBP") if ("s%d_icBT_re" % step) in fx else ())
+    for name in names:
         re, im = get(name)
         if ("s%d_%s_re" % (step, name)) in fx:
             out[name] = rel_full(fx, step, name, re, im)

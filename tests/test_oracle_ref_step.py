@@ -13,7 +13,6 @@ import pytest
 
 import oracle_lib
 import ref_fixtures as rf
-from lsd.initial import FIELD_ID
 
 
 def _oracle(fx, threads=8):
@@ -57,8 +56,21 @@ def _run_and_compare(case, nsteps):
         t_ref = fx["t"][k - 1]
         assert abs(st.t - t_ref) <= 1e-12 * t_ref, (k, st.t, t_ref)
         assert rf.dterr_ok(fx, k, st.last_dterr), (k, st.last_dterr, fx["dterr"][k - 1])
+        if "omega" in fx:
+            # rot_omega after the step and the torques of its last evaluation;
+            # the rate integrates the torques, so where the fixture records the
+            # reference's own spread the bound takes its largest relative
+            # value over the steps so far
+            om, vt, mt = o.rotation()
+            for got, key in ((om, "omega"), (vt, "velTorq"), (mt, "magTorq")):
+                ref = float(fx[key][k - 1])
+                rel = 1e-11
+                if key + "_spread" in fx:
+                    sp = fx[key + "_spread"][:k] / np.maximum(np.abs(fx[key][:k]), 1e-300)
+                    rel += 3.0 * float(sp.max())
+                assert abs(got - ref) <= rel * abs(ref) + 1e-300, (k, key, got, ref)
         if k in keep:
-            errs = rf.compare(fx, case, k, lambda n: o.get_coll(FIELD_ID[n]), k)
+            errs = rf.compare(fx, case, k, o.get_named, k)
             worst = max(errs.values())
             assert worst <= rf.step_bound(fx, case, k), "step %d: %s" % (k, errs)
 
@@ -96,3 +108,18 @@ def test_oracle_steps_equal_reference_config3():
     # L = M = 170, N = 96, 256 x 512: within the reference's own rounding
     # spread (ref_fixtures.REL_STEP)
     _run_and_compare("cfg3", 2)
+
+
+def test_oracle_conducting_rotating_inner_core_equal_reference():
+    # i_Nic = 8, b_rot_tstep: the coupled inner-core/outer-core bands of
+    # mag_bc_Tor / mag_bc_Pol, the inner core's induction by its rotation,
+    # the viscous and magnetic torques, rot_predictor / rot_corrector and the
+    # rotating no-slip wall of vel_setbc_Tor, from a rotation rate of 0.3
+    _run_and_compare("ic", 20)
+
+
+def test_oracle_christensen_case2_equal_reference():
+    # Christensen case 2 (i_Nic = 16, rotating inner core, N = 64, L = 64,
+    # M = 16, Mp = 4, adaptive control 1) from the case-1 test state, its
+    # field extended into the inner core as io_ic_extend does
+    _run_and_compare("case2", 10)
