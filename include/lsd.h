@@ -158,6 +158,19 @@ void lsd_destroy(lsd_ctx* ctx);
  * fill of io_load_state / io_load_coll (program/io.F90:349-483,1278-1355). */
 int lsd_set_coll(lsd_ctx* ctx, int field, const double* re, const double* im);
 int lsd_get_coll(lsd_ctx* ctx, int field, double* re, double* im);
+/* The inner core's magnetic field with cfg->Nic > 1: which = 0 mag_icTor,
+ * 1 mag_icPol, re/im [H][Nic] on the inner-core points
+ * r = r_i cos(pi (2 Nic - n) / (2 Nic - 1)), n = Nic+1 .. 2 Nic
+ * (modules/meshs.F90:165-169); its last point is the inner-core boundary,
+ * the outer core's first.  Replaces the icBT/icBP fill of io_load_state
+ * (program/io.F90:440-447).  LSD_E_ARG for an insulating inner core. */
+int lsd_set_ic(lsd_ctx* ctx, int which, const double* re, const double* im);
+int lsd_get_ic(lsd_ctx* ctx, int which, double* re, double* im);
+/* Inner-core rotation (modules/rotation.F90): out[0..2] = rot_omega,
+ * rot_velTorq and rot_magTorq of the last nonlinear evaluation; set sets
+ * rot_omega (io_load_state reads it as icOmega, program/io.F90:430-431). */
+int lsd_get_rotation(lsd_ctx* ctx, double* out);
+int lsd_set_rotation(lsd_ctx* ctx, double omega);
 /* State files: io_save_state / io_load_state (program/io.F90:1502-1621,
  * 349-483) in the netCDF classic format (CDF-1 written; CDF-1 and CDF-2
  * read), variables uT, uP, C, Comp, BT, BP (and zero icBT, icBP) as

@@ -49,6 +49,14 @@ static int check_config(const lsd_config* c, int* H, int* Th, int* Ph) {
         set_error("the mixed forms need Ph <= 1024 (fused physical-space pass)");
         return LSD_E_CONFIG;
     }
+    if (c->Nic < 0 || (c->Nic > 1 && (!c->mag || c->Nic < 8 || c->Nic > c->N))) {
+        set_error("i_Nic > 1 needs the magnetic field and 8 <= i_Nic <= i_N");
+        return LSD_E_CONFIG;
+    }
+    if (c->rot != 0 && c->rot != 1) {
+        set_error("rot (b_rot_tstep) must be 0 or 1");
+        return LSD_E_CONFIG;
+    }
     if (H) *H = c->L * c->M - c->Mp * (c->M - 1) * c->M / 2;
     if (Th) *Th = th;
     if (Ph) *Ph = ph;
@@ -93,7 +101,9 @@ static void release(Ctx* c) {
                     c->d_atab32, c->d_tw32, c->d_slot_t0, c->d_row_h, c->d_atab4, c->d_ana_base, c->d_acol,
                     c->d_spec_in, c->d_spec_out, c->d_fourier, c->d_folded, c->d_phys, c->d_nlphys,
                     c->d_tw, c->d_partial, c->d_cfl_d, c->d_cfl_part, c->d_cfl, c->d_tabb, c->d_atabb,
-                    c->d_bx_sbase};
+                    c->d_bx_sbase, c->d_icr, c->d_icdr1, c->d_icdr2, c->d_icradlap, c->d_iclu, c->d_icpiv,
+                    c->d_icy, c->d_icblk, c->d_icord, c->d_icpartial, c->d_rot, c->d_gwsin,
+                    c->d_icf[0], c->d_icf[1], c->d_icn[0], c->d_icn[1], c->d_ichist[0], c->d_ichist[1]};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (int k = 0; k < LSD_NUM_COLL; k++)
@@ -104,6 +114,7 @@ static void release(Ctx* c) {
     }
     if (c->d_rhs2) (void)hipFree(c->d_rhs2);
     if (c->h_partial) (void)hipHostFree(c->h_partial);
+    if (c->h_icpartial) (void)hipHostFree(c->h_icpartial);
     if (c->h_cfl) (void)hipHostFree(c->h_cfl);
     for (hipEvent_t e : c->prof_events) (void)hipEventDestroy(e);
     drop_graphs(*c);
@@ -207,6 +218,8 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
     c->adaptive = cfg->dt < 0.0;
     c->dt = std::fabs(cfg->dt);
     c->mixed = cfg->precision != LSD_PLAIN_F64;
+    c->Nic = cfg->Nic > 1 ? cfg->Nic : 1;
+    c->NT = c->Nic + cfg->N - 1;
     *out = c;
     LSD_HIP(hipSetDevice(device));
     LSD_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -295,12 +308,58 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
     const int nblk_max = std::max(c->nblk_form[0], c->nblk_form[1]);
     if (dalloc(&c->d_partial, (size_t)8 * nblk_max)) return LSD_E_HIP;
     LSD_HIP(hipHostMalloc((void**)&c->h_partial, sizeof(double) * 8 * nblk_max, hipHostMallocDefault));
+    // rotation state; with a conducting inner core its mesh, operators,
+    // fields and the blocks of its coupled solves
+    if (dalloc(&c->d_rot, ROT_COUNT) || dalloc(&c->d_gwsin, Th)) return LSD_E_HIP;
+    std::vector<int> icorder;
+    std::vector<int4> icblocks;
+    if (c->Nic > 1) {
+        const int Ni = c->Nic, NT = c->NT;
+        c->ic_hb = ic_block_harmonics(N, Ni);
+        if (c->ic_hb < 1) {
+            set_error("i_N + i_Nic too large for the inner-core solves");
+            return LSD_E_CONFIG;
+        }
+        std::vector<std::vector<int>> byl(L);
+        for (int h = 0; h < H; h++) byl[c->hl[h]].push_back(h);
+        for (int l = 0; l < L; l++)
+            for (size_t k = 0; k < byl[l].size(); k += c->ic_hb) {
+                int cnt = (int)std::min(byl[l].size() - k, (size_t)c->ic_hb);
+                icblocks.push_back(make_int4(l, (int)icorder.size(), cnt, 0));
+                for (int q = 0; q < cnt; q++) icorder.push_back(byl[l][k + q]);
+            }
+        c->n_icblk = (int)icblocks.size();
+        if (dalloc(&c->d_icr, (size_t)7 * Ni) || dalloc(&c->d_icdr1, (size_t)7 * Ni) ||
+            dalloc(&c->d_icdr2, (size_t)7 * Ni) || dalloc(&c->d_icradlap, (size_t)7 * Ni) ||
+            dalloc(&c->d_iclu, (size_t)2 * L * NT * LSD_LDAB) || dalloc(&c->d_icpiv, (size_t)2 * L * NT) ||
+            dalloc(&c->d_icy, (size_t)L * Ni * 7) || dalloc(&c->d_icblk, icblocks.size()) ||
+            dalloc(&c->d_icord, H) || dalloc(&c->d_icpartial, (size_t)4 * c->n_icblk))
+            return LSD_E_HIP;
+        for (int k = 0; k < 2; k++)
+            if (dalloc(&c->d_icf[k], (size_t)H * Ni) || dalloc(&c->d_icn[k], (size_t)H * Ni) ||
+                dalloc(&c->d_ichist[k], (size_t)H * Ni))
+                return LSD_E_HIP;
+        LSD_HIP(hipHostMalloc((void**)&c->h_icpartial, sizeof(double) * 4 * c->n_icblk, hipHostMallocDefault));
+    }
 
     hipStream_t st = c->stream;
     LSD_HIP(hipMemcpyAsync(c->d_hl, c->hl.data(), sizeof(int) * H, hipMemcpyHostToDevice, st));
     LSD_HIP(hipMemcpyAsync(c->d_hm, c->hm.data(), sizeof(int) * H, hipMemcpyHostToDevice, st));
     LSD_HIP(hipMemcpyAsync(c->d_pidx, pidx.data(), sizeof(int) * pidx.size(), hipMemcpyHostToDevice, st));
     LSD_HIP(hipMemcpyAsync(c->d_apos, apos.data(), sizeof(int) * H, hipMemcpyHostToDevice, st));
+    if (c->Nic > 1) {
+        LSD_HIP(hipMemcpyAsync(c->d_icord, icorder.data(), sizeof(int) * H, hipMemcpyHostToDevice, st));
+        LSD_HIP(hipMemcpyAsync(c->d_icblk, icblocks.data(), sizeof(int4) * icblocks.size(), hipMemcpyHostToDevice,
+                               st));
+        for (int k = 0; k < 2; k++) {
+            LSD_HIP(hipMemsetAsync(c->d_icf[k], 0, sizeof(double2) * H * c->Nic, st));
+            LSD_HIP(hipMemsetAsync(c->d_icn[k], 0, sizeof(double2) * H * c->Nic, st));
+            LSD_HIP(hipMemsetAsync(c->d_ichist[k], 0, sizeof(double2) * H * c->Nic, st));
+        }
+        LSD_HIP(hipMemsetAsync(c->d_iclu, 0, sizeof(double) * 2 * L * c->NT * LSD_LDAB, st));
+        LSD_HIP(hipMemsetAsync(c->d_icpiv, 0, sizeof(int) * 2 * L * c->NT, st));
+        LSD_HIP(hipMemsetAsync(c->d_icy, 0, sizeof(double) * L * c->Nic * 7, st));
+    }
     for (int form = 0; form < 2; form++) {
         if (!c->d_sblk[form]) continue;
         LSD_HIP(hipMemcpyAsync(c->d_sord[form], sorder[form].data(), sizeof(int) * H, hipMemcpyHostToDevice, st));
@@ -389,6 +448,7 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
     }
     if (cfg->precision == LSD_MIXED_BF16X3 && (rc = tables_bf16(*c))) return rc;
     c->ana_wide_ok = c->d_atab4 != nullptr;
+    if ((rc = precompute_innercore(*c))) return rc;
     if ((rc = build_operators(*c))) return rc;
     LSD_HIP(hipStreamSynchronize(st));
     load_tuning(*c);
@@ -558,6 +618,8 @@ static int synthesise(Ctx& c, bool f, bool to_phys) {
     RC(launch_synthesis_group(c, SYN_SCALAR, ns, sc_in, nullptr, sc_out, nullptr, f));
     RC(launch_synthesis_group(c, SYN_VECTOR, nv, v_s, v_t, v_th, v_ph, f));
     if (ng) RC(launch_synthesis_group(c, SYN_GRAD, ng, g_in, nullptr, g_th, g_ph, f));
+    // non_rotation after the transforms (program/main.F90:108)
+    RC(launch_torques(c, f));
     if (to_phys && !f) {
         RC(launch_fft_c2r(c, fourier(c, 0), phys(c, 0), 6));
         if (c.cfg.mag) RC(launch_fft_c2r(c, fourier(c, 6), phys(c, 6), 6));
@@ -608,6 +670,7 @@ static int analyse(Ctx& c, bool f) {
     RC(launch_analysis_group(c, ANA_VECTOR, nv, v0, v1, v2, q, s_, t, f));
     if (ns) RC(launch_analysis_group(c, ANA_SCALAR, ns, sg, nullptr, nullptr, so, nullptr, nullptr, f));
     RC(launch_spec_post(c, f));
+    RC(launch_ic_nonlinear(c));
     return LSD_OK;
 }
 
@@ -1103,6 +1166,57 @@ int lsd_set_coll(lsd_ctx* ctx, int field, const double* re, const double* im) {
     std::vector<double2> tmp(HN);
     for (size_t q = 0; q < HN; q++) tmp[q] = make_double2(re[q], im[q]);
     LSD_HIP(hipMemcpyAsync(c.d_f[field], tmp.data(), sizeof(double2) * HN, hipMemcpyHostToDevice, c.stream));
+    LSD_HIP(hipStreamSynchronize(c.stream));
+    return LSD_OK;
+}
+
+int lsd_set_ic(lsd_ctx* ctx, int which, const double* re, const double* im) {
+    CTX_OR_FAIL(ctx);
+    if (!c.ic() || which < 0 || which > 1 || !re || !im) {
+        lsd::set_error("lsd_set_ic: no conducting inner core, bad field or null pointer");
+        return LSD_E_ARG;
+    }
+    const size_t n = (size_t)c.H * c.Nic;
+    std::vector<double2> tmp(n);
+    for (size_t q = 0; q < n; q++) tmp[q] = make_double2(re[q], im[q]);
+    LSD_HIP(hipMemcpyAsync(c.d_icf[which], tmp.data(), sizeof(double2) * n, hipMemcpyHostToDevice, c.stream));
+    LSD_HIP(hipStreamSynchronize(c.stream));
+    return LSD_OK;
+}
+
+int lsd_get_ic(lsd_ctx* ctx, int which, double* re, double* im) {
+    CTX_OR_FAIL(ctx);
+    if (!c.ic() || which < 0 || which > 1 || !re || !im) {
+        lsd::set_error("lsd_get_ic: no conducting inner core, bad field or null pointer");
+        return LSD_E_ARG;
+    }
+    const size_t n = (size_t)c.H * c.Nic;
+    std::vector<double2> tmp(n);
+    LSD_HIP(hipMemcpyAsync(tmp.data(), c.d_icf[which], sizeof(double2) * n, hipMemcpyDeviceToHost, c.stream));
+    LSD_HIP(hipStreamSynchronize(c.stream));
+    for (size_t q = 0; q < n; q++) {
+        re[q] = tmp[q].x;
+        im[q] = tmp[q].y;
+    }
+    return LSD_OK;
+}
+
+int lsd_get_rotation(lsd_ctx* ctx, double* out) {
+    CTX_OR_FAIL(ctx);
+    if (!out) {
+        lsd::set_error("lsd_get_rotation: null output");
+        return LSD_E_ARG;
+    }
+    double r[3];
+    LSD_HIP(hipMemcpyAsync(r, c.d_rot, sizeof(double) * 3, hipMemcpyDeviceToHost, c.stream));
+    LSD_HIP(hipStreamSynchronize(c.stream));
+    for (int k = 0; k < 3; k++) out[k] = r[k];
+    return LSD_OK;
+}
+
+int lsd_set_rotation(lsd_ctx* ctx, double omega) {
+    CTX_OR_FAIL(ctx);
+    LSD_HIP(hipMemcpyAsync(c.d_rot + lsd::ROT_OMEGA, &omega, sizeof(double), hipMemcpyHostToDevice, c.stream));
     LSD_HIP(hipStreamSynchronize(c.stream));
     return LSD_OK;
 }

@@ -239,6 +239,28 @@ struct Ctx {
     double* d_cfl = nullptr;           // [1] tim_cfl_dt
     double* h_cfl = nullptr;           // pinned copy
 
+    // conducting inner core (cfg.Nic > 1) and the inner-core rotation
+    // (cfg.rot), modules/magnetic.F90, modules/rotation.F90
+    int Nic = 1, NT = 0;               // inner-core points; coupled band length Nic + N - 1
+    double* d_icr = nullptr;           // [7][Nic] inner-core r^p, p = -3..3
+    double* d_icdr1 = nullptr;         // [Nic][7]
+    double* d_icdr2 = nullptr;         // [Nic][7]
+    double* d_icradlap = nullptr;      // [Nic][7]
+    double* d_iclu = nullptr;          // [2][L][NT][10] coupled factors of BT, BP
+    int* d_icpiv = nullptr;            // [2][L][NT]
+    double* d_icy = nullptr;           // [L][Nic][7] inner-core Y
+    double2* d_icf[2] = {};            // icBT, icBP [H][Nic]
+    double2* d_icn[2] = {};            // their N
+    double2* d_ichist[2] = {};         // their saved N
+    int4* d_icblk = nullptr;           // coupled-solve blocks (degree, first, count)
+    int* d_icord = nullptr;            // [H] harmonics grouped by degree
+    int n_icblk = 0, ic_hb = 0;
+    double* d_icpartial = nullptr;     // [4][n_icblk] correction norms: BT, icBT, BP, icBP
+    double* h_icpartial = nullptr;     // pinned copy
+    double* d_rot = nullptr;           // [ROT_COUNT], see RotSlot
+    double* d_gwsin = nullptr;         // [Th] leg_gwP0 sin(theta)
+    bool ic() const { return Nic > 1 && cfg.mag; }
+
     size_t coll_elems() const { return (size_t)H * N; }
     size_t phys_elems() const { return (size_t)Th * Ph * N; }
     size_t fourier_elems() const { return (size_t)N * Th * MP; }
@@ -260,6 +282,21 @@ int precompute_tables(Ctx& c);
 int tables_to_f32(Ctx& c);
 int tables_ana4(Ctx& c);
 int precompute_mesh(Ctx& c);
+
+// Inner core and rotation (precompute.hip set-up, innercore.hip kernels).
+// d_rot holds the rotation state and its constants.
+enum RotSlot {
+    ROT_OMEGA = 0, ROT_VTORQ, ROT_MTORQ, ROT_VTORQ_S, ROT_MTORQ_S,  // rot_omega, torques, saved torques
+    ROT_INERTIA, ROT_RI, ROT_VFAC, ROT_MFAC, ROT_DT, ROT_COUNT
+};
+int precompute_innercore(Ctx& c);
+int build_ic_operators(Ctx& c);
+size_t ic_smem_bytes(int N, int Nic, int S);
+int ic_block_harmonics(int N, int Nic);
+int launch_ic_solve(Ctx& c, bool pred);
+int launch_torques(Ctx& c, bool f32);
+int launch_ic_nonlinear(Ctx& c);
+int launch_rot_update(Ctx& c, bool pred);
 int build_operators(Ctx& c);
 
 // legendre.hip

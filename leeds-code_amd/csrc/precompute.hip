@@ -640,6 +640,182 @@ int tables_ana4(Ctx& c) {
     return LSD_OK;
 }
 
+// ------------------------------------------------------------ inner core
+
+// Inner-core points r = r_i cos(pi (2 Nic - n) / (2 Nic - 1)),
+// n = Nic+1 .. 2 Nic (modules/meshs.F90:165-169), the outer half of a
+// Chebyshev grid over the diameter, with their powers r^p, p = -3..3.
+__global__ void k_icgrid(int Ni, double rratio, double* __restrict__ r) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (k > Ni) return;
+    const double PI = 3.1415926535897931;
+    const double ri = rratio / (1.0 - rratio);
+    const int N_ = 2 * Ni, n = k + Ni;
+    const double r1 = ri * cos(PI * (N_ - n) / (double)(N_ - 1));
+    for (int p = -3; p <= 3; p++) r[(p + 3) * Ni + k - 1] = (p == 1) ? r1 : ipow_d(r1, p);
+}
+
+// rot_precompute (modules/rotation.F90:20-36): the inertia (8 pi / 15)
+// r_i^5; the torque factors of non_rotation (modules/nonlinear.F90:497-499)
+// r_i^3 Pm and r_i^3 2 Pm / E; and per theta leg_gwP0 sin(theta)
+// (modules/legendre.F90:103-104: P_0^0 gw / 2, mirrored).
+__global__ void k_rot_setup(int N, int Th, int Th2, const double* __restrict__ r, const double* __restrict__ gw,
+                            const double* __restrict__ sn, const double* __restrict__ p00, double Pm, double E,
+                            double* __restrict__ rot, double* __restrict__ gwsin) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < Th) {
+        const int jj = j < Th2 ? j : Th - 1 - j;
+        gwsin[j] = (p00[jj] * gw[jj] * 0.5) * sn[j];
+    }
+    if (j == 0) {
+        const double PI = 3.1415926535897931;
+        const double ri = r[4 * N], ri3 = r[6 * N];
+        rot[ROT_OMEGA] = 0.0;
+        rot[ROT_VTORQ] = rot[ROT_MTORQ] = rot[ROT_VTORQ_S] = rot[ROT_MTORQ_S] = 0.0;
+        rot[ROT_INERTIA] = (8.0 * PI / 15.0) * ipow_d(ri, 5);
+        rot[ROT_RI] = ri;
+        rot[ROT_VFAC] = ri3 * Pm;
+        rot[ROT_MFAC] = ri3 * 2 * Pm / E;
+    }
+}
+
+int precompute_innercore(Ctx& c) {
+    LSD_HIP(hipMemsetAsync(c.d_rot, 0, sizeof(double) * ROT_COUNT, c.stream));
+    hipLaunchKernelGGL(k_rot_setup, dim3((c.Th + 63) / 64), dim3(64), 0, c.stream, c.N, c.Th, c.Th2, c.d_r, c.d_gw,
+                       c.d_sin, c.d_tab, c.cfg.Pm, c.cfg.E, c.d_rot, c.d_gwsin);
+    LSD_HIP(hipGetLastError());
+    if (c.Nic <= 1) return LSD_OK;
+    const int Ni = c.Nic;
+    hipLaunchKernelGGL(k_icgrid, dim3((Ni + 63) / 64), dim3(64), 0, c.stream, Ni, c.cfg.rratio, c.d_icr);
+    double *ainv = nullptr, *r1dr = nullptr, *w = nullptr;
+    int* info = nullptr;
+    AsyncScratch s_ainv(c.stream), s_info(c.stream), s_r1dr(c.stream), s_w(c.stream);
+    LSD_HIP(s_ainv.alloc(&ainv, sizeof(double) * 49 * Ni));
+    LSD_HIP(s_info.alloc(&info, sizeof(int)));
+    LSD_HIP(s_r1dr.alloc(&r1dr, sizeof(double) * 7 * Ni));
+    LSD_HIP(s_w.alloc(&w, sizeof(double) * Ni));
+    LSD_HIP(hipMemsetAsync(info, 0, sizeof(int), c.stream));
+    LSD_HIP(hipMemsetAsync(c.d_icdr1, 0, sizeof(double) * 7 * Ni, c.stream));
+    LSD_HIP(hipMemsetAsync(c.d_icdr2, 0, sizeof(double) * 7 * Ni, c.stream));
+    hipLaunchKernelGGL(k_fd, dim3((Ni + 63) / 64), dim3(64), 0, c.stream, Ni, c.d_icr, c.d_icdr1, c.d_icdr2, ainv,
+                       info);
+    hipLaunchKernelGGL(k_mesh_ops, dim3((Ni + 63) / 64), dim3(64), 0, c.stream, Ni, c.d_icr, c.d_icdr1, c.d_icdr2,
+                       ainv, c.d_icradlap, r1dr, w);
+    LSD_HIP(hipGetLastError());
+    int h_info = 0;
+    LSD_HIP(hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    LSD_HIP(hipStreamSynchronize(c.stream));
+    if (h_info) {
+        set_error("matrix inversion error 1");
+        return LSD_E_SINGULAR;
+    }
+    return LSD_OK;
+}
+
+// One work-item per (field, degree l >= 1): tim_iclumesh_Tor_X /
+// tim_iclumesh_Pol_X (modules/timestep.F90:218-344) for Nic > 1 with the BC
+// rows of mag_bc_Tor / mag_bc_Pol (modules/magnetic.F90:137-282), Boussinesq
+// without a diffusivity profile (MagTorLap = MagPolLap = radLap): one band
+// over the inner-core points 1..Nic and the outer-core points Nic..NT (the
+// inner-core boundary is point Nic of both); row 1 regular at the innermost
+// point, row Nic the continuity of dB/dr across the boundary, row NT
+// insulating; factorised in place.  Field 0 also writes the inner core's
+// explicit operator Y = c1/dt + (1-c) (radLap - l(l+1)/r^2)
+// (timestep.F90:421-449).
+__global__ void k_icoperators(int L, int N, int Ni, OpParams op, const double* __restrict__ r,
+                              const double* __restrict__ radlap, const double* __restrict__ dr1,
+                              const double* __restrict__ icr, const double* __restrict__ icradlap,
+                              const double* __restrict__ icdr1, double* __restrict__ iclu, int* __restrict__ icpiv,
+                              double* __restrict__ icy, int* __restrict__ info) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    const int kind = blockIdx.y;
+    if (l >= L) return;
+    const int NT = Ni + N - 1;
+    double* ab = iclu + ((size_t)kind * L + l) * NT * LSD_LDAB;
+    int* pv = icpiv + ((size_t)kind * L + l) * NT;
+    if (l == 0) {
+        info[kind * L] = 0;
+        return;
+    }
+    const double c1_ = 1.0 / op.dt;
+    double c2_ = 1.0 * op.implicit;
+    if (op.implicit == 0.0) c2_ = 1.0;
+    const double ll1 = (double)(l * (l + 1));
+    const double* icrm2 = icr + 1 * Ni;
+    const double* icrm1 = icr + 2 * Ni;
+    const double* rm2 = r + 1 * N;
+    const double* rm1 = r + 2 * N;
+    for (int q = 0; q < NT * LSD_LDAB; q++) ab[q] = 0.0;
+    for (int j = 0; j < Ni; j++)
+        for (int q = 0; q < 7; q++) ab[j * LSD_LDAB + LSD_KL + q] = -c2_ * icradlap[j * 7 + q];
+    for (int j = 0; j < Ni; j++) {
+        double* d = &ab[j * LSD_LDAB + 2 * LSD_KL];
+        *d = (*d + c1_) + c2_ * ll1 * icrm2[j];
+    }
+    for (int j = 0; j < N; j++)
+        for (int q = 0; q < 7; q++)
+            ab[(Ni - 1 + j) * LSD_LDAB + LSD_KL + q] = ab[(Ni - 1 + j) * LSD_LDAB + LSD_KL + q] - c2_ * radlap[j * 7 + q];
+    for (int j = 0; j < N; j++) {
+        double* d = &ab[(Ni - 1 + j) * LSD_LDAB + 2 * LSD_KL];
+        *d = (*d + c1_) + c2_ * ll1 * rm2[j];
+    }
+#define LUA(i, j) ab[((j) - 1) * LSD_LDAB + (2 * LSD_KL + (i) - (j))]
+#define DR1(i, j) dr1[((j) - 1) * 7 + (LSD_KL + (i) - (j))]
+#define IDR1(i, j) icdr1[((j) - 1) * 7 + (LSD_KL + (i) - (j))]
+    for (int j = 1; j <= 1 + LSD_KL; j++) LUA(1, j) = 0.0;
+    for (int j = NT - LSD_KL; j <= NT; j++) LUA(NT, j) = 0.0;
+    for (int j = Ni - LSD_KL; j <= Ni + LSD_KL; j++) LUA(Ni, j) = 0.0;
+    if (kind == 0) {
+        LUA(1, 1) = 1.0;
+        LUA(NT, NT) = 1.0;
+    } else {
+        for (int j = 1; j <= 1 + LSD_KL; j++) LUA(1, j) = IDR1(1, j);
+        LUA(1, 1) = LUA(1, 1) - icrm1[0] * l;
+        for (int j = N - LSD_KL; j <= N; j++) LUA(NT, j + Ni - 1) = DR1(N, j);
+        LUA(NT, NT) = LUA(NT, NT) + rm1[N - 1] * (l + 1);
+    }
+    for (int j = 2; j <= 1 + LSD_KL; j++) LUA(Ni, j + Ni - 1) = DR1(1, j);
+    for (int j = Ni - LSD_KL; j <= Ni - 1; j++) LUA(Ni, j) = -IDR1(Ni, j);
+    LUA(Ni, Ni) = DR1(1, 1) - IDR1(Ni, Ni);
+#undef LUA
+#undef DR1
+#undef IDR1
+    info[kind * L + l] = gbtrf_d(NT, ab, pv);
+    if (kind != 0) return;
+    double* y = icy + (size_t)l * Ni * 7;
+    const double c2y = 1.0 * (1.0 - op.implicit);
+    for (int j = 0; j < Ni; j++) {
+        for (int q = 0; q < 7; q++) y[j * 7 + q] = c2y * icradlap[j * 7 + q];
+        double* d = &y[j * 7 + LSD_KL];
+        *d = (*d + c1_) - c2y * (ll1 * icrm2[j]);
+    }
+}
+
+int build_ic_operators(Ctx& c) {
+    if (c.cfg.rot || c.Nic > 1)
+        LSD_HIP(hipMemcpyAsync(c.d_rot + ROT_DT, &c.dt, sizeof(double), hipMemcpyHostToDevice, c.stream));
+    if (!c.ic()) {
+        LSD_HIP(hipStreamSynchronize(c.stream));
+        return LSD_OK;
+    }
+    OpParams op{c.dt, c.cfg.implicit, c.cfg.Pm, c.cfg.Pr, c.cfg.Sc};
+    int* info = nullptr;
+    AsyncScratch s_info(c.stream);
+    LSD_HIP(s_info.alloc(&info, sizeof(int) * 2 * c.L));
+    hipLaunchKernelGGL(k_icoperators, dim3((c.L + 63) / 64, 2), dim3(64), 0, c.stream, c.L, c.N, c.Nic, op, c.d_r,
+                       c.d_radlap, c.d_dr1, c.d_icr, c.d_icradlap, c.d_icdr1, c.d_iclu, c.d_icpiv, c.d_icy, info);
+    LSD_HIP(hipGetLastError());
+    std::vector<int> h(2 * c.L);
+    LSD_HIP(hipMemcpyAsync(h.data(), info, sizeof(int) * 2 * c.L, hipMemcpyDeviceToHost, c.stream));
+    LSD_HIP(hipStreamSynchronize(c.stream));
+    for (int v : h)
+        if (v) {
+            set_error("mes_iclu_find");
+            return LSD_E_SINGULAR;
+        }
+    return LSD_OK;
+}
+
 int build_operators(Ctx& c) {
     OpParams op{c.dt, c.cfg.implicit, c.cfg.Pm, c.cfg.Pr, c.cfg.Sc};
     int L = c.L, N = c.N;
@@ -656,6 +832,8 @@ int build_operators(Ctx& c) {
     hipLaunchKernelGGL(k_green, dim3((L + 63) / 64), dim3(64), 0, c.stream, L, N, c.d_lu, c.d_piv, c.d_gre,
                        c.d_invg, ginfo, scratch);
     LSD_HIP(hipGetLastError());
+    int rc_ic = build_ic_operators(c);
+    if (rc_ic) return rc_ic;
     std::vector<int> info(NLU * L), gi(L);
     LSD_HIP(hipMemcpyAsync(info.data(), c.d_info, sizeof(int) * NLU * L, hipMemcpyDeviceToHost, c.stream));
     LSD_HIP(hipMemcpyAsync(gi.data(), ginfo, sizeof(int) * L, hipMemcpyDeviceToHost, c.stream));

@@ -28,56 +28,12 @@
 
 namespace lsd {
 
-// Lane view of one real column in LDS.
-template <typename T>
-struct ColT {
-    T* w;  // column base, stride S
-    int S, N;
-};
-typedef ColT<double> Col;
+#define RC_SOLVE(x)        \
+    do {                   \
+        int r_ = (x);      \
+        if (r_) return r_; \
+    } while (0)
 
-// dgbtrs('N') on one column (timestep.F90:566-588 -> LAPACK dgbtrs with
-// KL = 3, KU = 3, LDAB = 10): row interchanges and L^-1 forward, then the
-// upper band solve (dtbsv) backward.  ab = LDS factors of this (kind, l),
-// entry AB(i,j) (1-based) at ab[(j-1)*10 + i-1]; ip = LDS pivots.
-template <typename T>
-__device__ void gbtrs_col(const ColT<T>& c, const T* ab, const int* ip) {
-    const int N = c.N, S = c.S;
-    T* w = c.w;
-    T r0 = w[0], r1 = w[S], r2 = w[2 * S], r3 = w[3 * S];
-    for (int j = 0; j < N - 1; j++) {
-        const int p = ip[j] - 1 - j;  // 0..3 (pivot inside the band)
-        const T* a = ab + j * LSD_LDAB + 2 * LSD_KL + 1;
-        const T a0 = a[0], a1 = a[1], a2 = a[2];
-        T t = r0;
-        if (p == 1) { r0 = r1; r1 = t; }
-        else if (p == 2) { r0 = r2; r2 = t; }
-        else if (p == 3) { r0 = r3; r3 = t; }
-        const T yv = r0;
-        w[j * S] = yv;
-        r0 = r1 - a0 * yv;
-        r1 = r2 - a1 * yv;
-        r2 = r3 - a2 * yv;
-        r3 = j + 4 < N ? w[(j + 4) * S] : T(0);
-    }
-    // registers beyond row N-1 above only ever held rows that are never stored
-    T b[7];
-    b[0] = r0;
-    for (int d = 1; d < 7; d++) b[d] = w[(N - 1 - d) * S];
-    for (int j = N - 1; j >= 0; j--) {
-        const T* a = ab + j * LSD_LDAB;
-        T u[7];
-#pragma unroll
-        for (int d = 0; d < 7; d++) u[d] = a[2 * LSD_KL - d];
-        const T v = b[0] / u[0];
-        w[j * S] = v;
-#pragma unroll
-        for (int d = 1; d < 7; d++) b[d] = b[d] - v * u[d];
-#pragma unroll
-        for (int d = 0; d < 6; d++) b[d] = b[d + 1];
-        b[6] = j - 7 >= 0 ? w[(j - 7) * S] : T(0);
-    }
-}
 
 // Mixed forms: x = X^-1 b with FP32 factors and one FP64-residual sweep.
 // c holds b on entry and x on exit; c32 is the lane's FP32 work column;
@@ -106,28 +62,6 @@ __device__ void solve_refined(const Col& c, const ColT<float>& c32, const float*
     }
     gbtrs_col<float>(c32, ab32, ip);
     for (int n = 0; n < N; n++) c.w[n * S] = c.w[n * S] + (double)c32.w[n * S];
-}
-
-// w <- w + Y.x in place (tim_multY, timestep.F90:640-670): row n sums its
-// band in ascending column order; yk = LDS band, Y(n,j) = yk[j*7 + 3+n-j];
-// xs is this lane's real part of the old state column, read as a stream.
-__device__ void multy_col(const Col& c, const double* yk, const double* __restrict__ xs) {
-    const int N = c.N, S = c.S;
-    double xw[7];
-#pragma unroll
-    for (int d = 0; d < 7; d++) xw[d] = (d - 3 >= 0 && d - 3 < N) ? xs[2 * (d - 3)] : 0.0;
-    for (int n = 0; n < N; n++) {
-        double acc = c.w[n * S];
-#pragma unroll
-        for (int d = 0; d < 7; d++) {
-            const int j = n - 3 + d;
-            if (j >= 0 && j < N) acc = fma(yk[j * 7 + LSD_KL + n - j], xw[d], acc);
-        }
-        c.w[n * S] = acc;
-#pragma unroll
-        for (int d = 0; d < 6; d++) xw[d] = xw[d + 1];
-        xw[6] = n + 4 < N ? xs[2 * (n + 4)] : 0.0;
-    }
 }
 
 // vel_gre_adjust (velocity.F90:445-465) on one column; g = LDS [2][N].
@@ -186,6 +120,8 @@ __global__ void __launch_bounds__(64) k_predict(SolveArgs a) {
         if (act) {
             if (l != 0) multy_col(c, m.y, xs);
             c.w[0] = c.w[(N - 1) * S] = 0.0;
+            // vel_setbc_Tor: the rotating inner core's wall, T_10(r_i) = rot_omega r_i
+            if (f == 0 && a.rot && l == 1 && a.hm[h] == 0 && part == 0) c.w[0] = a.rot[ROT_OMEGA] * a.rot[ROT_RI];
             if (l != 0) solve_staged<MIX>(m, c, c32);
         }
         break;
@@ -276,7 +212,8 @@ __global__ void __launch_bounds__(64) k_correct(SolveArgs a) {
         if (act) {
             c.w[0] = c.w[(N - 1) * S] = 0.0;
             // vel_setbc_Tor for (1,0) once tim_it /= 0
-            if (f == 0 && l == 1 && a.hm[h] == 0 && part == 0) c.w[0] = 0.0 - a.x[0][(size_t)h * N].x;
+            if (f == 0 && l == 1 && a.hm[h] == 0 && part == 0)
+                c.w[0] = (a.rot ? a.rot[ROT_OMEGA] * a.rot[ROT_RI] : 0.0) - a.x[0][(size_t)h * N].x;
             if (solve) solve_staged<MIX>(m, c, c32);
             if (f == 1 && solve) c.w[0] = c.w[(N - 1) * S] = 0.0;
         }
@@ -371,10 +308,12 @@ static SolveArgs make_args(Ctx& c) {
     a.on[1] = 1;
     a.on[2] = c.cfg.cod;
     a.on[3] = c.cfg.comp;
-    a.on[4] = c.cfg.mag;
-    a.on[5] = c.cfg.mag;
+    // with a conducting inner core the magnetic field has its own coupled solves
+    a.on[4] = c.cfg.mag && !c.ic();
+    a.on[5] = c.cfg.mag && !c.ic();
     a.cimp = c.cfg.implicit;
     a.hm = c.d_hm;
+    a.rot = c.cfg.rot ? c.d_rot : nullptr;
     a.blocks = c.d_sblocks;
     a.order = c.d_sorder;
     a.lu = c.d_lu;
@@ -396,7 +335,7 @@ static SolveArgs make_args(Ctx& c) {
 }
 
 static int active_fields(const Ctx& c) {
-    return 2 + (c.cfg.cod ? 1 : 0) + (c.cfg.comp ? 1 : 0) + (c.cfg.mag ? 2 : 0);
+    return 2 + (c.cfg.cod ? 1 : 0) + (c.cfg.comp ? 1 : 0) + (c.cfg.mag && !c.ic() ? 2 : 0);
 }
 
 static size_t solve_lds(const Ctx& c) { return solve_lds_bytes(c); }
@@ -405,7 +344,11 @@ static size_t solve_lds(const Ctx& c) { return solve_lds_bytes(c); }
 static bool use_direct(const Ctx& c) { return c.solve_direct_ok && c.var[LSD_V_SOLVE] == 0; }
 
 int launch_predictor(Ctx& c) {
-    if (use_direct(c)) return launch_solve_direct(c, true);
+    RC_SOLVE(launch_rot_update(c, true));
+    if (use_direct(c)) {
+        RC_SOLVE(launch_solve_direct(c, true));
+        return c.ic() ? launch_ic_solve(c, true) : LSD_OK;
+    }
     SolveArgs a = make_args(c);
     // per field: read state and N, write state and saved N
     ProfScope ps(c, LSD_K_SOLVE, 4.0 * 16.0 * c.coll_elems() * active_fields(c));
@@ -413,13 +356,14 @@ int launch_predictor(Ctx& c) {
     if (c.mixed) hipLaunchKernelGGL(k_predict<true>, grid, dim3(64), solve_lds(c), c.stream, a);
     else hipLaunchKernelGGL(k_predict<false>, grid, dim3(64), solve_lds(c), c.stream, a);
     LSD_HIP(hipGetLastError());
-    return LSD_OK;
+    return c.ic() ? launch_ic_solve(c, true) : LSD_OK;
 }
 
 // Queues the corrector and the copy of its norm partials to pinned host
 // memory; every enabled field's slots are written by every block, so no
 // clearing is needed (and none is recorded when the step is captured).
 int launch_corrector_async(Ctx& c) {
+    RC_SOLVE(launch_rot_update(c, false));
     SolveArgs a = make_args(c);
     const int nb = c.n_partial_blocks;
     dim3 grid(nb, 6);
@@ -437,7 +381,7 @@ int launch_corrector_async(Ctx& c) {
     }
     LSD_HIP(hipGetLastError());
     LSD_HIP(hipMemcpyAsync(c.h_partial, c.d_partial, sizeof(double) * 8 * nb, hipMemcpyDeviceToHost, c.stream));
-    return LSD_OK;
+    return c.ic() ? launch_ic_solve(c, false) : LSD_OK;
 }
 
 // After the stream has drained: the sum of the per-field correction norms in
@@ -451,7 +395,12 @@ double corrector_norms(const Ctx& c) {
                   (slot >= 4 && c.cfg.mag);
         if (!on) continue;
         double s = 0.0;
-        for (int b = 0; b < nb; b++) s += c.h_partial[(size_t)slot * nb + b];
+        if (slot >= 4 && c.ic()) {
+            // the coupled inner-core solves' norms: BT, icBT, BP, icBP
+            for (int b = 0; b < c.n_icblk; b++) s += c.h_icpartial[(size_t)(slot - 4) * c.n_icblk + b];
+        } else {
+            for (int b = 0; b < nb; b++) s += c.h_partial[(size_t)slot * nb + b];
+        }
         tot += sqrt(s);
     }
     return tot;

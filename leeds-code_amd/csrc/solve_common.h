@@ -24,6 +24,7 @@ struct SolveArgs {
     double2* nn[6];      // current N
     double2* hist[6];    // saved N
     double* partial;     // [8][gridDim.x]
+    const double* rot;   // inner-core rotation state (RotSlot), null without b_rot_tstep
 };
 
 // LDS of one block.  Columns first, then the operator data of one (kind, l).
@@ -103,5 +104,78 @@ __device__ inline void stage_g(const SolveArgs& a, const Smem& m, int l) {
     for (int i = threadIdx.x; i < 2 * a.N; i += 64) m.g[i] = a.gre[off + i];
 }
 
+
+// Lane view of one real column in LDS.
+template <typename T>
+struct ColT {
+    T* w;  // column base, stride S
+    int S, N;
+};
+typedef ColT<double> Col;
+
+// dgbtrs('N') on one column (timestep.F90:566-588 -> LAPACK dgbtrs with
+// KL = 3, KU = 3, LDAB = 10): row interchanges and L^-1 forward, then the
+// upper band solve (dtbsv) backward.  ab = LDS factors of this (kind, l),
+// entry AB(i,j) (1-based) at ab[(j-1)*10 + i-1]; ip = LDS pivots.
+template <typename T>
+__device__ void gbtrs_col(const ColT<T>& c, const T* ab, const int* ip) {
+    const int N = c.N, S = c.S;
+    T* w = c.w;
+    T r0 = w[0], r1 = w[S], r2 = w[2 * S], r3 = w[3 * S];
+    for (int j = 0; j < N - 1; j++) {
+        const int p = ip[j] - 1 - j;  // 0..3 (pivot inside the band)
+        const T* a = ab + j * LSD_LDAB + 2 * LSD_KL + 1;
+        const T a0 = a[0], a1 = a[1], a2 = a[2];
+        T t = r0;
+        if (p == 1) { r0 = r1; r1 = t; }
+        else if (p == 2) { r0 = r2; r2 = t; }
+        else if (p == 3) { r0 = r3; r3 = t; }
+        const T yv = r0;
+        w[j * S] = yv;
+        r0 = r1 - a0 * yv;
+        r1 = r2 - a1 * yv;
+        r2 = r3 - a2 * yv;
+        r3 = j + 4 < N ? w[(j + 4) * S] : T(0);
+    }
+    // registers beyond row N-1 above only ever held rows that are never stored
+    T b[7];
+    b[0] = r0;
+    for (int d = 1; d < 7; d++) b[d] = w[(N - 1 - d) * S];
+    for (int j = N - 1; j >= 0; j--) {
+        const T* a = ab + j * LSD_LDAB;
+        T u[7];
+#pragma unroll
+        for (int d = 0; d < 7; d++) u[d] = a[2 * LSD_KL - d];
+        const T v = b[0] / u[0];
+        w[j * S] = v;
+#pragma unroll
+        for (int d = 1; d < 7; d++) b[d] = b[d] - v * u[d];
+#pragma unroll
+        for (int d = 0; d < 6; d++) b[d] = b[d + 1];
+        b[6] = j - 7 >= 0 ? w[(j - 7) * S] : T(0);
+    }
+}
+
+// w <- w + Y.x in place (tim_multY, timestep.F90:640-670): row n sums its
+// band in ascending column order; yk = LDS band, Y(n,j) = yk[j*7 + 3+n-j];
+// xs is this lane's real part of the old state column, read as a stream.
+__device__ inline void multy_col(const Col& c, const double* yk, const double* __restrict__ xs) {
+    const int N = c.N, S = c.S;
+    double xw[7];
+#pragma unroll
+    for (int d = 0; d < 7; d++) xw[d] = (d - 3 >= 0 && d - 3 < N) ? xs[2 * (d - 3)] : 0.0;
+    for (int n = 0; n < N; n++) {
+        double acc = c.w[n * S];
+#pragma unroll
+        for (int d = 0; d < 7; d++) {
+            const int j = n - 3 + d;
+            if (j >= 0 && j < N) acc = fma(yk[j * 7 + LSD_KL + n - j], xw[d], acc);
+        }
+        c.w[n * S] = acc;
+#pragma unroll
+        for (int d = 0; d < 6; d++) xw[d] = xw[d + 1];
+        xw[6] = n + 4 < N ? xs[2 * (n + 4)] : 0.0;
+    }
+}
 
 }  // namespace lsd

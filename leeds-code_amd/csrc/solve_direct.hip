@@ -52,6 +52,7 @@ struct DirArgs {
     double2* rhs[6];     // [H][N] predictor right-hand sides
     double2* rhs2;       // poloidal velocity: the Y x part of the XPol right-hand side
     double* partial;     // [8][gridDim.x]
+    const double* rot;   // inner-core rotation state (RotSlot), null without b_rot_tstep
 };
 
 // v[f] for a field index that is the same across the workgroup, as scalar
@@ -169,6 +170,8 @@ __global__ void __launch_bounds__(RHS_BLK) k_rhs_pred(DirArgs a, float inv_n) {
             b.x = keep * b.x;
             b.y = keep * b.y;
             if ((f == 2 || f == 3) && h == 0 && n == 0) b.x = 1.0;
+            // vel_setbc_Tor: the rotating inner core's wall, T_10(r_i) = rot_omega r_i
+            if (f == 0 && a.rot && l == 1 && n == 0 && a.hm[h] == 0) b.x = a.rot[ROT_OMEGA] * a.rot[ROT_RI];
             rhs[idx] = b;
         }
     }
@@ -424,7 +427,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
             BATCH_FENCE();
         }
         // vel_setbc_Tor for (1, 0) once tim_it /= 0
-        if (f == 0 && l == 1 && a.hm[h] == 0 && part == 0) b[0] = 0.0 - xcol[0].x;
+        if (f == 0 && l == 1 && a.hm[h] == 0 && part == 0)
+            b[0] = (a.rot ? a.rot[ROT_OMEGA] * a.rot[ROT_RI] : 0.0) - xcol[0].x;
         if (solve) gbtrs_reg<N, double>(b, s.ab, s.rd, pivots(a, lu_kind(f), l));
         if (f == 1 && solve) {
             b[0] = 0.0;
@@ -505,9 +509,11 @@ static DirArgs dir_args(Ctx& c) {
     a.on[1] = 1;
     a.on[2] = c.cfg.cod;
     a.on[3] = c.cfg.comp;
-    a.on[4] = c.cfg.mag;
-    a.on[5] = c.cfg.mag;
+    // with a conducting inner core the magnetic field has its own coupled solves
+    a.on[4] = c.cfg.mag && !c.ic();
+    a.on[5] = c.cfg.mag && !c.ic();
     a.cimp = c.cfg.implicit;
+    a.rot = c.cfg.rot ? c.d_rot : nullptr;
     a.hl = c.d_hl;
     a.hm = c.d_hm;
     a.blocks = c.d_sblocks;
@@ -558,7 +564,7 @@ static int launch_sdir_n(Ctx& c, const DirArgs& a) {
 // writes the saved N and the state.
 int launch_solve_direct(Ctx& c, bool pred) {
     const DirArgs a = dir_args(c);
-    const int nf = 2 + (c.cfg.cod ? 1 : 0) + (c.cfg.comp ? 1 : 0) + (c.cfg.mag ? 2 : 0);
+    const int nf = 2 + (c.cfg.cod ? 1 : 0) + (c.cfg.comp ? 1 : 0) + (c.cfg.mag && !c.ic() ? 2 : 0);
     ProfScope ps(c, LSD_K_SOLVE, (pred ? 6.0 : 5.0) * 16.0 * c.coll_elems() * nf);
     if (pred) {
         const size_t lds = sizeof(double2) * (size_t)c.solve_hb * c.N + sizeof(double) * 7 * (size_t)c.N;

@@ -30,7 +30,7 @@ EXPORTS = (
     "lsd_read_state", "lsd_write_state", "lsd_load_state", "lsd_save_state",
     "lsd_set_variant", "lsd_get_variant", "lsd_tune", "lsd_sht_roundtrip", "lsd_tra_spec2phys", "lsd_tra_phys2spec", "lsd_tra_grad",
     "lsd_tra_qst2rtp", "lsd_tra_rtp2qst", "lsd_sync", "lsd_profile_begin", "lsd_profile_end", "lsd_set_graph", "lsd_energy", "lsd_energy_noalloc",
-    "lsd_step_many",
+    "lsd_step_many", "lsd_set_ic", "lsd_get_ic", "lsd_get_rotation", "lsd_set_rotation",
     "lsd_mlp_create", "lsd_mlp_destroy", "lsd_mlp_num_params", "lsd_mlp_get_params", "lsd_mlp_set_params",
     "lsd_mlp_forward", "lsd_mlp_grad", "lsd_mlp_train",
 )
@@ -70,8 +70,10 @@ def load(path=LIB_PATH):
     lib.lsd_create.argtypes = [ctypes.POINTER(LsdConfig), ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
     lib.lsd_destroy.argtypes = [ctypes.c_void_p]
     lib.lsd_destroy.restype = None
-    for n in ("lsd_set_coll", "lsd_get_coll"):
+    for n in ("lsd_set_coll", "lsd_get_coll", "lsd_set_ic", "lsd_get_ic"):
         getattr(lib, n).argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+    lib.lsd_get_rotation.argtypes = [ctypes.c_void_p, _dp]
+    lib.lsd_set_rotation.argtypes = [ctypes.c_void_p, ctypes.c_double]
     for n in ("lsd_get_phys", "lsd_get_table", "lsd_get_mesh"):
         getattr(lib, n).argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
     for n in ("lsd_transform", "lsd_nonlinear", "lsd_predictor", "lsd_sync"):
@@ -181,9 +183,48 @@ class Context:
         return re, im
 
     def set_state(self, state):
+        """Fields by name (uT .. BP), and where the state has them the
+        inner-core fields icBT, icBP and the rotation rate omega."""
         from .initial import FIELD_ID
-        for name, (re, im) in state.items():
-            self.set_coll(FIELD_ID[name], re, im)
+        for name, v in state.items():
+            if name == "omega":
+                self.set_rotation(v)
+            elif name in ("icBT", "icBP"):
+                self.set_ic(0 if name == "icBT" else 1, *v)
+            else:
+                self.set_coll(FIELD_ID[name], *v)
+
+    def get_named(self, name):
+        if name in ("icBT", "icBP"):
+            return self.get_ic(0 if name == "icBT" else 1)
+        from .initial import FIELD_ID
+        return self.get_coll(FIELD_ID[name])
+
+    # --- conducting inner core and its rotation
+    @property
+    def Nic(self):
+        return max(1, self.cfg.Nic)
+
+    def set_ic(self, which, re, im):
+        re = np.ascontiguousarray(re, dtype=np.float64)
+        im = np.ascontiguousarray(im, dtype=np.float64)
+        assert re.size == self.H * self.Nic and im.size == self.H * self.Nic
+        self._chk(self.lib.lsd_set_ic(self.h, which, _p(re), _p(im)), "lsd_set_ic")
+
+    def get_ic(self, which):
+        re = np.zeros((self.H, self.Nic))
+        im = np.zeros((self.H, self.Nic))
+        self._chk(self.lib.lsd_get_ic(self.h, which, _p(re), _p(im)), "lsd_get_ic")
+        return re, im
+
+    def rotation(self):
+        """(rot_omega, rot_velTorq, rot_magTorq)"""
+        out = np.zeros(3)
+        self._chk(self.lib.lsd_get_rotation(self.h, _p(out)), "lsd_get_rotation")
+        return out
+
+    def set_rotation(self, omega):
+        self._chk(self.lib.lsd_set_rotation(self.h, float(omega)), "lsd_set_rotation")
 
     def get_phys(self, field):
         p = np.zeros((self.N, self.Ph, self.Th))

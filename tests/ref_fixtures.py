@@ -55,6 +55,23 @@ def dterr_ok(fx, k, dterr):
     return abs(dterr - ref) <= tol
 
 
+def rotation_errors(fx, k, got):
+    """Keys of rot_omega, rot_velTorq, rot_magTorq after step k (got, in
+    that order) that miss the fixture: 1e-11 relative, plus three times the
+    largest relative spread of the reference itself over steps 1..k where
+    the fixture records it (the rate integrates the torques)."""
+    bad = []
+    for v, key in zip(got, ("omega", "velTorq", "magTorq")):
+        ref = float(fx[key][k - 1])
+        rel = 1e-11
+        if key + "_spread" in fx:
+            sp = fx[key + "_spread"][:k] / np.maximum(np.abs(fx[key][:k]), 1e-300)
+            rel += 3.0 * float(sp.max())
+        if not abs(v - ref) <= rel * abs(ref) + 1e-300:
+            bad.append((key, float(v), ref))
+    return bad
+
+
 def step_bound(fx, case, k):
     """Bound on the relative per-field error after k steps."""
     b = REL_STEP[case] * k

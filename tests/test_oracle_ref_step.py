@@ -61,14 +61,8 @@ def _run_and_compare(case, nsteps):
             # the rate integrates the torques, so where the fixture records the
             # reference's own spread the bound takes its largest relative
             # value over the steps so far
-            om, vt, mt = o.rotation()
-            for got, key in ((om, "omega"), (vt, "velTorq"), (mt, "magTorq")):
-                ref = float(fx[key][k - 1])
-                rel = 1e-11
-                if key + "_spread" in fx:
-                    sp = fx[key + "_spread"][:k] / np.maximum(np.abs(fx[key][:k]), 1e-300)
-                    rel += 3.0 * float(sp.max())
-                assert abs(got - ref) <= rel * abs(ref) + 1e-300, (k, key, got, ref)
+            bad = rf.rotation_errors(fx, k, o.rotation())
+            assert not bad, (k, bad)
         if k in keep:
             errs = rf.compare(fx, case, k, o.get_named, k)
             worst = max(errs.values())
