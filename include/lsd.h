@@ -173,9 +173,10 @@ int lsd_get_rotation(lsd_ctx* ctx, double* out);
 int lsd_set_rotation(lsd_ctx* ctx, double omega);
 /* State files: io_save_state / io_load_state (program/io.F90:1502-1621,
  * 349-483) in the netCDF classic format (CDF-1 written; CDF-1 and CDF-2
- * read), variables uT, uP, C, Comp, BT, BP (and zero icBT, icBP) as
- * (ReIm, H, r) in the m-major io_state_nh order with attributes L, M, Mp,
- * the radial grid r, dt, and the global attribute t.  Reading maps each
+ * read), variables uT, uP, C, Comp, BT, BP, icBT, icBP as (ReIm, H, r)
+ * in the m-major io_state_nh order with attributes L, M, Mp, the radial
+ * grids r and icr, dt, icOmega and the global attribute t (these two
+ * calls carry no inner core: zero icBT, icBP).  Reading maps each
  * (l, m) of the run to the file's index (modes the file lacks stay zero;
  * the reference's optional d_init_pert noise is not added), interpolates
  * radially when the file's grid differs (7-point, io_interp_wts), loads C
@@ -187,7 +188,14 @@ int lsd_read_state(const char* path, const lsd_config* cfg, double* coll, double
 int lsd_write_state(const char* path, const lsd_config* cfg, const double* coll, double t, double dt);
 /* The same on a context's state: load sets the six fields and tim_t, and
  * with cfg->dt < 0 takes tim_dt from the file (refactorising the
- * operators); save writes the current fields, tim_t and tim_dt. */
+ * operators); save writes the current fields, tim_t and tim_dt.  With a
+ * conducting inner core (cfg->Nic > 1) load takes icBT, icBP from the file
+ * (rows 1..Nic of the file's inner core, regridded to this run's inner-core
+ * points when they differ) or, when the file has none, extends the outer
+ * core's field into the inner core as io_ic_extend does
+ * (program/io.F90:440-447, 1438-1477); with either Nic > 1 or cfg->rot it
+ * sets rot_omega from icOmega.  Save writes the inner-core fields, radii
+ * and rot_omega. */
 int lsd_load_state(lsd_ctx* ctx, const char* path);
 int lsd_save_state(lsd_ctx* ctx, const char* path);
 

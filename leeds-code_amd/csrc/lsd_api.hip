@@ -1241,6 +1241,53 @@ int lsd_write_state(const char* path, const lsd_config* cfg, const double* coll,
     return lsd::state_write(path, *cfg, coll, t, dt);
 }
 
+namespace lsd {
+
+// Host copy of a device array.
+static int to_host(const Ctx& c, const double* d, size_t n, std::vector<double>& out) {
+    out.resize(n);
+    LSD_HIP(hipMemcpyAsync(out.data(), d, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    LSD_HIP(hipStreamSynchronize(c.stream));
+    return LSD_OK;
+}
+
+// io_ic_extend (program/io.F90:1438-1477): an inner-core field a r^3 + b r^2
+// that meets the outer-core field at r_i in value and in the slope of the
+// first row of the FD weights dr(1); oc [2][H][N] (Re, Im), ic [2][H][Nic].
+static int ic_extend(const Ctx& c, const double* oc, double* ic) {
+    std::vector<double> dr1, r, icr;
+    int rc = to_host(c, c.d_dr1, (size_t)7 * c.N, dr1);
+    if (!rc) rc = to_host(c, c.d_r, (size_t)7 * c.N, r);
+    if (!rc) rc = to_host(c, c.d_icr, (size_t)7 * c.Nic, icr);
+    if (rc) return rc;
+    const int N = c.N, Ni = c.Nic;
+    const double ri = r[4 * N], ri2 = r[5 * N], ri3 = r[6 * N];
+    for (int part = 0; part < 2; part++)
+        for (int h = 0; h < c.H; h++) {
+            const double* o = oc + ((size_t)part * c.H + h) * N;
+            double* d = ic + ((size_t)part * c.H + h) * Ni;
+            const double f = o[0];
+            double f1 = 0.0;
+            for (int j = 1; j <= 1 + LSD_KL; j++) f1 = f1 + dr1[(j - 1) * 7 + (LSD_KL + 1 - j)] * o[j - 1];
+            const double a = (f1 * ri - 2.0 * f) / ri3;
+            const double b = (f - a * ri3) / ri2;
+            for (int n = 0; n < Ni; n++) d[n] = a * icr[6 * Ni + n] + b * icr[5 * Ni + n];
+        }
+    return LSD_OK;
+}
+
+static StateIc state_ic(const Ctx& c) {
+    StateIc ic;
+    ic.Nic = c.ic() ? c.Nic : 1;
+    if (c.ic()) {
+        std::vector<double> icr;
+        if (!to_host(c, c.d_icr, (size_t)7 * c.Nic, icr)) ic.ric.assign(icr.begin() + 4 * c.Nic, icr.begin() + 5 * c.Nic);
+    }
+    return ic;
+}
+
+}  // namespace lsd
+
 int lsd_load_state(lsd_ctx* ctx, const char* path) {
     CTX_OR_FAIL(ctx);
     if (c.it != 0) {
@@ -1250,10 +1297,32 @@ int lsd_load_state(lsd_ctx* ctx, const char* path) {
     const size_t HN = c.coll_elems();
     std::vector<double> buf(12 * HN);
     double t = 0.0, dt = 0.0;
-    int rc = lsd::state_read(path, c.cfg, buf.data(), &t, &dt);
+    lsd::StateIc ic = lsd::state_ic(c);
+    int rc = lsd::state_read(path, c.cfg, buf.data(), &t, &dt, &ic);
     if (rc) return rc;
     for (int k = 0; k < 6; k++) {
         rc = lsd_set_coll(ctx, k, buf.data() + (size_t)(2 * k) * HN, buf.data() + (size_t)(2 * k + 1) * HN);
+        if (rc) return rc;
+    }
+    if (c.ic()) {
+        // the file's inner-core field, or the outer core's extended into the
+        // inner core (program/io.F90:440-447)
+        const size_t HI = (size_t)c.H * c.Nic;
+        if (!ic.loaded) {
+            ic.f.assign(4 * HI, 0.0);
+            for (int k = 0; k < 2; k++) {
+                rc = lsd::ic_extend(c, buf.data() + (size_t)(2 * (4 + k)) * HN, ic.f.data() + (size_t)(2 * k) * HI);
+                if (rc) return rc;
+            }
+        }
+        for (int k = 0; k < 2; k++) {
+            rc = lsd_set_ic(ctx, k, ic.f.data() + (size_t)(2 * k) * HI, ic.f.data() + (size_t)(2 * k + 1) * HI);
+            if (rc) return rc;
+        }
+    }
+    // rot_omega from icOmega (program/io.F90:430-431)
+    if (c.cfg.rot || c.ic()) {
+        rc = lsd_set_rotation(ctx, ic.omega);
         if (rc) return rc;
     }
     // program/io.F90:394-430: tim_t from the file; with d_timestep < 0 also
@@ -1275,7 +1344,20 @@ int lsd_save_state(lsd_ctx* ctx, const char* path) {
         int rc = lsd_get_coll(ctx, k, buf.data() + (size_t)(2 * k) * HN, buf.data() + (size_t)(2 * k + 1) * HN);
         if (rc) return rc;
     }
-    return lsd::state_write(path, c.cfg, buf.data(), c.t, c.dt);
+    lsd::StateIc ic = lsd::state_ic(c);
+    if (c.ic()) {
+        const size_t HI = (size_t)c.H * c.Nic;
+        ic.f.assign(4 * HI, 0.0);
+        for (int k = 0; k < 2; k++) {
+            int rc = lsd_get_ic(ctx, k, ic.f.data() + (size_t)(2 * k) * HI, ic.f.data() + (size_t)(2 * k + 1) * HI);
+            if (rc) return rc;
+        }
+    }
+    double rot[3] = {0.0, 0.0, 0.0};
+    int rc = lsd_get_rotation(ctx, rot);
+    if (rc) return rc;
+    ic.omega = rot[0];
+    return lsd::state_write(path, c.cfg, buf.data(), c.t, c.dt, &ic);
 }
 
 int lsd_get_coll(lsd_ctx* ctx, int field, double* re, double* im) {

@@ -341,8 +341,18 @@ int energy_noalloc_init(int device);
 int energy_noalloc(Ctx& c, int which, double* E, double* El, double* Em);
 
 // state_io.hip: coll = [6][2][H][N] (uT, uP, C, Comp, BT, BP; Re, Im)
-int state_read(const char* path, const lsd_config& cfg, double* coll, double* t, double* dt);
-int state_write(const char* path, const lsd_config& cfg, const double* coll, double t, double dt);
+// The inner-core part of a state file: icBT, icBP on the inner-core radii
+// and icOmega (program/io.F90:430-447, 1585-1614).
+struct StateIc {
+    int Nic = 1;                // the run's inner-core points
+    std::vector<double> ric;    // the run's inner-core radii [Nic]
+    std::vector<double> f;      // [2][2][H][Nic] icBT, icBP (Re, Im)
+    bool loaded = false;        // read: the file held inner-core fields of more than one point
+    double omega = 0.0;         // icOmega
+};
+int state_read(const char* path, const lsd_config& cfg, double* coll, double* t, double* dt, StateIc* ic = nullptr);
+int state_write(const char* path, const lsd_config& cfg, const double* coll, double t, double dt,
+                const StateIc* ic = nullptr);
 
 // legendre_bf16.hip
 int tables_bf16(Ctx& c);

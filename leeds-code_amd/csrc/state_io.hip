@@ -313,7 +313,7 @@ struct Writer {
 }  // namespace
 
 // coll: [6][2][H][N] (uT, uP, C, Comp, BT, BP; Re then Im; reference coll layout)
-int state_read(const char* path, const lsd_config& cfg, double* coll, double* t, double* dt) {
+int state_read(const char* path, const lsd_config& cfg, double* coll, double* t, double* dt, StateIc* ic) {
     NcFile f;
     int rc = parse(path ? path : "", f);
     if (rc) return rc;
@@ -374,6 +374,58 @@ int state_read(const char* path, const lsd_config& cfg, double* coll, double* t,
             }
         }
     }
+    if (ic) {
+        // icOmega, and icBT / icBP when the file has an inner core of more
+        // than one point: rows 1..Nic(file) of each (ReIm, H) column of the
+        // outer core's length, regridded when the radii differ
+        // (io_load_coll with the inner-core grid, program/io.F90:440-447)
+        std::vector<double> om, icr;
+        ic->omega = get_doubles(f, "icOmega", shp, om) && om.size() == 1 ? om[0] : 0.0;
+        ic->loaded = false;
+        const bool has = ic->Nic > 1 && cfg.mag && f.vars.count("icBT") && f.vars.count("icBP") &&
+                         get_doubles(f, "icr", shp, icr) && icr.size() > 1;
+        if (has) {
+            const int Ni = ic->Nic, Nif = (int)icr.size();
+            bool inti = Nif != Ni;
+            if (!inti)
+                for (int n = 0; n < Ni; n++) inti = inti || std::fabs(icr[n] - ic->ric[n]) > 1e-8;
+            Interp ipi;
+            if (inti && !interp_weights(icr, ic->ric, ipi)) {
+                set_error("state file: inner-core interpolation weights are singular");
+                return LSD_E_SINGULAR;
+            }
+            ic->f.assign((size_t)4 * H * Ni, 0.0);
+            const char* icn[2] = {"icBT", "icBP"};
+            for (int k = 0; k < 2; k++) {
+                std::vector<double> v;
+                if (!get_doubles(f, icn[k], shp, v) || shp.size() != 3 || shp[0] != 2 || shp[2] < (size_t)Nif) {
+                    set_error(std::string("state file: variable ") + icn[k] + " is not (ReIm, H, r)");
+                    return LSD_E_ARG;
+                }
+                const NcVar& var = f.vars.at(icn[k]);
+                const int fL = att_int(var, "L", cfg.L), fM = att_int(var, "M", cfg.M), fMp = att_int(var, "Mp", cfg.Mp);
+                const size_t Hf = shp[1], Rf = shp[2];
+                for (int h = 0; h < H; h++) {
+                    const long j = state_nh(hl[h], hm[h], fL, fM, fMp);
+                    if (j < 0 || (size_t)j >= Hf) continue;
+                    for (int part = 0; part < 2; part++) {
+                        const double* src = &v[((size_t)part * Hf + (size_t)j) * Rf];
+                        double* dst = ic->f.data() + ((size_t)(2 * k + part) * H + h) * Ni;
+                        for (int n = 0; n < Ni; n++) {
+                            if (!inti) {
+                                dst[n] = src[n];
+                            } else {
+                                double s = 0.0;
+                                for (int c = 0; c < ipi.cnt[n]; c++) s += ipi.w[(size_t)n * 7 + c] * src[ipi.lo[n] + c];
+                                dst[n] = s;
+                            }
+                        }
+                    }
+                }
+            }
+            ic->loaded = true;
+        }
+    }
     if (t) {
         auto it = f.gatts.find("t");
         *t = (it != f.gatts.end() && !it->second.num.empty()) ? it->second.num[0] : 0.0;
@@ -385,7 +437,10 @@ int state_read(const char* path, const lsd_config& cfg, double* coll, double* t,
     return LSD_OK;
 }
 
-int state_write(const char* path, const lsd_config& cfg, const double* coll, double t, double dt) {
+int state_write(const char* path, const lsd_config& cfg, const double* coll, double t, double dt,
+                const StateIc* ic) {
+    const bool icf = ic && ic->Nic > 1 && (int)ic->ric.size() == ic->Nic;
+    const int Nic = icf ? ic->Nic : 1;
     const int N = cfg.N;
     std::vector<int> hl, hm;
     harmonics(cfg.L, cfg.M, cfg.Mp, hl, hm);
@@ -403,7 +458,7 @@ int state_write(const char* path, const lsd_config& cfg, const double* coll, dou
     w.name("r");
     w.u32(N);
     w.name("icr");
-    w.u32(1);
+    w.u32(Nic);
     w.name("H");
     w.u32(H);
     w.name("ReIm");
@@ -430,7 +485,7 @@ int state_write(const char* path, const lsd_config& cfg, const double* coll, dou
         bool coll;
         size_t count;
     };
-    std::vector<Def> defs = {{"r", {0}, false, (size_t)N}, {"icr", {1}, false, 1}, {"dt", {}, false, 1},
+    std::vector<Def> defs = {{"r", {0}, false, (size_t)N}, {"icr", {1}, false, (size_t)Nic}, {"dt", {}, false, 1},
                              {"icOmega", {}, false, 1}};
     for (const char* nm : names) defs.push_back({nm, {3, 2, 0}, true, 2 * HN});
     w.u32(NC_VARIABLE);
@@ -471,9 +526,14 @@ int state_write(const char* path, const lsd_config& cfg, const double* coll, dou
         for (int k = 0; k < 4; k++) w.b[begin_at[i] + k] = (unsigned char)(begins[i] >> (8 * (3 - k)));
     const std::vector<double> r = radial_grid(cfg);
     for (int n = 0; n < N; n++) w.f64(r[n]);
-    w.f64(0.0);  // icr: insulating inner core, one point
+    // icr: the inner-core radii (one point, r_i, for an insulating inner core,
+    // modules/meshs.F90:165-169 with i_Nic = 1)
+    if (icf)
+        for (int n = 0; n < Nic; n++) w.f64(ic->ric[n]);
+    else
+        w.f64(cfg.rratio / (1.0 - cfg.rratio));
     w.f64(dt);
-    w.f64(0.0);  // icOmega: no inner-core rotation
+    w.f64(ic ? ic->omega : 0.0);
     std::vector<double> buf((size_t)2 * HN);
     for (int k = 0; k < 8; k++) {
         std::fill(buf.begin(), buf.end(), 0.0);
@@ -482,6 +542,12 @@ int state_write(const char* path, const lsd_config& cfg, const double* coll, dou
                 for (int h = 0; h < H; h++)
                     std::memcpy(&buf[((size_t)part * H + pos[h]) * N], coll + ((size_t)(2 * k + part)) * HN + (size_t)h * N,
                                 sizeof(double) * N);
+        else if (icf)
+            // rows 1..Nic of each column (io_save_coll with i_Nic rows)
+            for (int part = 0; part < 2; part++)
+                for (int h = 0; h < H; h++)
+                    std::memcpy(&buf[((size_t)part * H + pos[h]) * N],
+                                ic->f.data() + ((size_t)(2 * (k - 6) + part) * H + h) * Nic, sizeof(double) * Nic);
         for (double x : buf) w.f64(x);
     }
     FILE* fp = std::fopen(path, "wb");

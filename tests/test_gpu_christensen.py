@@ -25,7 +25,7 @@ import numpy as np
 import pytest
 
 import ref_fixtures as rf
-from lsd.config import christensen_case1
+from lsd.config import christensen_case1, christensen_case2
 from lsd.gpu import Context
 
 pytestmark = pytest.mark.gpu
@@ -74,3 +74,53 @@ def test_christensen_case1_benchmark():
     assert abs(ke_m - KE_BENCH) < KE_TOL
     assert abs(me_m - ME_BENCH) < ME_TOL
     g.close()
+
+
+# Christensen case 2 at the resolution of the reference's benchmark run
+# (Benchmark_files/Boussinesq/Christensen_case2_benchmark/INFO): the series
+# of that directory are steady (vel_energy.dat, mag_energy.dat,
+# rot_torque.dat: KE 15474.553, ME 308651.156, rot_omega -13.297956, each
+# constant to 1e-7 relative over its 797 records).
+KE2, ME2, OM2 = 15474.553, 308651.156, -13.297956
+
+
+def test_christensen_case2_benchmark():
+    """Conducting, freely rotating inner core of 16 points: from the case-1
+    test state (its field extended into the inner core by lsd_load_state as
+    io_ic_extend does, rot_omega = 0) to t = 1.4 under the adaptive step;
+    the means of KE, ME and rot_omega over t in [1.2, 1.4] must match the
+    reference's steady values to 1e-4 relative (measured: 3e-6, 2e-6 and
+    2e-5 over t in [1, 9], with a residual oscillation of 3e-5 in KE that
+    the averaging window spans 2.4 periods of)."""
+    cfg = christensen_case2()
+    g = Context(cfg)
+    try:
+        g.load_state(os.path.join(HERE, "golden", "state_case1.cdf.in"))
+        s = rf.B_SCALE_OLD_STATE
+        for k in (4, 5):
+            re, im = g.get_coll(k)
+            g.set_coll(k, re * s, im * s)
+        for k in (0, 1):
+            re, im = g.get_ic(k)
+            g.set_ic(k, re * s, im * s)
+        t, steps = 0.0, 0
+        kes, mes, oms = [], [], []
+        while t < 1.4:
+            st = g.step(100)
+            steps += 100
+            t = st.t
+            if t >= 1.2:
+                kes.append(g.energy(0)[0])
+                mes.append(g.energy(1)[0])
+                oms.append(g.rotation()[0])
+            if steps % 10000 == 0:
+                print("step %d t %.4f KE %.3f ME %.2f omega %.5f" % (steps, t, g.energy(0)[0], g.energy(1)[0],
+                                                                     g.rotation()[0]), flush=True)
+        ke, me, om = np.mean(kes), np.mean(mes), np.mean(oms)
+        print("means over t in [1.2, 1.4] (%d samples): KE %.3f (ref %.3f) ME %.2f (ref %.2f) omega %.6f (ref %.6f)"
+              % (len(kes), ke, KE2, me, ME2, om, OM2), flush=True)
+        assert abs(ke / KE2 - 1) < 1e-4
+        assert abs(me / ME2 - 1) < 1e-4
+        assert abs(om / OM2 - 1) < 1e-4
+    finally:
+        g.close()

@@ -11,13 +11,18 @@ the torques to 1e-10 relative, same corrector counts; and the mixed form
 (FP32 transforms) against the plain form on the same case, 10 steps to
 1e-6 per field and in the rotation rate.
 """
+import os
+
 import numpy as np
 import pytest
 
 import oracle_lib as O
 from lsd import make_config, seeded_state, MIXED_F32
 from lsd.gpu import Context
-from lsd.initial import seeded_ic
+from lsd.config import christensen_case2
+from lsd.initial import seeded_ic, ic_extend
+
+HERE = os.path.dirname(os.path.abspath(__file__))
 
 pytestmark = pytest.mark.gpu
 
@@ -85,3 +90,38 @@ def test_innercore_mixed_form():
     finally:
         gp.close()
         gm.close()
+
+
+def test_state_file_inner_core(tmp_path):
+    """lsd_load_state on a file without an inner core (the reference's
+    case-1 test state) extends the outer core's field into the inner core
+    as io_ic_extend does (program/io.F90:1438-1477, here restated on the
+    host from the library's own radial weights); lsd_save_state writes
+    icBT, icBP and icOmega, and loading that file back restores them."""
+    cfg = christensen_case2()
+    g = Context(cfg)
+    try:
+        g.load_state(os.path.join(HERE, "golden", "state_case1.cdf.in"))
+        r, dr1 = g.mesh(0), g.mesh(1)
+        for k, name in ((0, "BT"), (1, "BP")):
+            want = ic_extend(cfg, g.get_coll(4 + k), r, dr1)
+            got = g.get_ic(k)
+            assert _rel(got, want) < 1e-14, name
+        g.set_rotation(-1.25)
+        g.step(3)
+        path = str(tmp_path / "state_ic.cdf")
+        g.save_state(path)
+        h = Context(cfg)
+        try:
+            h.load_state(path)
+            for k in range(6):
+                a, b = g.get_coll(k), h.get_coll(k)
+                assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), k
+            for k in range(2):
+                a, b = g.get_ic(k), h.get_ic(k)
+                assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), k
+            assert h.rotation()[0] == g.rotation()[0]
+        finally:
+            h.close()
+    finally:
+        g.close()
