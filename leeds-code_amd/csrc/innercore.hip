@@ -263,7 +263,8 @@ int launch_ic_nonlinear(Ctx& c) {
 // the inner boundary.  One workgroup.
 template <typename V>
 __global__ void __launch_bounds__(256) k_torques(const V* __restrict__ four, size_t per, int Th, int MP, int M,
-                                                 int mag, const double* __restrict__ gwsin, double* __restrict__ rot) {
+                                                 int visc, int mag, const double* __restrict__ gwsin,
+                                                 double* __restrict__ rot) {
     __shared__ double sv[256], sm[256];
     const double S = -2.0 / rot[ROT_RI];
     double v = 0.0, m = 0.0;
@@ -293,21 +294,22 @@ __global__ void __launch_bounds__(256) k_torques(const V* __restrict__ four, siz
     }
     if (threadIdx.x == 0) {
         const double FOURPI = 4.0 * 3.1415926535897931;
-        rot[ROT_VTORQ] = FOURPI * sv[0] * rot[ROT_VFAC];
+        // a stress-free inner boundary transmits no viscous torque
+        rot[ROT_VTORQ] = visc ? FOURPI * sv[0] * rot[ROT_VFAC] : 0.0;
         rot[ROT_MTORQ] = mag ? FOURPI * sm[0] * rot[ROT_MFAC] : 0.0;
     }
 }
 
 int launch_torques(Ctx& c, bool f32) {
     if (!c.cfg.rot) return LSD_OK;
-    const int mag = c.ic() ? 1 : 0;
+    const int mag = c.ic() ? 1 : 0, visc = c.cfg.vel_bc <= 2 ? 1 : 0;
     if (f32)
         hipLaunchKernelGGL(k_torques<float2>, dim3(1), dim3(256), 0, c.stream,
-                           reinterpret_cast<const float2*>(c.d_fourier), c.fourier_elems(), c.Th, c.MP, c.M, mag,
-                           c.d_gwsin, c.d_rot);
+                           reinterpret_cast<const float2*>(c.d_fourier), c.fourier_elems(), c.Th, c.MP, c.M, visc,
+                           mag, c.d_gwsin, c.d_rot);
     else
         hipLaunchKernelGGL(k_torques<double2>, dim3(1), dim3(256), 0, c.stream, c.d_fourier, c.fourier_elems(), c.Th,
-                           c.MP, c.M, mag, c.d_gwsin, c.d_rot);
+                           c.MP, c.M, visc, mag, c.d_gwsin, c.d_rot);
     LSD_HIP(hipGetLastError());
     return LSD_OK;
 }
@@ -330,6 +332,44 @@ __global__ void k_rot_update(double* rot, int pred, double cimp) {
 int launch_rot_update(Ctx& c, bool pred) {
     if (!c.cfg.rot) return LSD_OK;
     hipLaunchKernelGGL(k_rot_update, dim3(1), dim3(64), 0, c.stream, c.d_rot, pred ? 1 : 0, c.cfg.implicit);
+    LSD_HIP(hipGetLastError());
+    return LSD_OK;
+}
+
+// vel_angmomclr, Boussinesq (modules/velocity.F90:475-513): with both
+// walls stress-free, the rigid rotations omz (from (l, m) = (1, 0)) and
+// omx, omy (from (1, 1)) are removed from the toroidal velocity, each the
+// ratio of int r^2 (r T) dr to int r^2 r^2 dr with the intr2dr weights.
+// Thread 0 takes (1, 0), thread 1 (1, 1); sums in ascending radius.
+__global__ void k_angmomclr(int N, int h10, int h11, const double* __restrict__ r,
+                            const double* __restrict__ intr2dr, double2* __restrict__ uT) {
+    const int t = threadIdx.x;
+    const int h = t == 0 ? h10 : h11;
+    if (t > 1 || h < 0) return;
+    const double* r1 = r + 4 * N;
+    const double* r2 = r + 5 * N;
+    const double w = 8.0 / 3.0 * 3.1415926535897931;
+    double2* T = uT + (size_t)h * N;
+    double d1 = 0.0, d2 = 0.0, d3 = 0.0;
+    for (int n = 0; n < N; n++) {
+        d1 = d1 + (T[n].x * r1[n]) * intr2dr[n];
+        d2 = d2 + r2[n] * intr2dr[n];
+        d3 = d3 + (T[n].y * r1[n]) * intr2dr[n];
+    }
+    if (t == 0) {
+        const double omz = (w * d1) / (w * d2);
+        for (int n = 0; n < N; n++) T[n].x = T[n].x - omz * r1[n];
+    } else {
+        const double omx = (2.0 * w * d1) / (2.0 * w * d2);
+        const double omy = (-2.0 * w * d3) / (-2.0 * w * d2);
+        for (int n = 0; n < N; n++) T[n] = make_double2(T[n].x - omx * r1[n], T[n].y - omy * r1[n]);
+    }
+}
+
+int launch_angmomclr(Ctx& c) {
+    if (c.cfg.vel_bc != 4) return LSD_OK;
+    hipLaunchKernelGGL(k_angmomclr, dim3(1), dim3(64), 0, c.stream, c.N, c.h10, c.h11, c.d_r, c.d_intr2dr,
+                       c.d_f[LSD_VEL_TOR]);
     LSD_HIP(hipGetLastError());
     return LSD_OK;
 }

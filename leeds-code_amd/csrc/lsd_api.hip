@@ -32,8 +32,12 @@ static int check_config(const lsd_config* c, int* H, int* Th, int* Ph) {
     if (th % 2) { set_error("Th must be even"); return LSD_E_CONFIG; }
     if (ph < 2 * c->M || ph % 2) { set_error("Ph must be even and >= 2M"); return LSD_E_CONFIG; }
     if (!c->vel) { set_error("vel must be time-stepped"); return LSD_E_CONFIG; }
-    if (c->vel_bc != 1 || c->cod_bc != 1 || c->comp_bc != 1) {
-        set_error("only i_vel_bc = i_cod_bc = i_comp_bc = 1 are supported");
+    if (c->vel_bc < 1 || c->vel_bc > 4 || c->cod_bc != 1 || c->comp_bc != 1) {
+        set_error("i_vel_bc must be 1..4, i_cod_bc = i_comp_bc = 1");
+        return LSD_E_CONFIG;
+    }
+    if (c->Nic > 1 && c->vel_bc > 2) {
+        set_error("a conducting inner core needs a no-slip inner boundary (i_vel_bc = 1 or 2)");
         return LSD_E_CONFIG;
     }
     if (c->dt == 0.0) { set_error("dt must be nonzero (> 0 fixed, < 0 adaptive)"); return LSD_E_CONFIG; }
@@ -233,6 +237,10 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
         c->slots.back().len++;
     }
     c->nslot = (int)c->slots.size();
+    for (int h = 0; h < H; h++) {
+        if (c->hl[h] == 1 && c->hm[h] == 0) c->h10 = h;
+        if (c->hl[h] == 1 && c->hm[h] == 1) c->h11 = h;
+    }
     c->slot_order.resize(c->nslot);
     for (int s = 0; s < c->nslot; s++) c->slot_order[s] = s;
     std::stable_sort(c->slot_order.begin(), c->slot_order.end(),
@@ -562,6 +570,8 @@ static int synthesise(Ctx& c, bool f, bool to_phys) {
     void* g_th[2];
     void* g_ph[2];
     int ns = 0, nv = 0, ng = 0;
+    // vel_angmomclr before the velocity transform (program/main.F90:100)
+    RC(launch_angmomclr(c));
     // the vector fields (u; B) and the scalar fields (C; xi) are prepared in
     // one launch each
     const double2* vT[2];

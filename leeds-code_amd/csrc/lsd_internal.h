@@ -259,7 +259,11 @@ struct Ctx {
     double* h_icpartial = nullptr;     // pinned copy
     double* d_rot = nullptr;           // [ROT_COUNT], see RotSlot
     double* d_gwsin = nullptr;         // [Th] leg_gwP0 sin(theta)
+    int h10 = -1, h11 = -1;            // harmonics (l, m) = (1, 0), (1, 1) (-1 if absent)
     bool ic() const { return Nic > 1 && cfg.mag; }
+    // the rotating inner core moves the wall: b_rot_tstep with a no-slip
+    // inner boundary (vel_setbc_Tor, non_rotation)
+    bool rot_wall() const { return cfg.rot && cfg.vel_bc <= 2; }
 
     size_t coll_elems() const { return (size_t)H * N; }
     size_t phys_elems() const { return (size_t)Th * Ph * N; }
@@ -297,6 +301,7 @@ int launch_ic_solve(Ctx& c, bool pred);
 int launch_torques(Ctx& c, bool f32);
 int launch_ic_nonlinear(Ctx& c);
 int launch_rot_update(Ctx& c, bool pred);
+int launch_angmomclr(Ctx& c);
 int build_operators(Ctx& c);
 
 // legendre.hip

@@ -463,6 +463,7 @@ __device__ void gbtrs_d(int n, const double* __restrict__ ab, const int* __restr
 
 struct OpParams {
     double dt, implicit, Pm, Pr, Sc;
+    int vel_bc;  // i_vel_bc: 1 no-slip, 2 stress-free outer, 3 stress-free inner, 4 both stress-free
 };
 
 // One work-item per (operator kind, degree l): X = c1/dt - c2 c radLap_l
@@ -472,7 +473,7 @@ struct OpParams {
 // Y = c1/dt + c2 (1-c) radLap_l (timestep.F90:421-473).
 __global__ void k_operators(int L, int N, OpParams op, const double* __restrict__ r,
                             const double* __restrict__ radlap, const double* __restrict__ dr1,
-                            double* __restrict__ lu, int* __restrict__ piv, double* __restrict__ ym,
+                            const double* __restrict__ dr2, double* __restrict__ lu, int* __restrict__ piv, double* __restrict__ ym,
                             int* __restrict__ info, double* __restrict__ xband) {
     int l = blockIdx.x * blockDim.x + threadIdx.x;
     int kind = blockIdx.y;
@@ -509,11 +510,27 @@ __global__ void k_operators(int L, int N, OpParams op, const double* __restrict_
     }
 #define LUA(i, j) ab[((j) - 1) * LSD_LDAB + (2 * LSD_KL + (i) - (j))]
 #define DR1(i, j) dr1[((j) - 1) * 7 + (LSD_KL + (i) - (j))]
+#define DR2(i, j) dr2[((j) - 1) * 7 + (LSD_KL + (i) - (j))]
     for (int j = 1; j <= 1 + LSD_KL; j++) LUA(1, j) = 0.0;
     for (int j = N - LSD_KL; j <= N; j++) LUA(N, j) = 0.0;
-    if (kind == 1) {  // vel_bc_Pol, no-slip: dP/dr = 0
-        for (int j = 1; j <= 1 + LSD_KL; j++) LUA(1, j) = DR1(1, j);
-        for (int j = N - LSD_KL; j <= N; j++) LUA(N, j) = DR1(N, j);
+    // no-slip walls: inner for i_vel_bc 1, 2; outer for 1, 3
+    const bool ns_in = op.vel_bc == 1 || op.vel_bc == 2, ns_out = op.vel_bc == 1 || op.vel_bc == 3;
+    if (kind == 0) {  // vel_bc_Tor: no-slip T = 0, stress-free (dr - 1/r) T = 0
+        if (ns_in) {
+            LUA(1, 1) = 1.0;
+        } else {
+            for (int j = 1; j <= 1 + LSD_KL; j++) LUA(1, j) = DR1(1, j);
+            LUA(1, 1) = LUA(1, 1) - rm1[0];
+        }
+        if (ns_out) {
+            LUA(N, N) = 1.0;
+        } else {
+            for (int j = N - LSD_KL; j <= N; j++) LUA(N, j) = DR1(N, j);
+            LUA(N, N) = LUA(N, N) - rm1[N - 1];
+        }
+    } else if (kind == 1) {  // vel_bc_Pol: no-slip dP/dr = 0, stress-free d2P/dr2 = 0
+        for (int j = 1; j <= 1 + LSD_KL; j++) LUA(1, j) = ns_in ? DR1(1, j) : DR2(1, j);
+        for (int j = N - LSD_KL; j <= N; j++) LUA(N, j) = ns_out ? DR1(N, j) : DR2(N, j);
     } else if (kind == 6) {  // mag_bc_Pol: (dr - l/r) at r_i, (dr + (l+1)/r) at r_o
         for (int j = 1; j <= 1 + LSD_KL; j++) LUA(1, j) = DR1(1, j);
         LUA(1, 1) = LUA(1, 1) - rm1[0] * l;
@@ -525,6 +542,7 @@ __global__ void k_operators(int L, int N, OpParams op, const double* __restrict_
     }
 #undef LUA
 #undef DR1
+#undef DR2
     // keep X itself (7 bands per column) for residuals of the mixed forms
     double* xb = xband + ((size_t)kind * L + l) * N * 7;
     for (int j = 0; j < N; j++)
@@ -798,7 +816,7 @@ int build_ic_operators(Ctx& c) {
         LSD_HIP(hipStreamSynchronize(c.stream));
         return LSD_OK;
     }
-    OpParams op{c.dt, c.cfg.implicit, c.cfg.Pm, c.cfg.Pr, c.cfg.Sc};
+    OpParams op{c.dt, c.cfg.implicit, c.cfg.Pm, c.cfg.Pr, c.cfg.Sc, c.cfg.vel_bc};
     int* info = nullptr;
     AsyncScratch s_info(c.stream);
     LSD_HIP(s_info.alloc(&info, sizeof(int) * 2 * c.L));
@@ -817,10 +835,10 @@ int build_ic_operators(Ctx& c) {
 }
 
 int build_operators(Ctx& c) {
-    OpParams op{c.dt, c.cfg.implicit, c.cfg.Pm, c.cfg.Pr, c.cfg.Sc};
+    OpParams op{c.dt, c.cfg.implicit, c.cfg.Pm, c.cfg.Pr, c.cfg.Sc, c.cfg.vel_bc};
     int L = c.L, N = c.N;
     dim3 grid((L + 63) / 64, NLU);
-    hipLaunchKernelGGL(k_operators, grid, dim3(64), 0, c.stream, L, N, op, c.d_r, c.d_radlap, c.d_dr1,
+    hipLaunchKernelGGL(k_operators, grid, dim3(64), 0, c.stream, L, N, op, c.d_r, c.d_radlap, c.d_dr1, c.d_dr2,
                        c.d_lu, c.d_piv, c.d_y, c.d_info, c.d_xband);
     LSD_HIP(hipGetLastError());
     double* scratch = nullptr;

@@ -212,7 +212,7 @@ __global__ void __launch_bounds__(64) k_correct(SolveArgs a) {
         if (act) {
             c.w[0] = c.w[(N - 1) * S] = 0.0;
             // vel_setbc_Tor for (1,0) once tim_it /= 0
-            if (f == 0 && l == 1 && a.hm[h] == 0 && part == 0)
+            if (a.t10 && f == 0 && l == 1 && a.hm[h] == 0 && part == 0)
                 c.w[0] = (a.rot ? a.rot[ROT_OMEGA] * a.rot[ROT_RI] : 0.0) - a.x[0][(size_t)h * N].x;
             if (solve) solve_staged<MIX>(m, c, c32);
             if (f == 1 && solve) c.w[0] = c.w[(N - 1) * S] = 0.0;
@@ -313,7 +313,8 @@ static SolveArgs make_args(Ctx& c) {
     a.on[5] = c.cfg.mag && !c.ic();
     a.cimp = c.cfg.implicit;
     a.hm = c.d_hm;
-    a.rot = c.cfg.rot ? c.d_rot : nullptr;
+    a.rot = c.rot_wall() ? c.d_rot : nullptr;
+    a.t10 = c.cfg.vel_bc <= 2;
     a.blocks = c.d_sblocks;
     a.order = c.d_sorder;
     a.lu = c.d_lu;

@@ -25,6 +25,7 @@ struct SolveArgs {
     double2* hist[6];    // saved N
     double* partial;     // [8][gridDim.x]
     const double* rot;   // inner-core rotation state (RotSlot), null without b_rot_tstep
+    int t10;             // the inner wall is no-slip: T_10(r_i) row of vel_setbc_Tor
 };
 
 // LDS of one block.  Columns first, then the operator data of one (kind, l).

@@ -53,6 +53,7 @@ struct DirArgs {
     double2* rhs2;       // poloidal velocity: the Y x part of the XPol right-hand side
     double* partial;     // [8][gridDim.x]
     const double* rot;   // inner-core rotation state (RotSlot), null without b_rot_tstep
+    int t10;             // the inner wall is no-slip: T_10(r_i) row of vel_setbc_Tor
 };
 
 // v[f] for a field index that is the same across the workgroup, as scalar
@@ -427,7 +428,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
             BATCH_FENCE();
         }
         // vel_setbc_Tor for (1, 0) once tim_it /= 0
-        if (f == 0 && l == 1 && a.hm[h] == 0 && part == 0)
+        if (a.t10 && f == 0 && l == 1 && a.hm[h] == 0 && part == 0)
             b[0] = (a.rot ? a.rot[ROT_OMEGA] * a.rot[ROT_RI] : 0.0) - xcol[0].x;
         if (solve) gbtrs_reg<N, double>(b, s.ab, s.rd, pivots(a, lu_kind(f), l));
         if (f == 1 && solve) {
@@ -513,7 +514,8 @@ static DirArgs dir_args(Ctx& c) {
     a.on[4] = c.cfg.mag && !c.ic();
     a.on[5] = c.cfg.mag && !c.ic();
     a.cimp = c.cfg.implicit;
-    a.rot = c.cfg.rot ? c.d_rot : nullptr;
+    a.rot = c.rot_wall() ? c.d_rot : nullptr;
+    a.t10 = c.cfg.vel_bc <= 2;
     a.hl = c.d_hl;
     a.hm = c.d_hm;
     a.blocks = c.d_sblocks;

@@ -819,19 +819,32 @@ static int matrices(orc_sim* s) {
     const double* rm1 = s->rp + 2 * N;
     for (int l = 0; l < L; l++) {
         double* ab;
-        /* velocity toroidal */
+        /* velocity toroidal: no-slip T = 0, stress-free (dr - 1/r) T = 0
+         * (vel_bc_Tor: inner no-slip for i_vel_bc 1, 2; outer for 1, 3) */
+        const int ns_in = c->vel_bc == 1 || c->vel_bc == 2, ns_out = c->vel_bc == 1 || c->vel_bc == 3;
         if (l >= 1) {
             ab = s->lu[0] + (size_t)l * N * LDAB;
             build_X(s, 1.0, c->Pm, l, 0, ab);
-            LUA(ab, 1, 1) = 1.0;
-            LUA(ab, N, N) = 1.0;
+            if (ns_in) {
+                LUA(ab, 1, 1) = 1.0;
+            } else {
+                for (int j = 1; j <= 1 + KL; j++) LUA(ab, 1, j) = MSH(s->dr1, 1, j);
+                LUA(ab, 1, 1) = LUA(ab, 1, 1) - rm1[0];
+            }
+            if (ns_out) {
+                LUA(ab, N, N) = 1.0;
+            } else {
+                for (int j = N - KL; j <= N; j++) LUA(ab, N, j) = MSH(s->dr1, N, j);
+                LUA(ab, N, N) = LUA(ab, N, N) - rm1[N - 1];
+            }
             if (factor(ab, s->piv[0] + l * N, N, "mes_lu_find")) return LSD_E_SINGULAR;
             build_Y(s, 1.0, c->Pm, l, s->ym[0] + (size_t)l * N * MBW);
-            /* velocity poloidal */
+            /* velocity poloidal: no-slip dP/dr = 0, stress-free d2P/dr2 = 0
+             * (vel_bc_Pol, Boussinesq) */
             ab = s->lu[1] + (size_t)l * N * LDAB;
             build_X(s, 1.0, c->Pm, l, 0, ab);
-            for (int j = 1; j <= 1 + KL; j++) LUA(ab, 1, j) = MSH(s->dr1, 1, j);
-            for (int j = N - KL; j <= N; j++) LUA(ab, N, j) = MSH(s->dr1, N, j);
+            for (int j = 1; j <= 1 + KL; j++) LUA(ab, 1, j) = MSH(ns_in ? s->dr1 : s->dr2, 1, j);
+            for (int j = N - KL; j <= N; j++) LUA(ab, N, j) = MSH(ns_out ? s->dr1 : s->dr2, N, j);
             if (factor(ab, s->piv[1] + l * N, N, "mes_lu_find")) return LSD_E_SINGULAR;
             build_Y(s, 1.0, c->Pm, l, s->ym[1] + (size_t)l * N * MBW);
             /* Green's function operator */
@@ -916,8 +929,12 @@ orc_sim* orc_create(const lsd_config* cfg) {
     int H, Th, Ph;
     if (orc_sizes(cfg, &H, &Th, &Ph)) return NULL;
     if (!cfg->vel) { set_err("vel must be time-stepped"); return NULL; }
-    if (cfg->vel_bc != 1 || cfg->cod_bc != 1 || cfg->comp_bc != 1) {
-        set_err("only i_vel_bc = i_cod_bc = i_comp_bc = 1 are supported");
+    if (cfg->vel_bc < 1 || cfg->vel_bc > 4 || cfg->cod_bc != 1 || cfg->comp_bc != 1) {
+        set_err("i_vel_bc must be 1..4, i_cod_bc = i_comp_bc = 1");
+        return NULL;
+    }
+    if (cfg->Nic > 1 && cfg->vel_bc > 2) {
+        set_err("a conducting inner core needs a no-slip inner boundary (i_vel_bc = 1 or 2)");
         return NULL;
     }
     if (cfg->Nic < 0 || (cfg->Nic > 1 && (!cfg->mag || cfg->Nic < 2 * KL + 2))) {
@@ -1549,7 +1566,40 @@ static void scal_transform(orc_sim* s, const coll_t* C, int o) {
     spec_free(&a); spec_free(&b);
 }
 
+/* vel_angmomclr, Boussinesq (modules/velocity.F90:475-513): with both
+ * boundaries stress-free the rigid rotations omz, omx, omy are removed
+ * from T of (l, m) = (1, 0) and (1, 1), each the ratio of the moments
+ * int r^2 (r T) dr and int r^2 r^2 dr with the intr2dr weights. */
+static void angmomclr(orc_sim* s) {
+    if (s->c.vel_bc != 4) return;
+    const int N = s->N;
+    const double* r1 = s->rp + 4 * N;
+    const double* r2 = s->rp + 5 * N;
+    const double w = 8.0 / 3.0 * D_PI;
+    coll_t* T = &s->f[LSD_VEL_TOR];
+    for (int nh = 0; nh < s->H; nh++) {
+        if (s->hl[nh] != 1 || s->hm[nh] > 1) continue;
+        double* re = T->re + (size_t)nh * N;
+        double* im = T->im + (size_t)nh * N;
+        double d1 = 0.0, d2 = 0.0;
+        for (int n = 0; n < N; n++) d1 = d1 + (re[n] * r1[n]) * s->intr2dr[n];
+        for (int n = 0; n < N; n++) d2 = d2 + r2[n] * s->intr2dr[n];
+        if (s->hm[nh] == 0) {
+            const double omz = (w * d1) / (w * d2);
+            for (int n = 0; n < N; n++) re[n] = re[n] - omz * r1[n];
+        } else {
+            double d3 = 0.0;
+            for (int n = 0; n < N; n++) d3 = d3 + (im[n] * r1[n]) * s->intr2dr[n];
+            const double omx = (2.0 * w * d1) / (2.0 * w * d2);
+            const double omy = (-2.0 * w * d3) / (-2.0 * w * d2);
+            for (int n = 0; n < N; n++) re[n] = re[n] - omx * r1[n];
+            for (int n = 0; n < N; n++) im[n] = im[n] - omy * r1[n];
+        }
+    }
+}
+
 int orc_transform(orc_sim* s) {
+    angmomclr(s);
     vec_transform(s, &s->f[LSD_VEL_TOR], &s->f[LSD_VEL_POL], LSD_P_VEL_R);
     if (s->c.cod) scal_transform(s, &s->f[LSD_COD], LSD_P_COD_GR);
     if (s->c.comp) scal_transform(s, &s->f[LSD_COMP], LSD_P_COMP_GR);
@@ -1591,7 +1641,7 @@ static void rotation_torques(orc_sim* s) {
             const size_t q = (size_t)k * Th + j;
             p[q] = (S * up[q] - wt[q]) * s->sn[j];
         }
-    s->vtorq = surfintegral(s, p) * vfac;
+    s->vtorq = s->c.vel_bc > 2 ? 0.0 : surfintegral(s, p) * vfac;
     if (s->Nic == 1) {
         s->mtorq = 0.0;
     } else {
@@ -1898,7 +1948,7 @@ int orc_predictor(orc_sim* s) {
             if (l == 0) memcpy(v, nn, sizeof(double) * N);
             else multY(s, YMp(s, 0, l), x, nn, v);
             v[0] = 0.0; v[N - 1] = 0.0; /* vel_setbc_Tor (velocity.F90:279-303) */
-            if (l == 1 && m_ == 0 && part == 0) v[0] = s->omega * s->rp[4 * N];
+            if (l == 1 && m_ == 0 && part == 0 && s->c.vel_bc <= 2) v[0] = s->omega * s->rp[4 * N];
             if (l != 0) orc_gbtrs(N, LUp(s, 0, l), PVp(s, 0, l), v);
             memcpy(x, v, sizeof(double) * N);
             /* velocity poloidal */
@@ -1992,7 +2042,7 @@ int orc_corrector(orc_sim* s, double* dterr_out) {
                 v[0] = 0.0; v[N - 1] = 0.0;
                 /* vel_setbc_Tor with tim_it /= 0 (velocity.F90:292-302):
                  * T_10 at r_i gets rot_omega r_i - vel_Tor there. */
-                if (k == 0 && part == 0 && l == 1 && s->hm[nh] == 0)
+                if (k == 0 && part == 0 && l == 1 && s->hm[nh] == 0 && s->c.vel_bc <= 2)
                     v[0] = s->omega * s->rp[4 * N] - s->f[LSD_VEL_TOR].re[o];
                 if (ic && k >= 4) {
                     /* tim_nlincorr on the inner core, tim_zerobc, tim_icinvX */

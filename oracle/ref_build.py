@@ -62,7 +62,8 @@ def cases():
     return {"cfg1": config1(), "small": config_small(), "cfg3": config3(),
             "adapt": config_small(dt=-1e-4), "adapt1": config_small(dt=-1e-4, step_control=1),
             "case1": christensen_case1(),
-            "ic": config_small(Nic=8, rot=1), "case2": christensen_case2()}
+            "ic": config_small(Nic=8, rot=1), "case2": christensen_case2(),
+            "sf2": config_small(vel_bc=2), "sf3": config_small(vel_bc=3), "sf4": config_small(vel_bc=4)}
 
 
 def fdouble(x):

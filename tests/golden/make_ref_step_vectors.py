@@ -57,12 +57,17 @@ CASES = {
     # 16 points) from the case-1 test state, its field extended into the
     # inner core as io_ic_extend does (program/io.F90:1438-1477)
     "case2": (christensen_case2, -1, 0.0, 10),
+    # stress-free walls (vel_bc_Tor, vel_bc_Pol): outer (2), inner (3), both
+    # (4, with vel_angmomclr removing the rigid rotations)
+    "sf2": (lambda: config_small(vel_bc=2), 12, 1e-2, 20),
+    "sf3": (lambda: config_small(vel_bc=3), 13, 1e-2, 20),
+    "sf4": (lambda: config_small(vel_bc=4), 14, 1e-2, 20),
 }
 OMEGA0 = {"ic": 0.3, "case2": 0.0}
 FULL = {"small": (1, 2, 20), "cfg1": (1,), "cfg3": (), "adapt": (1, 10, 40), "adapt1": (1, 20, 40),
-        "case1": (), "ic": (1, 2, 20), "case2": ()}
+        "case1": (), "ic": (1, 2, 20), "case2": (), "sf2": (1, 20), "sf3": (1, 20), "sf4": (1, 2, 20)}
 SAMPLED = {"small": (), "cfg1": (10, 100), "cfg3": (1, 2), "adapt": (), "adapt1": (), "case1": (1, 10, 30),
-           "ic": (), "case2": (1, 10)}
+           "ic": (), "case2": (1, 10), "sf2": (), "sf3": (), "sf4": ()}
 STATE_FILE = {"case1": "state_case1.cdf.in", "case2": "state_case1.cdf.in"}
 # cases whose fixtures also record the reference's own per-step spread
 SPREAD = ("adapt", "adapt1", "case1", "case2")

@@ -20,7 +20,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 FIELDS = ("uT", "uP", "C", "X", "BT", "BP")
 REL_STEP = {"small": 1e-12, "cfg1": 1e-12, "cfg3": 6e-11, "adapt": 1e-12, "adapt1": 1e-12, "case1": 1e-12,
-            "ic": 1e-12, "case2": 1e-12}
+            "ic": 1e-12, "case2": 1e-12, "sf2": 1e-12, "sf3": 1e-12, "sf4": 1e-12}
 
 # tim_dterr is the norm of the corrector's increment X^-1 c (N* - N).  At
 # dt below ~1e-6 (the first steps of an adaptive run, which start from

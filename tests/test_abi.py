@@ -40,7 +40,10 @@ def test_config_sizes_follow_the_reference():
     (dict(L=8, M=16), "L > (M-1)*Mp"),
     (dict(Th=47), "Th must be even"),
     (dict(Ph=40), "Ph must be even and >= 2M"),
-    (dict(vel_bc=2), "only i_vel_bc"),
+    (dict(vel_bc=5), "i_vel_bc must be 1..4"),
+    (dict(Nic=16, vel_bc=3), "no-slip inner boundary"),
+    (dict(Nic=4), "8 <= i_Nic"),
+    (dict(Nic=16, mag=0), "needs the magnetic field"),
 ])
 def test_config_errors(kw, msg):
     with pytest.raises(LsdError, match=re.escape(msg)):

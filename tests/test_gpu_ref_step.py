@@ -14,8 +14,10 @@ CFL limit non_maxtstep and tim_new_tstep, the operators refactorised in
 FP64 at every change), the Christensen case-1 dynamo, a conducting
 inner core of 8 points that rotates (the coupled inner/outer-core solves,
 its induction, the torques and the rotation rate, also compared) and the
-Christensen case-2 benchmark (16 inner-core points, N = 64), replayed from
-graphs and launched eagerly.
+Christensen case-2 benchmark (16 inner-core points, N = 64) and the
+stress-free walls (i_vel_bc = 2, 3, 4; with both walls stress-free
+vel_angmomclr removes the rigid rotations), replayed from graphs and
+launched eagerly.
 """
 import numpy as np
 import pytest
@@ -29,7 +31,8 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.parametrize("case,nsteps,graph", [("small", 20, True), ("small", 20, False), ("cfg1", 100, True),
                                                ("cfg3", 2, True), ("adapt", 40, True), ("adapt", 40, False),
                                                ("adapt1", 40, True), ("case1", 30, True), ("ic", 20, True),
-                                               ("ic", 20, False), ("case2", 10, True)])
+                                               ("ic", 20, False), ("case2", 10, True), ("sf2", 20, True),
+                                               ("sf3", 20, True), ("sf4", 20, True), ("sf4", 20, False)])
 def test_plain_steps_equal_reference(case, nsteps, graph):
     fx = rf.load(case)
     cfg = rf.config(fx)

@@ -117,3 +117,11 @@ def test_oracle_christensen_case2_equal_reference():
     # M = 16, Mp = 4, adaptive control 1) from the case-1 test state, its
     # field extended into the inner core as io_ic_extend does
     _run_and_compare("case2", 10)
+
+
+@pytest.mark.parametrize("case", ["sf2", "sf3", "sf4"])
+def test_oracle_stress_free_walls_equal_reference(case):
+    # i_vel_bc = 2, 3, 4: stress-free outer, inner, both walls (vel_bc_Tor
+    # (dr - 1/r) T = 0, vel_bc_Pol d2P/dr2 = 0), with vel_angmomclr removing
+    # the rigid rotations of T at (l, m) = (1, 0), (1, 1) when both are
+    _run_and_compare(case, 20)
