@@ -312,8 +312,10 @@ int lsd_sync(lsd_ctx* ctx);
  * given here.  Every variant gives the same results, bit for bit for the
  * synthesis shapes and to rounding for the analysis and solve forms.
  *  SYN_*     FP32 synthesis: m per workgroup, 4 or 8 (one wave per m, one
- *            hemisphere of one output staged in LDS per pass); defaults
- *            4, 8, 8
+ *            hemisphere of one output staged in LDS per pass), or 5, 9:
+ *            the same shapes as persistent kernels (the CUs' worth of
+ *            workgroups pulling the work items from per-XCD queues);
+ *            defaults 4, 8, 8
  *  ANA       FP32 analysis: 0 wide-load (even Th/2 only; default), 1 LDS-staged
  *  SOLVE     banded solves: 0 two-pass register-column FP64 (N in {32, 48,
  *            64, 96}; default there), 1 LDS-column (the mixed forms: FP32
@@ -336,7 +338,9 @@ int lsd_get_variant(lsd_ctx* ctx, int which, int* value);
  * the cache named by the environment variable LSD_TUNING_CACHE, else
  * tuning_cache.jsonl beside the library, when present.  times_us[LSD_V_COUNT
  * * 2] (may be NULL) receives the time of variant 0 and 1 of each group
- * (0 where a variant does not apply). */
+ * (0 where a variant does not apply; for the synthesis the shapes 4 and 8,
+ * after which the persistent form of the faster is timed and kept if it is
+ * faster still). */
 int lsd_tune(lsd_ctx* ctx, int reps, const char* cache_path, double* times_us);
 
 /* Kernel classes for per-launch timing. */

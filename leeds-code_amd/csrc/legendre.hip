@@ -63,14 +63,17 @@ struct BlockId {
     int group, cc;
     bool valid;
 };
-__device__ __forceinline__ BlockId decode_block(int ngroups, int ncc) {
-    const int b = blockIdx.x, xcd = b & 7, q = b >> 3;
+__device__ __forceinline__ BlockId decode_block_at(int b, int ngroups, int ncc) {
+    const int xcd = b & 7, q = b >> 3;
     const int gl = q / ncc;
     BlockId r;
     r.cc = q - gl * ncc;
     r.group = gl * 8 + xcd;
     r.valid = r.group < ngroups;
     return r;
+}
+__device__ __forceinline__ BlockId decode_block(int ngroups, int ncc) {
+    return decode_block_at(blockIdx.x, ngroups, ncc);
 }
 static unsigned grid_blocks(int ngroups, int ncc) { return (unsigned)(((ngroups + 7) / 8) * 8 * ncc); }
 
@@ -411,16 +414,18 @@ __device__ __forceinline__ v4f mfma4(float a, float b, v4f c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+template <int MQ>
+constexpr int S4_STAGE = S4_NMIR<MQ> * S4_SH * S4_SHS<MQ>;
+
+// One work item (block number b of the static numbering): groups (16 m,
+// job, shell block); members (m quad, theta tile), the theta tiles of one
+// m quad consecutive: they read the same coefficients.
 template <int KIND, int MQ>
-__global__ void __launch_bounds__(64 * MQ) __attribute__((amdgpu_waves_per_eu(2)))
-k_syn4(Syn4Args a) {
+__device__ __forceinline__ void syn4_item(const Syn4Args& a, int b, float2* __restrict__ St) {
     constexpr int NOUT = KIND == 0 ? 1 : 2;
     constexpr int NB = S4_NB;
     constexpr int NMIR = S4_NMIR<MQ>, SHS = S4_SHS<MQ>;
-    __shared__ float2 St[NMIR * S4_SH * SHS];
-    // groups (16 m, job, shell block); members (m quad, theta tile), the
-    // theta tiles of one m quad consecutive: they read the same coefficients
-    const BlockId bid = decode_block(a.nmo * a.njob * a.nsb, (S4_MG / MQ) * a.nth);
+    const BlockId bid = decode_block_at(b, a.nmo * a.njob * a.nsb, (S4_MG / MQ) * a.nth);
     if (!bid.valid) return;
     const int mo = bid.group / (a.njob * a.nsb);
     const int gjs = bid.group - mo * a.njob * a.nsb;
@@ -564,6 +569,38 @@ k_syn4(Syn4Args a) {
     }
 }
 
+template <int KIND, int MQ>
+__global__ void __launch_bounds__(64 * MQ) __attribute__((amdgpu_waves_per_eu(2)))
+k_syn4(Syn4Args a) {
+    __shared__ float2 St[S4_STAGE<MQ>];
+    syn4_item<KIND, MQ>(a, blockIdx.x, St);
+}
+
+// Persistent form: as many workgroups as the CUs hold at once, each pulling
+// work items from its XCD's queue (wq[xcd] counts the items taken, item q of
+// XCD x is block 8 q + x of the static numbering, so the items sharing a
+// table block stay on one XCD and in the longest-slot-first order) until the
+// queue is empty; uneven slot lengths then leave no tail of idle CUs.  The
+// last workgroup to finish resets the queues (wq[8] counts finished
+// workgroups) for the next launch.
+template <int KIND, int MQ>
+__global__ void __launch_bounds__(64 * MQ) __attribute__((amdgpu_waves_per_eu(2)))
+k_syn4p(Syn4Args a, int* __restrict__ wq, int nq) {
+    __shared__ float2 St[S4_STAGE<MQ>];
+    __shared__ int s_q;
+    const int xcd = blockIdx.x & 7;
+    for (;;) {
+        if (threadIdx.x == 0) s_q = atomicAdd(&wq[xcd], 1);
+        __syncthreads();
+        const int q = s_q;
+        if (q >= nq) break;
+        syn4_item<KIND, MQ>(a, 8 * q + xcd, St);
+        __syncthreads();  // the item's staging reads are done before the next item's writes
+    }
+    if (threadIdx.x == 0 && atomicAdd(&wq[8], 1) == (int)gridDim.x - 1)
+        for (int k = 0; k < 9; k++) atomicExch(&wq[k], 0);
+}
+
 static int syn4_launch(Ctx& c, int kind, int njob, const float* tA, const float* tB, const void* const* in0,
                        const void* const* in1, void* const* out0, void* const* out1) {
     Syn4Args a;
@@ -592,7 +629,37 @@ static int syn4_launch(Ctx& c, int kind, int njob, const float* tA, const float*
     a.nmo = (c.M + S4_MG - 1) / S4_MG;
     a.njob = njob;
     auto grid = [&](int mq) { return dim3(grid_blocks(a.nmo * njob * a.nsb, (S4_MG / mq) * a.nth)); };
-    const int mq = c.var[LSD_V_SYN_SCALAR + kind] == 4 ? 4 : 8;
+    const int v = c.var[LSD_V_SYN_SCALAR + kind];
+    const int mq = (v == 4 || v == 5) ? 4 : 8;
+    if (v == 5 || v == 9) {
+        // persistent form: the CUs' worth of workgroups (a multiple of 8,
+        // so each XCD gets the same number) pulling the static blocks
+        const void* fn;
+        if (kind == SYN_SCALAR) fn = mq == 4 ? (const void*)&k_syn4p<0, 4> : (const void*)&k_syn4p<0, 8>;
+        else if (kind == SYN_VECTOR) fn = mq == 4 ? (const void*)&k_syn4p<1, 4> : (const void*)&k_syn4p<1, 8>;
+        else fn = mq == 4 ? (const void*)&k_syn4p<2, 4> : (const void*)&k_syn4p<2, 8>;
+        int& pg = c.syn_pgrid[kind][mq == 8 ? 1 : 0];
+        if (pg == 0) {
+            int per = 0;
+            LSD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, 64 * mq, 0));
+            pg = (std::max(per, 1) * c.ncu + 7) / 8 * 8;
+        }
+        const unsigned nb = grid(mq).x;
+        const dim3 pgrid(std::min((unsigned)pg, nb));
+        const int nq = (int)(nb / 8);
+        if (kind == SYN_SCALAR) {
+            if (mq == 4) hipLaunchKernelGGL((k_syn4p<0, 4>), pgrid, dim3(64 * 4), 0, c.stream, a, c.d_wq, nq);
+            else hipLaunchKernelGGL((k_syn4p<0, 8>), pgrid, dim3(64 * 8), 0, c.stream, a, c.d_wq, nq);
+        } else if (kind == SYN_VECTOR) {
+            if (mq == 4) hipLaunchKernelGGL((k_syn4p<1, 4>), pgrid, dim3(64 * 4), 0, c.stream, a, c.d_wq, nq);
+            else hipLaunchKernelGGL((k_syn4p<1, 8>), pgrid, dim3(64 * 8), 0, c.stream, a, c.d_wq, nq);
+        } else {
+            if (mq == 4) hipLaunchKernelGGL((k_syn4p<2, 4>), pgrid, dim3(64 * 4), 0, c.stream, a, c.d_wq, nq);
+            else hipLaunchKernelGGL((k_syn4p<2, 8>), pgrid, dim3(64 * 8), 0, c.stream, a, c.d_wq, nq);
+        }
+        LSD_HIP(hipGetLastError());
+        return LSD_OK;
+    }
     if (kind == SYN_SCALAR) {
         if (mq == 4) hipLaunchKernelGGL((k_syn4<0, 4>), grid(4), dim3(64 * 4), 0, c.stream, a);
         else hipLaunchKernelGGL((k_syn4<0, 8>), grid(8), dim3(64 * 8), 0, c.stream, a);

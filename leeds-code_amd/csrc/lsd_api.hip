@@ -107,7 +107,8 @@ static void release(Ctx* c) {
                     c->d_tw, c->d_partial, c->d_cfl_d, c->d_cfl_part, c->d_cfl, c->d_tabb, c->d_atabb,
                     c->d_bx_sbase, c->d_icr, c->d_icdr1, c->d_icdr2, c->d_icradlap, c->d_iclu, c->d_icpiv,
                     c->d_icy, c->d_icblk, c->d_icord, c->d_icpartial, c->d_rot, c->d_gwsin,
-                    c->d_icf[0], c->d_icf[1], c->d_icn[0], c->d_icn[1], c->d_ichist[0], c->d_ichist[1]};
+                    c->d_icf[0], c->d_icf[1], c->d_icn[0], c->d_icn[1], c->d_ichist[0], c->d_ichist[1],
+                    c->d_wq};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (int k = 0; k < LSD_NUM_COLL; k++)
@@ -165,7 +166,7 @@ static bool variant_ok(const Ctx& c, int which, int value) {
     switch (which) {
         case LSD_V_SYN_SCALAR:
         case LSD_V_SYN_VECTOR:
-        case LSD_V_SYN_GRAD: return value == 4 || value == 8;
+        case LSD_V_SYN_GRAD: return value == 4 || value == 8 || value == 5 || value == 9;
         case LSD_V_ANA: return value == 1 || (value == 0 && c.ana_wide_ok);
         case LSD_V_SOLVE: return (value == 0 && c.solve_direct_ok) || (value == 1 && c.hb_form[1] >= 1);
         default: return false;
@@ -318,7 +319,8 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
     LSD_HIP(hipHostMalloc((void**)&c->h_partial, sizeof(double) * 8 * nblk_max, hipHostMallocDefault));
     // rotation state; with a conducting inner core its mesh, operators,
     // fields and the blocks of its coupled solves
-    if (dalloc(&c->d_rot, ROT_COUNT) || dalloc(&c->d_gwsin, Th)) return LSD_E_HIP;
+    if (dalloc(&c->d_rot, ROT_COUNT) || dalloc(&c->d_gwsin, Th) || dalloc(&c->d_wq, 16)) return LSD_E_HIP;
+    LSD_HIP(hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device));
     std::vector<int> icorder;
     std::vector<int4> icblocks;
     if (c->Nic > 1) {
@@ -355,6 +357,7 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
     LSD_HIP(hipMemcpyAsync(c->d_hm, c->hm.data(), sizeof(int) * H, hipMemcpyHostToDevice, st));
     LSD_HIP(hipMemcpyAsync(c->d_pidx, pidx.data(), sizeof(int) * pidx.size(), hipMemcpyHostToDevice, st));
     LSD_HIP(hipMemcpyAsync(c->d_apos, apos.data(), sizeof(int) * H, hipMemcpyHostToDevice, st));
+    LSD_HIP(hipMemsetAsync(c->d_wq, 0, sizeof(int) * 16, st));
     if (c->Nic > 1) {
         LSD_HIP(hipMemcpyAsync(c->d_icord, icorder.data(), sizeof(int) * H, hipMemcpyHostToDevice, st));
         LSD_HIP(hipMemcpyAsync(c->d_icblk, icblocks.data(), sizeof(int4) * icblocks.size(), hipMemcpyHostToDevice,
@@ -1044,6 +1047,12 @@ static int tune(Ctx& c, int reps, const char* cache, double* tus) {
                 RC(time_us(c, reps, [&]() { return synthesise(c, true, false); }, &t[k][v]));
             }
             c.var[k] = t[k][0] <= t[k][1] ? 4 : 8;
+            // the persistent form of the faster shape
+            double tp = 0.0;
+            const int best = c.var[k];
+            c.var[k] = best + 1;
+            RC(time_us(c, reps, [&]() { return synthesise(c, true, false); }, &tp));
+            if (tp >= std::min(t[k][0], t[k][1])) c.var[k] = best;
         }
         if (c.ana_wide_ok) {
             for (int v = 0; v < 2; v++) {

@@ -260,6 +260,11 @@ struct Ctx {
     double* d_rot = nullptr;           // [ROT_COUNT], see RotSlot
     double* d_gwsin = nullptr;         // [Th] leg_gwP0 sin(theta)
     int h10 = -1, h11 = -1;            // harmonics (l, m) = (1, 0), (1, 1) (-1 if absent)
+    // persistent kernel forms: work-queue counters (zero between launches),
+    // the device's CU count and the workgroups of each persistent launch
+    int* d_wq = nullptr;               // [16]
+    int ncu = 0;
+    int syn_pgrid[3][2] = {};
     bool ic() const { return Nic > 1 && cfg.mag; }
     // the rotating inner core moves the wall: b_rot_tstep with a no-slip
     // inner boundary (vel_setbc_Tor, non_rotation)

@@ -1,7 +1,7 @@
 """Kernel variants (lsd_set_variant) and the autotuner (lsd_tune).
 
 Each variant computes the same step: the FP32 synthesis shapes (4 or 8 m
-per workgroup) bit for bit, the FP32 analysis forms (wide-load and
+per workgroup, static or persistent) bit for bit, the FP32 analysis forms (wide-load and
 LDS-staged) and the two solve forms (two-pass register-column FP64, and
 LDS-column: FP64 in the plain form, FP32 factors with one FP64-residual
 refinement sweep in the mixed form) to rounding.  At L = M = 170, N = 96 the
@@ -46,6 +46,25 @@ def test_synthesis_shapes_bit_for_bit():
     other, ev2 = run(cfg, st, 3, syn0=8, syn1=4, syn2=4)
     assert ev == ev2
     for (a, b), (c, d) in zip(base, other):
+        assert np.array_equal(a, c) and np.array_equal(b, d)
+    # the persistent forms (5, 9): the same work items pulled from queues
+    for shapes in ((5, 9, 9), (9, 5, 5)):
+        pers, ev3 = run(cfg, st, 3, syn0=shapes[0], syn1=shapes[1], syn2=shapes[2])
+        assert ev3 == ev
+        for (a, b), (c, d) in zip(base, pers):
+            assert np.array_equal(a, c) and np.array_equal(b, d)
+
+
+def test_persistent_synthesis_at_config3():
+    """The persistent synthesis at L = M = 170, N = 96 (about 2 work items
+    per CU, uneven slot lengths): bit for bit the static form, over steps
+    replayed from graphs (the queues reset themselves between launches)."""
+    cfg = config3(MIXED_F32)
+    st = seeded_state(cfg, SEEDS[3], AMPS[3])
+    base, ev = run(cfg, st, 3, syn0=4, syn1=8, syn2=8)
+    pers, ev2 = run(cfg, st, 3, syn0=5, syn1=9, syn2=9)
+    assert ev == ev2
+    for (a, b), (c, d) in zip(base, pers):
         assert np.array_equal(a, c) and np.array_equal(b, d)
 
 
