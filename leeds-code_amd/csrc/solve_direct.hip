@@ -95,6 +95,27 @@ __device__ __forceinline__ void store_pair(double2* col, int n, int part, double
     if (act) col[n + part] = part ? make_double2(recv, v1) : make_double2(v0, recv);
 }
 
+// The same for a column of FP32 (Re, Im) rows (8 bytes per row): the saved N
+// of the mixed forms, whose N comes from FP32 transforms.
+__device__ __forceinline__ float swap_pair(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ void load_pair(const float2* col, int n, int part, double& v0, double& v1) {
+    const float2 v = col[n + part];
+    const float recv = swap_pair(part ? v.x : v.y);
+    v0 = (double)(part ? recv : v.x);
+    v1 = (double)(part ? v.y : recv);
+}
+__device__ __forceinline__ void store_pair(float2* col, int n, int part, double v0, double v1, bool act) {
+    const float f0 = (float)v0, f1 = (float)v1;
+    const float recv = swap_pair(part ? f0 : f1);
+    if (act) col[n + part] = part ? make_float2(recv, f1) : make_float2(f0, recv);
+}
+
+// Storage of the saved N: FP64, or FP32 in the mixed forms.
+template <bool H32>
+using HistT = typename std::conditional<H32, float2, double2>::type;
+
 // LU kind of the (main) solve of field f; the poloidal velocity's XGre solve.
 __host__ __device__ inline int lu_kind(int f) { return f == 0 ? 0 : (f == 1 ? 2 : f + 1); }
 
@@ -119,6 +140,7 @@ __device__ __forceinline__ double2 band_acc(double2 acc, const double* __restric
 constexpr int RHS_BLK = 1024;
 constexpr int RHS_IT = (32 * 96 + RHS_BLK - 1) / RHS_BLK;  // points per thread at 32 harmonics, N <= 96
 
+template <bool H32>
 __global__ void __launch_bounds__(RHS_BLK) k_rhs_pred(DirArgs a, float inv_n) {
     extern __shared__ double2 xs_l[];  // [nh][N] state columns, then the band
     const int f = blockIdx.y;
@@ -129,7 +151,7 @@ __global__ void __launch_bounds__(RHS_BLK) k_rhs_pred(DirArgs a, float inv_n) {
     double* band = reinterpret_cast<double*>(xs_l + (size_t)nh * N);
     const double2* x = pick(a.x, f);
     const double2* nn = pick(a.nn, f);
-    double2* hist = pick(a.hist, f);
+    HistT<H32>* hist = reinterpret_cast<HistT<H32>*>(pick(a.hist, f));
     double2* rhs = pick(a.rhs, f);
     const bool has_y = f == 2 || f == 3 || l != 0;
     const double* yb = a.y + ((size_t)f * a.L + l) * N * 7;
@@ -157,7 +179,8 @@ __global__ void __launch_bounds__(RHS_BLK) k_rhs_pred(DirArgs a, float inv_n) {
         const size_t idx = (size_t)h * N + n;
         const double keep = (n == 0 || n == N - 1) ? 0.0 : 1.0;
         const double2 nv = nvs[k];
-        hist[idx] = nv;
+        if constexpr (H32) hist[idx] = make_float2((float)nv.x, (float)nv.y);
+        else hist[idx] = nv;
         const double2* xc = xs_l + hh * N;
         if (f == 1 && l != 0) {
             // XGre acts on N with zero boundary rows; the Y x part of the
@@ -375,7 +398,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
 // Corrector: b = c (N* - N) formed in registers (N* in the column, the saved
 // N streamed and replaced), x += X^-1 b, with the tim_addcorr norms; one
 // wave per (solve block, field).
-template <int N>
+template <int N, bool H32>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_scorr(DirArgs a) {
     static_assert(N % DIR_BATCH == 0, "N must be a multiple of the batch");
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -396,7 +419,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
         const bool solve = f == 2 || f == 3 || l != 0;
         double2* xcol = pick(a.x, f) + (size_t)h * N;
         const double2* ncol = pick(a.nn, f) + (size_t)h * N;
-        double2* hcol = pick(a.hist, f) + (size_t)h * N;
+        HistT<H32>* hcol = reinterpret_cast<HistT<H32>*>(pick(a.hist, f)) + (size_t)h * N;
         // N*, every row at once, issued ahead of the factor staging (8-byte
         // loads into the column; the streamed columns below move by pairs)
         const double* nsrc = reinterpret_cast<const double*>(ncol) + part;
@@ -543,7 +566,8 @@ template <int N, bool PRED>
 static void launch_sdir(Ctx& c, const DirArgs& a) {
     const dim3 grid(c.n_partial_blocks, 6);
     if (PRED) hipLaunchKernelGGL(k_spred<N>, grid, dim3(64), sizeof(DirSmem<N>), c.stream, a);
-    else hipLaunchKernelGGL(k_scorr<N>, grid, dim3(64), sizeof(DirSmem<N>), c.stream, a);
+    else if (c.mixed) hipLaunchKernelGGL((k_scorr<N, true>), grid, dim3(64), sizeof(DirSmem<N>), c.stream, a);
+    else hipLaunchKernelGGL((k_scorr<N, false>), grid, dim3(64), sizeof(DirSmem<N>), c.stream, a);
 }
 
 template <bool PRED>
@@ -567,11 +591,18 @@ static int launch_sdir_n(Ctx& c, const DirArgs& a) {
 int launch_solve_direct(Ctx& c, bool pred) {
     const DirArgs a = dir_args(c);
     const int nf = 2 + (c.cfg.cod ? 1 : 0) + (c.cfg.comp ? 1 : 0) + (c.cfg.mag && !c.ic() ? 2 : 0);
-    ProfScope ps(c, LSD_K_SOLVE, (pred ? 6.0 : 5.0) * 16.0 * c.coll_elems() * nf);
+    // bytes per element and field: predictor 64 + 32 (48 + 8 + 32 with the
+    // FP32 saved N of the mixed forms), corrector 80 (64)
+    const double per = pred ? (c.mixed ? 88.0 : 96.0) : (c.mixed ? 64.0 : 80.0);
+    ProfScope ps(c, LSD_K_SOLVE, per * c.coll_elems() * nf);
     if (pred) {
         const size_t lds = sizeof(double2) * (size_t)c.solve_hb * c.N + sizeof(double) * 7 * (size_t)c.N;
-        hipLaunchKernelGGL(k_rhs_pred, dim3(c.n_partial_blocks, 6), dim3(RHS_BLK), lds, c.stream, a,
-                           1.0f / (float)c.N);
+        if (c.mixed)
+            hipLaunchKernelGGL(k_rhs_pred<true>, dim3(c.n_partial_blocks, 6), dim3(RHS_BLK), lds, c.stream, a,
+                               1.0f / (float)c.N);
+        else
+            hipLaunchKernelGGL(k_rhs_pred<false>, dim3(c.n_partial_blocks, 6), dim3(RHS_BLK), lds, c.stream, a,
+                               1.0f / (float)c.N);
         LSD_HIP(hipGetLastError());
     }
     return pred ? launch_sdir_n<true>(c, a) : launch_sdir_n<false>(c, a);
