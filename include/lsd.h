@@ -36,10 +36,14 @@ extern "C" {
 
 /* Precision of the step.  PLAIN keeps every operation in FP64 and is the
  * drop-in; the mixed forms keep the state, the P_l^m recursion and the LU
- * factors in FP64. */
+ * factors in FP64.  MIXED_F32_MAG64 runs the Legendre sums of B and curl B
+ * in FP64 (rounded to FP32 after the sum) and everything else as
+ * MIXED_F32: the FP32 rounding of the Lorentz force's inputs is the largest
+ * error term of the mixed form on large grids. */
 #define LSD_PLAIN_F64     0
 #define LSD_MIXED_F32     1   /* FP32 transforms + FP32 solves, FP64 refinement   */
 #define LSD_MIXED_BF16X3  2   /* as F32, Legendre on bf16x3 split MFMA            */
+#define LSD_MIXED_F32_MAG64 3 /* as F32, B and curl B synthesised in FP64        */
 
 /* Run parameters; names follow Scripts/parameters.F90_template:19-264. */
 typedef struct lsd_config {

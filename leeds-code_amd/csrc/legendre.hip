@@ -120,6 +120,7 @@ struct SynArgs {
     const T* in1[LG_MAXJ];
     T* out0[LG_MAXJ];         // fourier [n][Th][MP] complex
     T* out1[LG_MAXJ];
+    int out32;                // outputs rounded to FP32 (float2 Fourier layout)
 };
 
 constexpr int SY_TH = 16;     // theta points per tile (the MFMA's 16 columns)
@@ -269,13 +270,15 @@ __global__ void __launch_bounds__(64 * SY_WAVES) k_syn(SynArgs<T> a) {
         v.x = f * sp[0];
         v.y = f * sp[1];
         T* out = o == 0 ? pickj(a.out0, job) : pickj(a.out1, job);
-        reinterpret_cast<T2*>(out)[((size_t)n * a.Th + jrow) * a.MP + m0 + mi] = v;
+        const size_t at = ((size_t)n * a.Th + jrow) * a.MP + m0 + mi;
+        if (a.out32) reinterpret_cast<float2*>(out)[at] = make_float2((float)v.x, (float)v.y);
+        else reinterpret_cast<T2*>(out)[at] = v;
     }
 }
 
 template <typename T>
 static int syn_launch(Ctx& c, int kind, int njob, const T* tA, const T* tB, const void* const* in0,
-                      const void* const* in1, void* const* out0, void* const* out1) {
+                      const void* const* in1, void* const* out0, void* const* out1, bool out32 = false) {
     if (njob < 1 || njob > LG_MAXJ) {
         set_error("synthesis: 1..6 transforms per launch");
         return LSD_E_ARG;
@@ -292,6 +295,7 @@ static int syn_launch(Ctx& c, int kind, int njob, const T* tA, const T* tB, cons
     a.tabA = tA;
     a.tabB = tB;
     a.rinv = c.d_r + 2 * c.N;
+    a.out32 = out32 ? 1 : 0;
     for (int k = 0; k < LG_MAXJ; k++) {
         const int q = k < njob ? k : 0;
         a.in0[k] = reinterpret_cast<const T*>(in0[q]);
@@ -320,9 +324,10 @@ static int syn4_launch(Ctx& c, int kind, int njob, const float* tA, const float*
 // kind SYN_VECTOR: (in0[k] = s, in1[k] = t) -> (out0[k] = theta, out1[k] = phi)
 // kind SYN_GRAD:   in0[k] -> (out0[k] = theta, out1[k] = phi)
 // f32: FP32 tables, inputs and outputs (float2 spectral / Fourier layouts),
-// on the FP32 kernel below.
+// on the FP32 kernel below.  out32 (with f32 false): FP64 tables, inputs and
+// sums, outputs rounded to FP32.
 int launch_synthesis_group(Ctx& c, int kind, int njob, const void* const* in0, const void* const* in1,
-                           void* const* out0, void* const* out1, bool f32) {
+                           void* const* out0, void* const* out1, bool f32, bool out32) {
     size_t HT = (size_t)c.H * c.Th2;
     const double unit = 4.0 * (double)c.H * c.Th2 * c.N;  // flops per product unit
     ProfScope ps(c, LSD_K_SYNTH, unit * (kind == SYN_SCALAR ? 1 : kind == SYN_VECTOR ? 4 : 2) * njob);
@@ -339,7 +344,7 @@ int launch_synthesis_group(Ctx& c, int kind, int njob, const void* const* in0, c
         const size_t p1 = (kind == SYN_VECTOR ? 4 : 2) * HTp;
         return syn4_launch(c, kind, njob, c.d_tab32 + p0, c.d_tab32 + p1, in0, in1, out0, out1);
     }
-    return syn_launch<double>(c, kind, njob, c.d_tab + o0, c.d_tab + o1, in0, in1, out0, out1);
+    return syn_launch<double>(c, kind, njob, c.d_tab + o0, c.d_tab + o1, in0, in1, out0, out1, out32);
 }
 
 int launch_synthesis_p(Ctx& c, int kind, const void* in0, const void* in1, void* out0, void* out1, bool f32) {

@@ -45,8 +45,8 @@ static int check_config(const lsd_config* c, int* H, int* Th, int* Ph) {
         set_error("i_time_step_control must be 1 or 2 (0 selects 2)");
         return LSD_E_CONFIG;
     }
-    if (c->precision != LSD_PLAIN_F64 && c->precision != LSD_MIXED_F32 && c->precision != LSD_MIXED_BF16X3) {
-        set_error("precision must be LSD_PLAIN_F64, LSD_MIXED_F32 or LSD_MIXED_BF16X3");
+    if (c->precision < LSD_PLAIN_F64 || c->precision > LSD_MIXED_F32_MAG64) {
+        set_error("precision must be LSD_PLAIN_F64, LSD_MIXED_F32, LSD_MIXED_BF16X3 or LSD_MIXED_F32_MAG64");
         return LSD_E_CONFIG;
     }
     if (c->precision != LSD_PLAIN_F64 && ph > 1024) {
@@ -179,13 +179,21 @@ static void load_tuning(Ctx& c) {
     if (!f) return;
     char line[1024];
     TuneEntry e, hit;
-    bool found = false;
-    while (std::fgets(line, sizeof line, f))
-        if (parse_tune_line(line, e) && e.L == c.L && e.M == c.M && e.Mp == c.Mp && e.N == c.N && e.Th == c.Th &&
-            e.Ph == c.Ph && e.prec == c.cfg.precision) {
+    bool found = false, own = false;
+    while (std::fgets(line, sizeof line, f)) {
+        if (!(parse_tune_line(line, e) && e.L == c.L && e.M == c.M && e.Mp == c.Mp && e.N == c.N && e.Th == c.Th &&
+              e.Ph == c.Ph))
+            continue;
+        if (e.prec == c.cfg.precision) {
+            hit = e;
+            found = own = true;
+        } else if (c.cfg.precision == LSD_MIXED_F32_MAG64 && e.prec == LSD_MIXED_F32 && !own) {
+            // the FP64-magnetic form runs the FP32 form's kernels for everything
+            // else, so it takes that form's choices when it has none of its own
             hit = e;
             found = true;
         }
+    }
     std::fclose(f);
     if (!found) return;
     for (int k = 0; k < LSD_V_COUNT; k++)
@@ -223,6 +231,7 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
     c->adaptive = cfg->dt < 0.0;
     c->dt = std::fabs(cfg->dt);
     c->mixed = cfg->precision != LSD_PLAIN_F64;
+    c->mag64 = cfg->precision == LSD_MIXED_F32_MAG64 && cfg->mag;
     c->Nic = cfg->Nic > 1 ? cfg->Nic : 1;
     c->NT = c->Nic + cfg->N - 1;
     *out = c;
@@ -573,6 +582,15 @@ static int synthesise(Ctx& c, bool f, bool to_phys) {
     void* g_th[2];
     void* g_ph[2];
     int ns = 0, nv = 0, ng = 0;
+    // B and curl B of the LSD_MIXED_F32_MAG64 form: FP64 preparation and
+    // Legendre sums, FP32 Fourier outputs (two scalar, two vector jobs)
+    const bool m64 = f && c.mag64;
+    const void* m_sc_in[2];
+    void* m_sc_out[2];
+    const void* m_v_s[2];
+    const void* m_v_t[2];
+    void* m_v_th[2];
+    void* m_v_ph[2];
     // vel_angmomclr before the velocity transform (program/main.F90:100)
     RC(launch_angmomclr(c));
     // the vector fields (u; B) and the scalar fields (C; xi) are prepared in
@@ -583,9 +601,24 @@ static int synthesise(Ctx& c, bool f, bool to_phys) {
     int nvf = 0;
     auto vec = [&](int T, int P, int si, int pr) {
         void** o = vo[nvf];
-        for (int k = 0; k < 6; k++) o[k] = spec_in_p(c, si + k, f);
+        // the FP32 slots fill the first half of the allocation, so the FP64
+        // inputs of B and curl B take FP64 slots from the second half
+        const bool own = m64 && T == LSD_MAG_TOR;
+        static_assert(SI_COUNT / 2 + 6 <= SI_COUNT, "six FP64 slots in the second half");
+        for (int k = 0; k < 6; k++) o[k] = own ? spec_in_p(c, SI_COUNT / 2 + k, false) : spec_in_p(c, si + k, f);
         vT[nvf] = c.d_f[T];
         vP[nvf++] = c.d_f[P];
+        if (own) {
+            for (int u = 0; u < 2; u++) {
+                m_sc_in[u] = o[3 * u];
+                m_sc_out[u] = fourier_p(c, pr + 3 * u, true);
+                m_v_s[u] = o[3 * u + 1];
+                m_v_t[u] = o[3 * u + 2];
+                m_v_th[u] = fourier_p(c, pr + 3 * u + 1, true);
+                m_v_ph[u] = fourier_p(c, pr + 3 * u + 2, true);
+            }
+            return;
+        }
         // q of the field and of its curl (scalar), s and t of both (vector)
         sc_in[ns] = o[0];
         sc_out[ns++] = fourier_p(c, pr, f);
@@ -602,7 +635,11 @@ static int synthesise(Ctx& c, bool f, bool to_phys) {
     };
     vec(LSD_VEL_TOR, LSD_VEL_POL, SI_UQ, LSD_P_VEL_R);
     if (c.cfg.mag) vec(LSD_MAG_TOR, LSD_MAG_POL, SI_BQ, LSD_P_MAG_R);
-    {
+    if (m64 && nvf == 2) {
+        void* const* po[2] = {vo[0], vo[1]};
+        RC(launch_vec_prep(c, 1, vT, vP, po, true));
+        RC(launch_vec_prep(c, 1, vT + 1, vP + 1, po + 1, false));
+    } else {
         void* const* po[2] = {vo[0], vo[1]};
         RC(launch_vec_prep(c, nvf, vT, vP, po, f));
     }
@@ -631,6 +668,10 @@ static int synthesise(Ctx& c, bool f, bool to_phys) {
     RC(launch_synthesis_group(c, SYN_SCALAR, ns, sc_in, nullptr, sc_out, nullptr, f));
     RC(launch_synthesis_group(c, SYN_VECTOR, nv, v_s, v_t, v_th, v_ph, f));
     if (ng) RC(launch_synthesis_group(c, SYN_GRAD, ng, g_in, nullptr, g_th, g_ph, f));
+    if (m64 && nvf == 2) {
+        RC(launch_synthesis_group(c, SYN_SCALAR, 2, m_sc_in, nullptr, m_sc_out, nullptr, false, true));
+        RC(launch_synthesis_group(c, SYN_VECTOR, 2, m_v_s, m_v_t, m_v_th, m_v_ph, false, true));
+    }
     // non_rotation after the transforms (program/main.F90:108)
     RC(launch_torques(c, f));
     if (to_phys && !f) {

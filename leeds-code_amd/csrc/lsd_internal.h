@@ -111,6 +111,7 @@ struct Ctx {
     int MP;                        // Fourier row length: M rounded up to 16
     double dt;
     bool mixed = false;            // FP32 transforms and solves (LSD_MIXED_*)
+    bool mag64 = false;            // B and curl B synthesised in FP64 (LSD_MIXED_F32_MAG64)
     hipStream_t stream = nullptr;
 
     std::vector<int> hl, hm;       // per harmonic l, m_
@@ -317,7 +318,7 @@ int launch_analysis(Ctx& c, int kind, const double2* g0, const double2* g1, cons
 
 int launch_synthesis_p(Ctx& c, int kind, const void* in0, const void* in1, void* out0, void* out1, bool f32);
 int launch_synthesis_group(Ctx& c, int kind, int njob, const void* const* in0, const void* const* in1,
-                           void* const* out0, void* const* out1, bool f32);
+                           void* const* out0, void* const* out1, bool f32, bool out32 = false);
 int launch_analysis_group(Ctx& c, int kind, int njob, const void* const* g0, const void* const* g1,
                           const void* const* g2, void* const* o0, void* const* o1, void* const* o2, bool f32);
 int launch_analysis_p(Ctx& c, int kind, const void* g0, const void* g1, const void* g2, void* o0, void* o1,

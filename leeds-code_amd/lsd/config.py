@@ -11,6 +11,7 @@ import math
 PLAIN_F64 = 0
 MIXED_F32 = 1
 MIXED_BF16X3 = 2
+MIXED_F32_MAG64 = 3  # as MIXED_F32, with B and curl B synthesised in FP64
 
 
 class LsdConfig(ctypes.Structure):

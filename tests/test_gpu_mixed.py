@@ -15,6 +15,7 @@ import numpy as np
 import pytest
 
 from lsd import config1, config3, config_small, grid_for, make_config, seeded_state, MIXED_F32, PLAIN_F64, SEEDS, AMPS
+from lsd import MIXED_F32_MAG64
 from lsd.gpu import Context
 from lsd.initial import FIELD_ID
 
@@ -46,9 +47,12 @@ def pair(cfg_fn, seed, amp):
     return cp, gp, gm
 
 
-@pytest.mark.parametrize("which", ["small", "config1"])
+@pytest.mark.parametrize("which", ["small", "config1", "config1_mag64"])
 def test_mixed_small_cases(which):
     fn = config_small if which == "small" else config1
+    if which.endswith("mag64"):
+        def fn(precision, _f=fn):
+            return _f(precision=MIXED_F32_MAG64 if precision == MIXED_F32 else precision)
     cfg, gp, gm = pair(fn, 5, 1e-3)
     print("case", which, cfg.sizes(), flush=True)
     sp, sm = gp.step(10), gm.step(10)
@@ -117,14 +121,18 @@ def test_mixed_large_grid_two_steps():
     thread per phi point of 1024), and the LDS-column solves (N = 128) with
     FP32 factors and one FP64-residual sweep.
 
-    Bound: 2e-6 after 2 steps (measured 1.11e-6 in uT, every other field
-    below 3e-9).  The uT error is the FP32 rounding of the Lorentz force,
-    which enters with the factor 2 Pm / E = 4e4: it is flat across the
-    degrees (about 4e-7 of the uT norm in each band of 32 degrees), so it
-    grows with the number of harmonics, while 99 % of the seeded uT norm sits
-    at l < 32.  With the Lorentz force left out of both forms it falls to
-    1.5e-7; with FP64 solves in the mixed form it is unchanged.  At
-    L = M = 170 the same state gives 5.3e-7 after 2 steps."""
+    Bound for LSD_MIXED_F32: 2e-6 after 2 steps (measured 1.11e-6 in uT,
+    every other field below 3e-9).  The uT error is the FP32 rounding of the
+    Lorentz force, which enters with the factor 2 Pm / E = 4e4: it is flat
+    across the degrees (about 4e-7 of the uT norm in each band of 32
+    degrees), so it grows with the number of harmonics, while 99 % of the
+    seeded uT norm sits at l < 32.  With the Lorentz force left out of both
+    forms it falls to 1.5e-7; with FP64 solves in the mixed form it is
+    unchanged.  At L = M = 170 the same state gives 5.3e-7 after 2 steps.
+    Rounding each stage of the oracle to FP32 in turn puts the largest share
+    on the FP32 Legendre sums of B and curl B, then the analysis sums, the
+    FFTs and the rounded tables; LSD_MIXED_F32_MAG64 runs the sums of B and
+    curl B in FP64 and is bounded by 1e-6 in every field."""
     base = config3()
 
     def cfg_fn(precision):
@@ -137,13 +145,23 @@ def test_mixed_large_grid_two_steps():
     assert sp.evaluations == sm.evaluations
     e = field_errors(gm, gp, cfg)
     print("after 2 steps", {k: "%.2e" % v for k, v in e.items()}, flush=True)
+    gm.close()
     for k, v in e.items():
         assert v < (2e-6 if k == "uT" else 1e-6), (k, v)
+    cb = cfg_fn(precision=MIXED_F32_MAG64)
+    gb = Context(cb)
+    gb.set_state(seeded_state(cb, SEEDS[3], AMPS[3]))
+    sb = gb.step(2)
+    assert sb.evaluations == sp.evaluations
+    e = field_errors(gb, gp, cfg)
+    print("FP64 sums of B and curl B, after 2 steps", {k: "%.2e" % v for k, v in e.items()}, flush=True)
+    gb.close()
     gp.close()
-    gm.close()
+    for k, v in e.items():
+        assert v < 1e-6, (k, v)
 
 
-@pytest.mark.parametrize("prec", [PLAIN_F64, MIXED_F32])
+@pytest.mark.parametrize("prec", [PLAIN_F64, MIXED_F32, MIXED_F32_MAG64])
 def test_graph_replays_match_eager(prec):
     """The captured-graph step against host-launched steps: every replay,
     with eager work and synchronisations of the program's own between
