@@ -320,7 +320,8 @@ int lsd_sync(lsd_ctx* ctx);
  *            the same shapes as persistent kernels (the CUs' worth of
  *            workgroups pulling the work items from per-XCD queues);
  *            defaults 4, 8, 8
- *  ANA       FP32 analysis: 0 wide-load (even Th/2 only; default), 1 LDS-staged
+ *  ANA       FP32 analysis: 0 wide-load (even Th/2 only; default), 1 LDS-staged,
+ *            2 the wide-load kernel in the persistent form
  *  SOLVE     banded solves: 0 two-pass register-column FP64 (N in {32, 48,
  *            64, 96}; default there), 1 LDS-column (the mixed forms: FP32
  *            factors with one FP64-residual refinement sweep) */

@@ -926,14 +926,14 @@ struct A4Ops {
     float4 w[2][3];  // tables P, L, S at the two theta of each K step
 };
 
+// One work item (block b of the static numbering) of the FP32 analysis.
 template <bool VEC>
-__global__ void __launch_bounds__(64 * A4_W) __attribute__((amdgpu_waves_per_eu(2)))
-k_ana4(Ana4Args a) {
+__device__ __forceinline__ void ana4_item(const Ana4Args& a, int b) {
     constexpr int NO = VEC ? 3 : 1;
     constexpr int NG = VEC ? 5 : 1;
     // groups (slot, job), longest slots first; members (class, 64-harmonic
     // block, shell-block pair) on one XCD, reading the same folded rows
-    const BlockId bid = decode_block(a.ngroups, 2 * a.maxhb * a.nsbp);
+    const BlockId bid = decode_block_at(b, a.ngroups, 2 * a.maxhb * a.nsbp);
     if (!bid.valid) return;
     const int job = bid.group % a.njob;
     const int sl = uniform(a.order[bid.group / a.njob]);
@@ -1050,6 +1050,33 @@ k_ana4(Ana4Args a) {
     }
 }
 
+template <bool VEC>
+__global__ void __launch_bounds__(64 * A4_W) __attribute__((amdgpu_waves_per_eu(2)))
+k_ana4(Ana4Args a) {
+    ana4_item<VEC>(a, blockIdx.x);
+}
+
+// Persistent form: the CUs' worth of workgroups pulling items from per-XCD
+// queues (item q of XCD x is block 8 q + x of the static numbering), as the
+// persistent synthesis does; the item has no barriers, so every wave comes
+// back to the loop.  The last workgroup to finish resets the queues.
+template <bool VEC>
+__global__ void __launch_bounds__(64 * A4_W) __attribute__((amdgpu_waves_per_eu(2)))
+k_ana4p(Ana4Args a, int* __restrict__ wq, int nq) {
+    __shared__ int s_q;
+    const int xcd = blockIdx.x & 7;
+    for (;;) {
+        if (threadIdx.x == 0) s_q = atomicAdd(&wq[xcd], 1);
+        __syncthreads();
+        const int q = s_q;
+        __syncthreads();  // every wave has read s_q before the next write
+        if (q >= nq) break;
+        ana4_item<VEC>(a, 8 * q + xcd);
+    }
+    if (threadIdx.x == 0 && atomicAdd(&wq[8], 1) == (int)gridDim.x - 1)
+        for (int k = 0; k < 9; k++) atomicExch(&wq[k], 0);
+}
+
 static int ana4_launch(Ctx& c, int kind, int njob, const void* const* g0, const void* const* g1,
                        const void* const* g2, void* const* o0, void* const* o1, void* const* o2) {
     Ana4Args a;
@@ -1079,6 +1106,22 @@ static int ana4_launch(Ctx& c, int kind, int njob, const void* const* g0, const 
     a.njob = njob;
     a.ngroups = c.nslot * njob;
     dim3 grid(grid_blocks(a.ngroups, 2 * a.maxhb * a.nsbp));
+    if (c.var[LSD_V_ANA] == 2) {
+        // persistent form on the analysis half of the queue counters
+        int& pg = c.ana_pgrid[kind == ANA_SCALAR ? 0 : 1];
+        if (pg == 0) {
+            int per = 0;
+            const void* fn = kind == ANA_SCALAR ? (const void*)&k_ana4p<false> : (const void*)&k_ana4p<true>;
+            LSD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, 64 * A4_W, 0));
+            pg = (std::max(per, 1) * c.ncu + 7) / 8 * 8;
+        }
+        const dim3 pgrid(std::min((unsigned)pg, grid.x));
+        const int nq = (int)(grid.x / 8);
+        if (kind == ANA_SCALAR) hipLaunchKernelGGL(k_ana4p<false>, pgrid, dim3(64 * A4_W), 0, c.stream, a, c.d_wq + 16, nq);
+        else hipLaunchKernelGGL(k_ana4p<true>, pgrid, dim3(64 * A4_W), 0, c.stream, a, c.d_wq + 16, nq);
+        LSD_HIP(hipGetLastError());
+        return LSD_OK;
+    }
     if (kind == ANA_SCALAR) hipLaunchKernelGGL(k_ana4<false>, grid, dim3(64 * A4_W), 0, c.stream, a);
     else hipLaunchKernelGGL(k_ana4<true>, grid, dim3(64 * A4_W), 0, c.stream, a);
     LSD_HIP(hipGetLastError());
@@ -1137,7 +1180,7 @@ int launch_analysis_group(Ctx& c, int kind, int njob, const void* const* g0, con
     const double unit = 4.0 * (double)c.H * c.Th2 * c.N;
     ProfScope ps(c, LSD_K_ANAL, unit * (kind == ANA_SCALAR ? 1 : 5) * njob);
     if (f32 && c.cfg.precision == LSD_MIXED_BF16X3) return anab_launch(c, kind, njob, g0, g1, g2, o0, o1, o2);
-    if (f32 && c.d_atab4 && c.var[LSD_V_ANA] == 0) {
+    if (f32 && c.d_atab4 && (c.var[LSD_V_ANA] == 0 || c.var[LSD_V_ANA] == 2)) {
         if (njob < 1 || njob > LG_MAXJ) {
             set_error("analysis: 1..6 transforms per launch");
             return LSD_E_ARG;

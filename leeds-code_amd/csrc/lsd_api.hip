@@ -167,7 +167,7 @@ static bool variant_ok(const Ctx& c, int which, int value) {
         case LSD_V_SYN_SCALAR:
         case LSD_V_SYN_VECTOR:
         case LSD_V_SYN_GRAD: return value == 4 || value == 8 || value == 5 || value == 9;
-        case LSD_V_ANA: return value == 1 || (value == 0 && c.ana_wide_ok);
+        case LSD_V_ANA: return value == 1 || ((value == 0 || value == 2) && c.ana_wide_ok);
         case LSD_V_SOLVE: return (value == 0 && c.solve_direct_ok) || (value == 1 && c.hb_form[1] >= 1);
         default: return false;
     }
@@ -328,7 +328,7 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
     LSD_HIP(hipHostMalloc((void**)&c->h_partial, sizeof(double) * 8 * nblk_max, hipHostMallocDefault));
     // rotation state; with a conducting inner core its mesh, operators,
     // fields and the blocks of its coupled solves
-    if (dalloc(&c->d_rot, ROT_COUNT) || dalloc(&c->d_gwsin, Th) || dalloc(&c->d_wq, 16)) return LSD_E_HIP;
+    if (dalloc(&c->d_rot, ROT_COUNT) || dalloc(&c->d_gwsin, Th) || dalloc(&c->d_wq, 32)) return LSD_E_HIP;
     LSD_HIP(hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device));
     std::vector<int> icorder;
     std::vector<int4> icblocks;
@@ -366,7 +366,7 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
     LSD_HIP(hipMemcpyAsync(c->d_hm, c->hm.data(), sizeof(int) * H, hipMemcpyHostToDevice, st));
     LSD_HIP(hipMemcpyAsync(c->d_pidx, pidx.data(), sizeof(int) * pidx.size(), hipMemcpyHostToDevice, st));
     LSD_HIP(hipMemcpyAsync(c->d_apos, apos.data(), sizeof(int) * H, hipMemcpyHostToDevice, st));
-    LSD_HIP(hipMemsetAsync(c->d_wq, 0, sizeof(int) * 16, st));
+    LSD_HIP(hipMemsetAsync(c->d_wq, 0, sizeof(int) * 32, st));
     if (c->Nic > 1) {
         LSD_HIP(hipMemcpyAsync(c->d_icord, icorder.data(), sizeof(int) * H, hipMemcpyHostToDevice, st));
         LSD_HIP(hipMemcpyAsync(c->d_icblk, icblocks.data(), sizeof(int4) * icblocks.size(), hipMemcpyHostToDevice,
@@ -1101,6 +1101,13 @@ static int tune(Ctx& c, int reps, const char* cache, double* tus) {
                 RC(time_us(c, reps, [&]() { return analyse(c, true); }, &t[LSD_V_ANA][v]));
             }
             c.var[LSD_V_ANA] = t[LSD_V_ANA][0] <= t[LSD_V_ANA][1] ? 0 : 1;
+            if (c.var[LSD_V_ANA] == 0) {
+                // the persistent form of the wide-load kernel
+                double tp = 0.0;
+                c.var[LSD_V_ANA] = 2;
+                RC(time_us(c, reps, [&]() { return analyse(c, true); }, &tp));
+                if (tp >= t[LSD_V_ANA][0]) c.var[LSD_V_ANA] = 0;
+            }
         }
     }
     // solve forms: predictor + corrector on the current state, which is

@@ -263,9 +263,10 @@ struct Ctx {
     int h10 = -1, h11 = -1;            // harmonics (l, m) = (1, 0), (1, 1) (-1 if absent)
     // persistent kernel forms: work-queue counters (zero between launches),
     // the device's CU count and the workgroups of each persistent launch
-    int* d_wq = nullptr;               // [16]
+    int* d_wq = nullptr;               // [32]: synthesis queues [0, 9), analysis [16, 25)
     int ncu = 0;
     int syn_pgrid[3][2] = {};
+    int ana_pgrid[2] = {};
     bool ic() const { return Nic > 1 && cfg.mag; }
     // the rotating inner core moves the wall: b_rot_tstep with a no-slip
     // inner boundary (vel_setbc_Tor, non_rotation)

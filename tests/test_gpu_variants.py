@@ -1,8 +1,8 @@
 """Kernel variants (lsd_set_variant) and the autotuner (lsd_tune).
 
 Each variant computes the same step: the FP32 synthesis shapes (4 or 8 m
-per workgroup, static or persistent) bit for bit, the FP32 analysis forms (wide-load and
-LDS-staged) and the two solve forms (two-pass register-column FP64, and
+per workgroup, static or persistent) bit for bit, the FP32 analysis forms (wide-load,
+static or persistent, bit for bit; LDS-staged to rounding) and the two solve forms (two-pass register-column FP64, and
 LDS-column: FP64 in the plain form, FP32 factors with one FP64-residual
 refinement sweep in the mixed form) to rounding.  At L = M = 170, N = 96 the
 mixed form with FP32 refined solves must stay within 1e-6 of the plain
@@ -76,6 +76,23 @@ def test_analysis_forms_agree():
     e = rel(a, b)
     print("analysis forms differ by %.2e after 10 steps" % e, flush=True)
     assert e < 1e-6
+    # the persistent wide-load form: the same work items, bit for bit
+    p, _ = run(cfg, st, 10, ana=2)
+    for (x, y), (u, v) in zip(a, p):
+        assert np.array_equal(x, u) and np.array_equal(y, v)
+
+
+def test_persistent_analysis_at_config3():
+    """The persistent wide-load analysis at L = M = 170, N = 96, with the
+    persistent synthesis beside it (both sets of queues in use), over steps
+    replayed from graphs: bit for bit the static forms."""
+    cfg = config3(MIXED_F32)
+    st = seeded_state(cfg, SEEDS[3], AMPS[3])
+    base, ev = run(cfg, st, 3, syn0=4, syn1=8, syn2=8, ana=0)
+    pers, ev2 = run(cfg, st, 3, syn0=5, syn1=9, syn2=9, ana=2)
+    assert ev == ev2
+    for (a, b), (c, d) in zip(base, pers):
+        assert np.array_equal(a, c) and np.array_equal(b, d)
 
 
 def test_solve_forms_agree_plain():
