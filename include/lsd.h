@@ -324,7 +324,9 @@ int lsd_sync(lsd_ctx* ctx);
  *            2 the wide-load kernel in the persistent form
  *  SOLVE     banded solves: 0 two-pass register-column FP64 (N in {32, 48,
  *            64, 96}; default there), 1 LDS-column (the mixed forms: FP32
- *            factors with one FP64-residual refinement sweep) */
+ *            factors with one FP64-residual refinement sweep), 2 LDS-column
+ *            with FP64 factors in the mixed forms (their default for other
+ *            N) */
 #define LSD_V_SYN_SCALAR 0
 #define LSD_V_SYN_VECTOR 1
 #define LSD_V_SYN_GRAD   2

@@ -101,7 +101,7 @@ static void release(Ctx* c) {
     void* ptrs[] = {c->d_hl, c->d_hm, c->d_pidx, c->d_apos, c->d_slots, c->d_slot_order, c->d_mslot, c->d_gp, c->d_gw,
                     c->d_sin, c->d_tab, c->d_atab, c->d_r, c->d_dr1, c->d_dr2, c->d_radlap, c->d_r1dr,
                     c->d_intr2dr, c->d_ll, c->d_lu, c->d_piv, c->d_pivpk, c->d_y, c->d_gre, c->d_invg, c->d_info,
-                    c->d_lut32, c->d_xband, c->d_sblk[0], c->d_sblk[1], c->d_sord[0], c->d_sord[1], c->d_tab32,
+                    c->d_lut32, c->d_xband, c->d_sblk[0], c->d_sblk[1], c->d_sblk[2], c->d_sord[0], c->d_sord[1], c->d_sord[2], c->d_tab32,
                     c->d_atab32, c->d_tw32, c->d_slot_t0, c->d_row_h, c->d_atab4, c->d_ana_base, c->d_acol,
                     c->d_spec_in, c->d_spec_out, c->d_fourier, c->d_folded, c->d_phys, c->d_nlphys,
                     c->d_tw, c->d_partial, c->d_cfl_d, c->d_cfl_part, c->d_cfl, c->d_tabb, c->d_atabb,
@@ -168,7 +168,9 @@ static bool variant_ok(const Ctx& c, int which, int value) {
         case LSD_V_SYN_VECTOR:
         case LSD_V_SYN_GRAD: return value == 4 || value == 8 || value == 5 || value == 9;
         case LSD_V_ANA: return value == 1 || ((value == 0 || value == 2) && c.ana_wide_ok);
-        case LSD_V_SOLVE: return (value == 0 && c.solve_direct_ok) || (value == 1 && c.hb_form[1] >= 1);
+        case LSD_V_SOLVE:
+            return (value == 0 && c.solve_direct_ok) || (value == 1 && c.hb_form[1] >= 1) ||
+                   (value == 2 && c.mixed && c.hb_form[2] >= 1);
         default: return false;
     }
 }
@@ -203,7 +205,8 @@ static void load_tuning(Ctx& c) {
 
 // Points the solve kernels at the block set of the selected form.
 static void select_solve_form(Ctx& c) {
-    const int form = (c.var[LSD_V_SOLVE] == 0 && c.solve_direct_ok) ? 0 : 1;
+    const int v = c.var[LSD_V_SOLVE];
+    const int form = (v == 0 && c.solve_direct_ok) ? 0 : (v == 2 && c.hb_form[2] >= 1) ? 2 : 1;
     c.d_sblocks = c.d_sblk[form];
     c.d_sorder = c.d_sord[form];
     c.n_partial_blocks = c.nblk_form[form];
@@ -299,16 +302,17 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
     c->solve_direct_ok = direct;
     c->hb_form[0] = 32;
     c->hb_form[1] = solve_block_harmonics(N, c->mixed);
+    c->hb_form[2] = c->mixed ? solve_block_harmonics(N, false) : 0;
     if (c->hb_form[1] < 1 && !direct) {
         set_error("N too large for the banded-solve kernels");
         return LSD_E_CONFIG;
     }
-    std::vector<int> sorder[2];
-    std::vector<int4> sblocks[2];
+    std::vector<int> sorder[3];
+    std::vector<int4> sblocks[3];
     {
         std::vector<std::vector<int>> byl(L);
         for (int h = 0; h < H; h++) byl[c->hl[h]].push_back(h);
-        for (int form = 0; form < 2; form++) {
+        for (int form = 0; form < 3; form++) {
             const int hb = c->hb_form[form];
             if (hb < 1 || (form == 0 && !direct)) continue;
             for (int l = 0; l < L; l++)
@@ -321,9 +325,11 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
             c->nblk_form[form] = (int)sblocks[form].size();
         }
     }
-    c->var[LSD_V_SOLVE] = direct ? 0 : 1;
+    // without the two-pass kernels the mixed forms solve with FP64 factors
+    // by default: the FP32 factors' refinement sweep costs more than it saves
+    c->var[LSD_V_SOLVE] = direct ? 0 : (c->hb_form[2] >= 1 ? 2 : 1);
     select_solve_form(*c);
-    const int nblk_max = std::max(c->nblk_form[0], c->nblk_form[1]);
+    const int nblk_max = std::max(std::max(c->nblk_form[0], c->nblk_form[1]), c->nblk_form[2]);
     if (dalloc(&c->d_partial, (size_t)8 * nblk_max)) return LSD_E_HIP;
     LSD_HIP(hipHostMalloc((void**)&c->h_partial, sizeof(double) * 8 * nblk_max, hipHostMallocDefault));
     // rotation state; with a conducting inner core its mesh, operators,
@@ -380,7 +386,7 @@ static int create(const lsd_config* cfg, int device, Ctx** out) {
         LSD_HIP(hipMemsetAsync(c->d_icpiv, 0, sizeof(int) * 2 * L * c->NT, st));
         LSD_HIP(hipMemsetAsync(c->d_icy, 0, sizeof(double) * L * c->Nic * 7, st));
     }
-    for (int form = 0; form < 2; form++) {
+    for (int form = 0; form < 3; form++) {
         if (!c->d_sblk[form]) continue;
         LSD_HIP(hipMemcpyAsync(c->d_sord[form], sorder[form].data(), sizeof(int) * H, hipMemcpyHostToDevice, st));
         LSD_HIP(hipMemcpyAsync(c->d_sblk[form], sblocks[form].data(), sizeof(int4) * sblocks[form].size(),

@@ -200,10 +200,12 @@ struct Ctx {
     bool ana_wide_ok = false;          // the wide-load FP32 analysis applies (even Th2)
     bool solve_direct_ok = false;      // the two-pass solves apply (N in {32, 48, 64, 96})
     // solve blocks of the two forms: [0] two-pass (32 harmonics), [1] LDS-column
-    int4* d_sblk[2] = {nullptr, nullptr};
-    int* d_sord[2] = {nullptr, nullptr};
-    int nblk_form[2] = {0, 0};
-    int hb_form[2] = {0, 0};
+    // solve block sets: 0 two-pass, 1 LDS-column, 2 LDS-column with FP64
+    // factors in the mixed forms
+    int4* d_sblk[3] = {nullptr, nullptr, nullptr};
+    int* d_sord[3] = {nullptr, nullptr, nullptr};
+    int nblk_form[3] = {0, 0, 0};
+    int hb_form[3] = {0, 0, 0};
 
     // captured step graphs: [evaluate, predictor, evaluate, corrector] and
     // [evaluate, corrector], each ending with the copy of the norm partials

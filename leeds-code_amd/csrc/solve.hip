@@ -296,7 +296,11 @@ int solve_block_harmonics(int N, bool mixed) {
     return hb;
 }
 
-size_t solve_lds_bytes(const Ctx& c) { return solve_smem_bytes(c.N, 2 * c.solve_hb + 1, c.mixed); }
+// FP32 factors with the FP64-residual sweep: the mixed forms, unless the
+// LDS-column form with FP64 factors (variant 2) is selected
+static bool fp32_factors(const Ctx& c) { return c.mixed && c.var[LSD_V_SOLVE] != 2; }
+
+size_t solve_lds_bytes(const Ctx& c) { return solve_smem_bytes(c.N, 2 * c.solve_hb + 1, fp32_factors(c)); }
 
 static SolveArgs make_args(Ctx& c) {
     SolveArgs a;
@@ -354,7 +358,7 @@ int launch_predictor(Ctx& c) {
     // per field: read state and N, write state and saved N
     ProfScope ps(c, LSD_K_SOLVE, 4.0 * 16.0 * c.coll_elems() * active_fields(c));
     dim3 grid(c.n_partial_blocks, 6);
-    if (c.mixed) hipLaunchKernelGGL(k_predict<true>, grid, dim3(64), solve_lds(c), c.stream, a);
+    if (fp32_factors(c)) hipLaunchKernelGGL(k_predict<true>, grid, dim3(64), solve_lds(c), c.stream, a);
     else hipLaunchKernelGGL(k_predict<false>, grid, dim3(64), solve_lds(c), c.stream, a);
     LSD_HIP(hipGetLastError());
     return c.ic() ? launch_ic_solve(c, true) : LSD_OK;
@@ -374,7 +378,7 @@ int launch_corrector_async(Ctx& c) {
     } else {
         // per field: read state, N and saved N, write state and saved N
         ProfScope ps(c, LSD_K_SOLVE, 5.0 * 16.0 * c.coll_elems() * active_fields(c));
-        if (c.mixed) {
+        if (fp32_factors(c)) {
             hipLaunchKernelGGL(k_correct<true>, grid, dim3(64), solve_lds(c), c.stream, a);
         } else {
             hipLaunchKernelGGL(k_correct<false>, grid, dim3(64), solve_lds(c), c.stream, a);

@@ -53,7 +53,7 @@ def main(argv=None):
     ap.add_argument("--reps", type=int, default=5)
     args = ap.parse_args(argv)
     from .config import make_config, PLAIN_F64, MIXED_F32, MIXED_BF16X3
-    from .gpu import Context, K_SYNTH, K_ANAL, K_SOLVE
+    from .gpu import Context, K_SYNTH, K_ANAL, K_SOLVE, V_SOLVE
     from .initial import seeded_state
 
     def cfg(prec):
@@ -86,6 +86,8 @@ def main(argv=None):
     res = {}
     for prec, name in ((PLAIN_F64, "fp64"), (MIXED_F32, "fp32_refined")):
         g = Context(cfg(prec))
+        if prec == MIXED_F32:
+            g.set_variant(V_SOLVE, 1)  # the FP32 factors (the mixed default at N = 128 is FP64)
         g.set_state(st)
         names = ("uT", "uP", "C", "X", "BT", "BP")
         for k, n in enumerate(names):

@@ -118,8 +118,8 @@ def test_mixed_large_grid_two_steps():
     """The largest grid of the survey's cases (lmax = 255 as L = M = 256 with M
     even, 384 x 768, N = 128) with the physics of the L = M = 170 case: the
     fused physical pass on its general-length path (768 = 8.8.4.3, one
-    thread per phi point of 1024), and the LDS-column solves (N = 128) with
-    FP32 factors and one FP64-residual sweep.
+    thread per phi point of 1024), and the LDS-column solves (N = 128), with
+    FP64 factors (the mixed forms' default there).
 
     Bound for LSD_MIXED_F32: 2e-6 after 2 steps (measured 1.11e-6 in uT,
     every other field below 3e-9).  The uT error is the FP32 rounding of the

@@ -122,6 +122,21 @@ def test_fp32_refined_solves_at_config3():
     gm.close()
 
 
+def test_fp64_lds_column_solves_in_mixed_form():
+    """LSD_V_SOLVE = 2: the LDS-column kernels with FP64 factors in a mixed
+    context, against the two-pass FP64 kernels of the same context (which
+    keep the saved N in FP32): the same corrector counts and 1e-6 after 10
+    steps."""
+    cfg = config1(MIXED_F32)
+    st = seeded_state(cfg, 6, 1e-3)
+    a, ea = run(cfg, st, 10, solve=0)
+    b, eb = run(cfg, st, 10, solve=2)
+    assert ea == eb
+    e = rel(a, b)
+    print("two-pass and FP64 LDS-column solves (mixed form) differ by %.2e after 10 steps" % e, flush=True)
+    assert e < 1e-6
+
+
 def test_variant_checks():
     cfg = config1(MIXED_F32)
     g = Context(cfg)
