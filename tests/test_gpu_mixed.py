@@ -121,8 +121,8 @@ def test_mixed_large_grid_two_steps():
     thread per phi point of 1024), and the LDS-column solves (N = 128), with
     FP64 factors (the mixed forms' default there).
 
-    Bound for LSD_MIXED_F32: 2e-6 after 2 steps (measured 1.11e-6 in uT,
-    every other field below 3e-9).  The uT error is the FP32 rounding of the
+    Bound for LSD_MIXED_F32: 2e-6 after 2 steps (measured 1.32e-6 in uT,
+    every other field below 3e-9), 1e-6 after 100 steps.  The uT error is the FP32 rounding of the
     Lorentz force, which enters with the factor 2 Pm / E = 4e4: it is flat
     across the degrees (about 4e-7 of the uT norm in each band of 32
     degrees), so it grows with the number of harmonics, while 99 % of the
@@ -145,7 +145,6 @@ def test_mixed_large_grid_two_steps():
     assert sp.evaluations == sm.evaluations
     e = field_errors(gm, gp, cfg)
     print("after 2 steps", {k: "%.2e" % v for k, v in e.items()}, flush=True)
-    gm.close()
     for k, v in e.items():
         assert v < (2e-6 if k == "uT" else 1e-6), (k, v)
     cb = cfg_fn(precision=MIXED_F32_MAG64)
@@ -155,10 +154,20 @@ def test_mixed_large_grid_two_steps():
     assert sb.evaluations == sp.evaluations
     e = field_errors(gb, gp, cfg)
     print("FP64 sums of B and curl B, after 2 steps", {k: "%.2e" % v for k, v in e.items()}, flush=True)
-    gb.close()
-    gp.close()
     for k, v in e.items():
         assert v < 1e-6, (k, v)
+    # the 2-step uT excess of the FP32 form is a transient of the seeded
+    # state's high degrees: after 100 steps both forms are within 1e-6
+    gp.step(98)
+    for name, g in (("FP32", gm), ("FP64 magnetic sums", gb)):
+        g.step(98)
+        e = field_errors(g, gp, cfg)
+        print(name, "after 100 steps", {k: "%.2e" % v for k, v in e.items()}, flush=True)
+        for k, v in e.items():
+            assert v < 1e-6, (name, k, v)
+    gb.close()
+    gm.close()
+    gp.close()
 
 
 @pytest.mark.parametrize("prec", [PLAIN_F64, MIXED_F32, MIXED_F32_MAG64])
