@@ -318,7 +318,9 @@ int lsd_sync(lsd_ctx* ctx);
  *  SYN_*     FP32 synthesis: m per workgroup, 4 or 8 (one wave per m, one
  *            hemisphere of one output staged in LDS per pass), or 5, 9:
  *            the same shapes as persistent kernels (the CUs' worth of
- *            workgroups pulling the work items from per-XCD queues);
+ *            workgroups pulling the work items from per-XCD queues), or
+ *            6, 10: persistent, each workgroup issuing its next item's
+ *            loads before the current item's epilogue;
  *            defaults 4, 8, 8
  *  ANA       FP32 analysis: 0 wide-load (even Th/2 only; default), 1 LDS-staged,
  *            2 the wide-load kernel in the persistent form

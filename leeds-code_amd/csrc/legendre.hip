@@ -606,6 +606,213 @@ k_syn4p(Syn4Args a, int* __restrict__ wq, int nq) {
         for (int k = 0; k < 9; k++) atomicExch(&wq[k], 0);
 }
 
+// Persistent form that overlaps each item's epilogue with the next item's
+// first operand loads: a workgroup takes its next item from the queue as
+// soon as its contraction ends, issues that item's ring of loads, and only
+// then stages and stores the finished item.  The item's steps are those of
+// syn4_item, split into begin (decode, first loads), run (contraction) and
+// end (epilogue).
+struct S4Item {
+    bool live;  // a valid block with m0 < M (uniform over the workgroup)
+    bool run;   // this wave's m < M (uniform over the wave)
+    int job, m0, jb, n0, ncl, h0, len, rows;
+    size_t tb0;
+    const float* in0;
+    const float* in1;
+};
+
+template <int KIND>
+__device__ __forceinline__ void syn4q_load(const Syn4Args& a, const S4Item& it, S4Ops& op, int k0, int q4) {
+    const int C2 = 2 * a.N;
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+        const int k = min(k0 + 2 * q4 + p, it.rows - 1);
+        op.ta[p] = *reinterpret_cast<const float4*>(a.tabA + it.tb0 + (size_t)k * a.Th2p);
+        if (KIND != 0) op.tb[p] = *reinterpret_cast<const float4*>(a.tabB + it.tb0 + (size_t)k * a.Th2p);
+        const size_t h = (size_t)min(it.h0 + k, a.H - 1);
+        op.x[p] = *reinterpret_cast<const float2*>(it.in0 + h * C2 + 2 * it.ncl);
+        if (KIND == 1) op.t[p] = *reinterpret_cast<const float2*>(it.in1 + h * C2 + 2 * it.ncl);
+    }
+}
+
+template <int KIND, int MQ>
+__device__ __forceinline__ S4Item syn4q_begin(const Syn4Args& a, int b, S4Ops (&buf)[S4_NB]) {
+    S4Item it;
+    it.live = false;
+    it.run = false;
+    const BlockId bid = decode_block_at(b, a.nmo * a.njob * a.nsb, (S4_MG / MQ) * a.nth);
+    if (!bid.valid) return it;
+    const int mo = bid.group / (a.njob * a.nsb);
+    const int gjs = bid.group - mo * a.njob * a.nsb;
+    const int sb = gjs - (gjs / a.nsb) * a.nsb;
+    it.job = gjs / a.nsb;
+    const int mq = mo * (S4_MG / MQ) + bid.cc / a.nth, tt = bid.cc % a.nth;
+    it.m0 = mq * MQ;
+    if (it.m0 >= a.M) return it;
+    it.live = true;
+    it.jb = tt * S4_TH;
+    it.n0 = sb * S4_SH;
+    it.in0 = pickj(a.in0, it.job);
+    it.in1 = pickj(a.in1, it.job);
+    const int tid = threadIdx.x, wave = uniform(tid >> 6), lane = tid & 63, l16 = lane & 15, q4 = lane >> 4;
+    it.ncl = min(it.n0 + l16, a.N - 1);
+    const int m = it.m0 + wave;
+    if (m < a.M) {
+        const int si = a.mslot[m];
+        const Slot s = a.slots[si];
+        it.h0 = s.h0;
+        it.len = s.len;
+        it.rows = (s.len + 7) & ~7;
+        it.tb0 = (size_t)a.t0[si] * a.Th2p + it.jb + 4 * l16;
+        it.run = true;
+#pragma unroll
+        for (int u = 0; u < S4_NB; u++) {
+            syn4q_load<KIND>(a, it, buf[u], 8 * u, q4);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    return it;
+}
+
+template <int KIND>
+__device__ __forceinline__ void syn4q_step(const S4Ops& op, v4f (&Y)[KIND == 0 ? 1 : 2][2][2][4]) {
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+        const float ta[4] = {op.ta[p].x, op.ta[p].y, op.ta[p].z, op.ta[p].w};
+        const float x[2] = {op.x[p].x, op.x[p].y};
+        if constexpr (KIND == 0) {
+#pragma unroll
+            for (int u = 0; u < 2; u++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) Y[0][p][u][t] = mfma4(x[u], ta[t], Y[0][p][u][t]);
+        } else if constexpr (KIND == 2) {
+            const float tb[4] = {op.tb[p].x, op.tb[p].y, op.tb[p].z, op.tb[p].w};
+            const float xr[2] = {-x[1], x[0]};
+#pragma unroll
+            for (int u = 0; u < 2; u++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    Y[0][1 - p][u][t] = mfma4(x[u], ta[t], Y[0][1 - p][u][t]);
+                    Y[1][p][u][t] = mfma4(xr[u], tb[t], Y[1][p][u][t]);
+                }
+        } else {
+            const float tb[4] = {op.tb[p].x, op.tb[p].y, op.tb[p].z, op.tb[p].w};
+            const float tv[2] = {op.t[p].x, op.t[p].y};
+            const float tr[2] = {tv[1], -tv[0]}, sr[2] = {-x[1], x[0]};
+#pragma unroll
+            for (int u = 0; u < 2; u++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    Y[0][1 - p][u][t] = mfma4(x[u], ta[t], Y[0][1 - p][u][t]);
+                    Y[0][p][u][t] = mfma4(tr[u], tb[t], Y[0][p][u][t]);
+                    Y[1][p][u][t] = mfma4(sr[u], tb[t], Y[1][p][u][t]);
+                    Y[1][1 - p][u][t] = mfma4(tv[u], ta[t], Y[1][1 - p][u][t]);
+                }
+        }
+    }
+}
+
+template <int KIND, int MQ>
+__device__ __forceinline__ void syn4q_end(const Syn4Args& a, const S4Item& it,
+                                          const v4f (&Y)[KIND == 0 ? 1 : 2][2][2][4], float2* __restrict__ St) {
+    constexpr int NOUT = KIND == 0 ? 1 : 2;
+    constexpr int NMIR = S4_NMIR<MQ>, SHS = S4_SHS<MQ>;
+    constexpr int NPASS = NOUT * (2 / NMIR), NQ = MQ / 2;
+    const int tid = threadIdx.x, wave = uniform(tid >> 6), lane = tid & 63, l16 = lane & 15, q4 = lane >> 4;
+#pragma unroll
+    for (int pass = 0; pass < NPASS; pass++) {
+        const int o = pass / (2 / NMIR), mir0 = NMIR == 2 ? 0 : pass % 2;
+        if (pass) __syncthreads();  // the previous pass's reads are done
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int sh = 4 * q4 + r, th = 16 * t + l16;
+                const float y0r = Y[o][0][0][t][r], y1r = Y[o][1][0][t][r];
+                const float y0i = Y[o][0][1][t][r], y1i = Y[o][1][1][t][r];
+#pragma unroll
+                for (int mm = 0; mm < NMIR; mm++)
+                    St[(mm * S4_SH + sh) * SHS + th * S4_MQP<MQ> + wave] =
+                        (mir0 + mm) ? make_float2(y0r - y1r, y0i - y1i) : make_float2(y0r + y1r, y0i + y1i);
+            }
+        __syncthreads();
+        float* out = o == 0 ? pickj(a.out0, it.job) : pickj(a.out1, it.job);
+        for (int idx = tid; idx < NQ * NMIR * S4_SH * S4_TH; idx += 64 * MQ) {
+            const int qq = idx % NQ, item = idx / NQ;
+            const int th = item % S4_TH, rest = item / S4_TH;
+            const int sh = rest % S4_SH, mm = rest / S4_SH, mir = mir0 + mm;
+            const int n = it.n0 + sh, j = it.jb + th;
+            if (n >= a.N || j >= a.Th2) continue;
+            const int jrow = mir ? a.Th - 1 - j : j;
+            if (mir && jrow < a.Th2) continue;
+            const float2* sp = St + (mm * S4_SH + sh) * SHS + th * S4_MQP<MQ> + 2 * qq;
+            float4 v = make_float4(sp[0].x, sp[0].y, sp[1].x, sp[1].y);
+            if constexpr (KIND == 2) {
+                const float f = (float)a.rinv[n];
+                v.x *= f; v.y *= f; v.z *= f; v.w *= f;
+            }
+            reinterpret_cast<float4*>(out + 2 * (((size_t)n * a.Th + jrow) * a.MP + it.m0))[qq] = v;
+        }
+    }
+}
+
+template <int KIND, int MQ>
+__global__ void __launch_bounds__(64 * MQ) __attribute__((amdgpu_waves_per_eu(2)))
+k_syn4q(Syn4Args a, int* __restrict__ wq, int nq) {
+    constexpr int NOUT = KIND == 0 ? 1 : 2;
+    __shared__ float2 St[S4_STAGE<MQ>];
+    __shared__ int s_q;
+    const int xcd = blockIdx.x & 7, q4 = (threadIdx.x & 63) >> 4;
+    // s_q is read by every wave before thread 0 writes it again
+    auto pull = [&]() {
+        if (threadIdx.x == 0) s_q = atomicAdd(&wq[xcd], 1);
+        __syncthreads();
+        const int q = s_q;
+        __syncthreads();
+        return q;
+    };
+    S4Ops buf[S4_NB];
+    int q = pull();
+    S4Item it;
+    it.live = false;
+    it.run = false;
+    if (q < nq) it = syn4q_begin<KIND, MQ>(a, 8 * q + xcd, buf);
+    while (q < nq) {
+        v4f Y[NOUT][2][2][4];
+#pragma unroll
+        for (int o = 0; o < NOUT; o++)
+#pragma unroll
+            for (int p = 0; p < 2; p++)
+#pragma unroll
+                for (int u = 0; u < 2; u++)
+#pragma unroll
+                    for (int t = 0; t < 4; t++) Y[o][p][u][t] = v4f{0, 0, 0, 0};
+        if (it.run) {
+            for (int k0 = 0; k0 < it.len; k0 += 8 * S4_NB) {
+#pragma unroll
+                for (int u = 0; u < S4_NB; u++) {
+                    if (k0 + 8 * u < it.len) syn4q_step<KIND>(buf[u], Y);
+                    syn4q_load<KIND>(a, it, buf[u], k0 + 8 * u + 8 * S4_NB, q4);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+        }
+        // the next item's loads go out before this item's epilogue (the
+        // queue's barriers also order the previous epilogue's LDS reads
+        // before this one's writes)
+        const int qn = pull();
+        S4Item nx;
+        nx.live = false;
+        nx.run = false;
+        if (qn < nq) nx = syn4q_begin<KIND, MQ>(a, 8 * qn + xcd, buf);
+        if (it.live) syn4q_end<KIND, MQ>(a, it, Y, St);
+        it = nx;
+        q = qn;
+    }
+    if (threadIdx.x == 0 && atomicAdd(&wq[8], 1) == (int)gridDim.x - 1)
+        for (int k = 0; k < 9; k++) atomicExch(&wq[k], 0);
+}
+
 static int syn4_launch(Ctx& c, int kind, int njob, const float* tA, const float* tB, const void* const* in0,
                        const void* const* in1, void* const* out0, void* const* out1) {
     Syn4Args a;
@@ -635,7 +842,36 @@ static int syn4_launch(Ctx& c, int kind, int njob, const float* tA, const float*
     a.njob = njob;
     auto grid = [&](int mq) { return dim3(grid_blocks(a.nmo * njob * a.nsb, (S4_MG / mq) * a.nth)); };
     const int v = c.var[LSD_V_SYN_SCALAR + kind];
-    const int mq = (v == 4 || v == 5) ? 4 : 8;
+    const int mq = (v >= 4 && v <= 6) ? 4 : 8;
+    if (v == 6 || v == 10) {
+        // persistent form with the next item's loads issued before each
+        // epilogue
+        const void* fn;
+        if (kind == SYN_SCALAR) fn = mq == 4 ? (const void*)&k_syn4q<0, 4> : (const void*)&k_syn4q<0, 8>;
+        else if (kind == SYN_VECTOR) fn = mq == 4 ? (const void*)&k_syn4q<1, 4> : (const void*)&k_syn4q<1, 8>;
+        else fn = mq == 4 ? (const void*)&k_syn4q<2, 4> : (const void*)&k_syn4q<2, 8>;
+        int& pg = c.syn_qgrid[kind][mq == 8 ? 1 : 0];
+        if (pg == 0) {
+            int per = 0;
+            LSD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, 64 * mq, 0));
+            pg = (std::max(per, 1) * c.ncu + 7) / 8 * 8;
+        }
+        const unsigned nb = grid(mq).x;
+        const dim3 pgrid(std::min((unsigned)pg, nb));
+        const int nq = (int)(nb / 8);
+        if (kind == SYN_SCALAR) {
+            if (mq == 4) hipLaunchKernelGGL((k_syn4q<0, 4>), pgrid, dim3(64 * 4), 0, c.stream, a, c.d_wq, nq);
+            else hipLaunchKernelGGL((k_syn4q<0, 8>), pgrid, dim3(64 * 8), 0, c.stream, a, c.d_wq, nq);
+        } else if (kind == SYN_VECTOR) {
+            if (mq == 4) hipLaunchKernelGGL((k_syn4q<1, 4>), pgrid, dim3(64 * 4), 0, c.stream, a, c.d_wq, nq);
+            else hipLaunchKernelGGL((k_syn4q<1, 8>), pgrid, dim3(64 * 8), 0, c.stream, a, c.d_wq, nq);
+        } else {
+            if (mq == 4) hipLaunchKernelGGL((k_syn4q<2, 4>), pgrid, dim3(64 * 4), 0, c.stream, a, c.d_wq, nq);
+            else hipLaunchKernelGGL((k_syn4q<2, 8>), pgrid, dim3(64 * 8), 0, c.stream, a, c.d_wq, nq);
+        }
+        LSD_HIP(hipGetLastError());
+        return LSD_OK;
+    }
     if (v == 5 || v == 9) {
         // persistent form: the CUs' worth of workgroups (a multiple of 8,
         // so each XCD gets the same number) pulling the static blocks

@@ -166,7 +166,7 @@ static bool variant_ok(const Ctx& c, int which, int value) {
     switch (which) {
         case LSD_V_SYN_SCALAR:
         case LSD_V_SYN_VECTOR:
-        case LSD_V_SYN_GRAD: return value == 4 || value == 8 || value == 5 || value == 9;
+        case LSD_V_SYN_GRAD: return (value >= 4 && value <= 6) || (value >= 8 && value <= 10);
         case LSD_V_ANA: return value == 1 || ((value == 0 || value == 2) && c.ana_wide_ok);
         case LSD_V_SOLVE:
             return (value == 0 && c.solve_direct_ok) || (value == 1 && c.hb_form[1] >= 1) ||
@@ -1094,12 +1094,21 @@ static int tune(Ctx& c, int reps, const char* cache, double* tus) {
                 RC(time_us(c, reps, [&]() { return synthesise(c, true, false); }, &t[k][v]));
             }
             c.var[k] = t[k][0] <= t[k][1] ? 4 : 8;
-            // the persistent form of the faster shape
-            double tp = 0.0;
+            // the two persistent forms of the faster shape (plain, and with
+            // the next item's loads issued before each epilogue)
             const int best = c.var[k];
-            c.var[k] = best + 1;
-            RC(time_us(c, reps, [&]() { return synthesise(c, true, false); }, &tp));
-            if (tp >= std::min(t[k][0], t[k][1])) c.var[k] = best;
+            int choice = best;
+            double tbest = std::min(t[k][0], t[k][1]);
+            for (int d = 1; d <= 2; d++) {
+                double tp = 0.0;
+                c.var[k] = best + d;
+                RC(time_us(c, reps, [&]() { return synthesise(c, true, false); }, &tp));
+                if (tp < tbest) {
+                    tbest = tp;
+                    choice = best + d;
+                }
+            }
+            c.var[k] = choice;
         }
         if (c.ana_wide_ok) {
             for (int v = 0; v < 2; v++) {

@@ -268,6 +268,7 @@ struct Ctx {
     int* d_wq = nullptr;               // [32]: synthesis queues [0, 9), analysis [16, 25)
     int ncu = 0;
     int syn_pgrid[3][2] = {};
+    int syn_qgrid[3][2] = {};
     int ana_pgrid[2] = {};
     bool ic() const { return Nic > 1 && cfg.mag; }
     // the rotating inner core moves the wall: b_rot_tstep with a no-slip

@@ -47,8 +47,9 @@ def test_synthesis_shapes_bit_for_bit():
     assert ev == ev2
     for (a, b), (c, d) in zip(base, other):
         assert np.array_equal(a, c) and np.array_equal(b, d)
-    # the persistent forms (5, 9): the same work items pulled from queues
-    for shapes in ((5, 9, 9), (9, 5, 5)):
+    # the persistent forms (5, 9; 6, 10 with the next item's loads issued
+    # before each epilogue): the same work items pulled from queues
+    for shapes in ((5, 9, 9), (9, 5, 5), (6, 10, 10), (10, 6, 6)):
         pers, ev3 = run(cfg, st, 3, syn0=shapes[0], syn1=shapes[1], syn2=shapes[2])
         assert ev3 == ev
         for (a, b), (c, d) in zip(base, pers):
@@ -65,6 +66,10 @@ def test_persistent_synthesis_at_config3():
     pers, ev2 = run(cfg, st, 3, syn0=5, syn1=9, syn2=9)
     assert ev == ev2
     for (a, b), (c, d) in zip(base, pers):
+        assert np.array_equal(a, c) and np.array_equal(b, d)
+    pref, ev3 = run(cfg, st, 3, syn0=10, syn1=10, syn2=10)
+    assert ev == ev3
+    for (a, b), (c, d) in zip(base, pref):
         assert np.array_equal(a, c) and np.array_equal(b, d)
 
 
@@ -141,7 +146,7 @@ def test_variant_checks():
     cfg = config1(MIXED_F32)
     g = Context(cfg)
     with pytest.raises(LsdError):
-        g.set_variant(V_SYN_SCALAR, 6)
+        g.set_variant(V_SYN_SCALAR, 7)
     with pytest.raises(LsdError):
         g.set_variant(99, 0)
     g.close()
