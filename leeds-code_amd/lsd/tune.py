@@ -29,6 +29,10 @@ def configurations():
                                                                   precision=prec)
     out["case1_plain"] = christensen_case1()
     out["small_mixed"] = config_small(precision=MIXED_F32)
+    # the largest grid of the mixed-form tests (lmax 255, 384 x 768, N = 128)
+    base = config3()
+    out["L256_mixed"] = make_config(L=256, M=256, N=128, Th=384, Ph=768, comp=1, Ra=base.Ra, Ra_comp=base.Ra_comp,
+                                    Pr=1.0, Sc=1.0, Pm=2.0, E=1e-4, dt=2e-6, precision=MIXED_F32)
     return out
 
 
